@@ -1,0 +1,215 @@
+/*
+ * rf_capi.h -- C ABI of librf_hip.so, the gfx950 spectral random-field path.
+ *
+ * This is the boundary the Fortran facade (randomfield_amd/fortran/
+ * randomField_gpu.f90) binds to with iso_c_binding, and that C, C++ and
+ * Python (ctypes) callers use directly.  Plain pointers, sizes and PODs only.
+ *
+ * What each entry point replaces in the reference (cottereau/randomField):
+ *   rf_plan_spectrum / rf_spectrum_tables
+ *       set_kMaxND, set_kPoints, set_kSign, set_SkVec and the amplitude
+ *       multipliers (spectra_RF.f90:15-223, randomFieldND.f90:260-266,409,
+ *       500-501).
+ *   rf_plan_create + rf_plan_execute (rf_generate on host buffers)
+ *       create_RF_Unstruct_Init -> gen_Std_Gauss -> gen_Std_Gauss_{Shinozuka,
+ *       Randomization,Isotropic} (randomFieldND.f90:27-132,184-514), the body
+ *       behind the generic createRandomField (calls_RF.f90:20-22).
+ *   rf_marginal_device
+ *       multiVariateTransformation (randomFieldND.f90:139-178).
+ *   rf_blend_device
+ *       generateUnityPartition_Matrix + add_RF_to_Group + the boundary
+ *       correction of build_random_field (localization_RF.f90:199-263,
+ *       705-857; calls_RF.f90:209-369).
+ *   rf_stats_device / rf_normalize_device
+ *       calculate_average_and_stdVar_MPI (statistics_RF.f90:18-174) and
+ *       normalize_randField (randomFieldND.f90:820-926).
+ *
+ * Conventions
+ *   - Layouts are the reference's column-major ones: points x[i*nDim + d]
+ *     (nDim x N), fields out[n*N + i] (N x Nmc, point fastest).
+ *   - Every call returns RF_OK (0) or a negative RF_E_* status; the message
+ *     of the last failure on a context is rf_last_error(ctx).  The reference
+ *     stops or calls MPI_ABORT instead (randomFieldND.f90:162-166,
+ *     math_RF.f90:318-331).
+ *   - "Uniforms" are raw U[0,1) draws in the reference's draw order
+ *     (random_number after random_seed(PUT=seed)); the library turns them into
+ *     phases 2*pi*U with the reference's pi literal.  A NULL draw pointer
+ *     selects the library's device generator (Philox4x32-10 keyed by
+ *     rf_params.seed), a documented stream that differs from the compiler's.
+ *   - One context per host thread; calls on a context are ordered on its
+ *     stream.  Device-pointer entry points take a hipStream_t as void*
+ *     (NULL = the context's stream).
+ */
+#ifndef RF_CAPI_H
+#define RF_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RF_CAPI_VERSION 1
+
+/* status codes */
+#define RF_OK               0
+#define RF_E_ARG          (-1)   /* invalid argument / shape mismatch        */
+#define RF_E_LOGNORMAL    (-2)   /* lognormal marginal with fieldAvg <= 0    */
+#define RF_E_NOMEM        (-3)   /* device or host allocation failed         */
+#define RF_E_HIP          (-4)   /* HIP runtime error                        */
+#define RF_E_UNSUPPORTED  (-5)   /* combination not implemented              */
+#define RF_E_NODEVICE     (-6)   /* no usable gfx950 device                  */
+
+/* methods: constants_RF.f90:12-15 (FFT = 4 is not provided) */
+#define RF_ISOTROPIC      1
+#define RF_SHINOZUKA      2
+#define RF_RANDOMIZATION  3
+/* correlation models: constants_RF.f90:17-18 */
+#define RF_CM_GAUSSIAN    1
+#define RF_CM_COS         2
+/* first-order marginals: constants_RF.f90:20-21 (0 = leave standard) */
+#define RF_FOM_NONE       0
+#define RF_FOM_GAUSSIAN   1
+#define RF_FOM_LOGNORMAL  2
+/* arithmetic modes */
+#define RF_REF_COMPAT     0   /* reference behaviour wherever it is defined */
+#define RF_FIXED          1   /* intended mathematics                        */
+/* precision forms of the spectral sum */
+#define RF_FORM_FP64        0 /* FP64 direct sum, reference operation order  */
+#define RF_FORM_FP32        1 /* FP64 k.x mod 2pi, FP32 cos, FP32 accumulate */
+#define RF_FORM_MFMA_F16    2 /* [cos Kx|sin Kx].[A cos;-A sin] FP16 hi/lo   */
+#define RF_FORM_MFMA_BF16   3 /* same with BF16 hi/lo                         */
+/* wavevector sampling for RF_RANDOMIZATION */
+#define RF_KSAMP_BOX      0   /* reference: U[0,kMax]^d, lattice-rule count  */
+#define RF_KSAMP_DENSITY  1   /* k ~ N(0, 2 pi I) (normalised), N_k given    */
+/* output element types */
+#define RF_DTYPE_F64      0
+#define RF_DTYPE_F32      1
+
+typedef struct rf_params {
+    int32_t nDim;        /* 1..3 (isotropic: 2..3)                            */
+    int32_t method;      /* RF_ISOTROPIC / RF_SHINOZUKA / RF_RANDOMIZATION     */
+    int32_t corrMod;     /* RF_CM_*                                            */
+    int32_t mode;        /* RF_REF_COMPAT / RF_FIXED                           */
+    int32_t form;        /* RF_FORM_*                                          */
+    int32_t kSampling;   /* RF_KSAMP_* (method 3 only)                         */
+    int32_t Nmc;         /* realizations                                       */
+    int32_t margiFirst;  /* RF_FOM_*: marginal map fused into generation       */
+    int64_t nKDensity;   /* N_k for RF_KSAMP_DENSITY                            */
+    int64_t xSlab;       /* REF_COMPAT point-slab size; 0 = reference rule      */
+    double  corrL[3];    /* correlation lengths (> 0)                          */
+    double  xRange[3];   /* extent of the generation box, caller units         */
+    double  fieldAvg;    /* marginal mean                                      */
+    double  fieldVar;    /* marginal variance                                  */
+    uint64_t seed;       /* device generator key when no uniforms are given    */
+    int32_t isoShared;   /* isotropic: one direction set for all realizations  */
+    int32_t reserved[7]; /* zero                                               */
+} rf_params;
+
+typedef struct rf_spectrum_info {
+    int64_t kNTotal;        /* wavevectors in the table                        */
+    int64_t kN_unsigned;    /* points of one sign block                        */
+    int64_t nKUniform;      /* uniforms placing k (method 3), drawn first      */
+    int64_t nPhaseUniform;  /* uniforms per realization                        */
+    int32_t kNStep[3];
+    int32_t nSignBlocks;
+    int32_t nVarIso;        /* isotropic draw arrays per realization           */
+    int32_t pad;
+    double  kMax[3];
+    double  kDelta[3];
+    double  ampMult;        /* sum multiplier (isotropic: overall scale)       */
+    double  isoStep;        /* isotropic radial step                           */
+} rf_spectrum_info;
+
+typedef struct rf_context rf_context;
+typedef struct rf_field_plan rf_field_plan;
+
+/* ---- context ------------------------------------------------------------ */
+int         rf_context_create(rf_context** ctx, int device);
+int         rf_context_destroy(rf_context* ctx);
+const char* rf_last_error(const rf_context* ctx);   /* ctx may be NULL       */
+int         rf_version(void);
+int         rf_context_sync(rf_context* ctx);
+
+/* ---- host-side spectrum (no GPU) ------------------------------------------ */
+int rf_plan_spectrum(const rf_params* p, rf_spectrum_info* info);
+/* kPoints: nDim x kNTotal (isotropic: 1 x kNTotal radii); Sk: kNTotal.
+ * kUniform: info.nKUniform draws (method 3) or NULL otherwise. */
+int rf_spectrum_tables(const rf_params* p, const double* kUniform,
+                       double* kPoints, double* Sk);
+
+/* ---- generation ----------------------------------------------------------- */
+/* Builds the device-resident wavevector table and per-realization
+ * coefficients.  All pointers are HOST pointers:
+ *   kUniform     info.nKUniform draws (method 3), NULL = device generator
+ *   phaseUniform Nmc x info.nPhaseUniform draws, realization-major, NULL =
+ *                device generator
+ *   calculate    Nmc flags (0 = leave the realization zero), NULL = all.  */
+int rf_plan_create(rf_context* ctx, const rf_params* p,
+                   const double* kUniform, const double* phaseUniform,
+                   const uint8_t* calculate, rf_field_plan** plan);
+int rf_plan_destroy(rf_field_plan* plan);
+int rf_plan_info(const rf_field_plan* plan, rf_spectrum_info* info);
+/* The hot path.  d_x: device nDim x N FP64 points in caller units;
+ * d_out: device N x Nmc in outDtype.  Stream-ordered, asynchronous. */
+int rf_plan_execute(rf_field_plan* plan, const double* d_x, int64_t N,
+                    void* d_out, int32_t outDtype, void* stream);
+/* Host-buffer convenience: create, upload, execute, download, destroy.
+ * x: nDim x N, out: N x Nmc (outDtype). */
+int rf_generate(rf_context* ctx, const rf_params* p, const double* x,
+                int64_t N, const double* kUniform, const double* phaseUniform,
+                const uint8_t* calculate, void* out, int32_t outDtype);
+
+/* ---- post-processing on device buffers ------------------------------------ */
+/* multiVariateTransformation in place over count values of dtype. */
+int rf_marginal_device(rf_context* ctx, int32_t margiFirst, double fieldAvg,
+                       double fieldVar, void* d_field, int64_t count,
+                       int32_t dtype, void* stream);
+
+/* multiVariateTransformation of count FP64 values in a HOST buffer (uploaded,
+ * transformed on the device, downloaded; synchronous). */
+int rf_marginal_host(rf_context* ctx, int32_t margiFirst, double fieldAvg,
+                     double fieldVar, double* field, int64_t count);
+
+/* Overlap blend of nSub structured subdomain fields into a group grid.
+ *   groupN[nDim]            group grid points per axis (x fastest)
+ *   origins[nSub*nDim]      first group index of each subdomain
+ *   sizes[nSub*nDim]        points per axis of each subdomain
+ *   nOvl[nDim]              overlap points (localization_RF.f90:746)
+ *   faceMasks[nSub]         bit 2*axis+face: taper that face (0 lower, 1 up)
+ *   d_fields[nSub]          device pointers, sizes[s] product x Nmc
+ *   d_group                 device, prod(groupN) x Nmc, overwritten
+ * REF_COMPAT: linear reference weights then column 1 divided by the reverse
+ * taper of the group grid; FIXED: square-root exact partition of unity. */
+int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN,
+                    int32_t Nmc, int32_t nSub, const int32_t* origins,
+                    const int32_t* sizes, const int32_t* nOvl,
+                    const int32_t* faceMasks, int32_t mode,
+                    const void* const* d_fields, void* d_group,
+                    int32_t dtype, void* stream);
+
+/* Per-realization FP64 mean/std (biased) and global mean/std; host outputs
+ * avg[Nmc], std[Nmc], g[2] (synchronous). */
+int rf_stats_device(rf_context* ctx, const void* d_field, int64_t N,
+                    int32_t Nmc, int32_t dtype, double* avg, double* std,
+                    double* g);
+/* Subtract each realization's mean and divide by its RMS (FP64 sums). */
+int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N,
+                        int32_t Nmc, int32_t dtype, void* stream);
+
+/* Device generator: count uniforms U[0,1) of stream (seed, tag) into d_out
+ * (FP64), element e = Philox4x32-10(counter=(e lo, e hi, tag, 0),
+ * key=(seed lo, seed hi)), 53-bit mantissa from words 0..1. */
+int rf_device_uniforms(rf_context* ctx, uint64_t seed, uint32_t tag,
+                       int64_t count, double* d_out, void* stream);
+/* Host mirror of that stream (element e), and the raw generator. */
+double rf_uniform_host(uint64_t seed, uint32_t tag, uint64_t e);
+void   rf_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2],
+                             uint32_t out[4]);
+/* Streams of the device generator: wavevector draws use tag 1, phase draws
+ * tag 2 (element = realization * nPhaseUniform + index). */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

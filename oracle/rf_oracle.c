@@ -1,0 +1,521 @@
+/*
+ * rf_oracle.c -- FP64 CPU restatement of randomField's spectral path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the GPU product (see rf_oracle.h).
+ * Compiled with -ffp-contract=off so every a*b+c rounds twice, as the
+ * reference does at the -O0 its makefiles ship (makefileIGLOO:30).
+ *
+ * Citations are file:line in the reference (cottereau/randomField).
+ */
+#include "rf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- spectra */
+
+/* spectra_RF.f90:15-44 -- 99 % spectral-area cut, gaussian model only */
+void rfo_kmax(int corrMod, int nDim, double* kMax)
+{
+    double v = 0.0;
+    if (corrMod == RFO_CM_GAUSSIAN || corrMod == RFO_CM_COS) {
+        /* the reference only sets kMax for cm_GAUSSIAN; cm_COS reuses it */
+        if (nDim == 1) v = 6.457;
+        else if (nDim == 2) v = 7.035;
+        else v = 7.355;
+    }
+    for (int i = 0; i < nDim; i++) kMax[i] = v;
+}
+
+/* math_RF.f90:380-391 */
+static int cyclic_mod(int pos, int base)
+{
+    int r = pos % base;
+    return r == 0 ? base : r;
+}
+
+/* spectra_RF.f90:267-286 -- kSign(j,i), j = 1..2^(nDim-1), i = 1..nDim */
+void rfo_ksign(int nDim, double* kSign)
+{
+    int rows = 1 << (nDim - 1);
+    for (int i = 0; i < nDim; i++)
+        for (int j = 0; j < rows; j++) kSign[i * rows + j] = 1.0;
+    for (int i = 2; i <= nDim; i++) {
+        for (int j = 1; j <= rows; j++) {
+            int seedStep = 1 << (nDim - i);
+            /* int((j-0.9)/seedStep): Fortran int() truncates toward zero */
+            int pos = cyclic_mod((int)(((double)j - 0.9) / (double)seedStep), 2);
+            if (pos % 2 == 1) kSign[(i - 1) * rows + (j - 1)] = -1.0;
+        }
+    }
+}
+
+int rfo_make_plan(int nDim, int method, int corrMod, int mode, int ksamp,
+                  int64_t nKDensity, const double* xRangeNorm, rfo_plan* p)
+{
+    memset(p, 0, sizeof(*p));
+    if (nDim < 1 || nDim > 3) return -1;
+    p->nDim = nDim; p->method = method; p->corrMod = corrMod;
+    p->mode = mode; p->ksamp = ksamp;
+    rfo_kmax(corrMod, nDim, p->kMax);
+    if (method == RFO_ISOTROPIC) {
+        /* spectra_RF.f90:76-79 */
+        if (nDim < 2) return -1;
+        double s2 = 0.0;
+        for (int i = 0; i < nDim; i++) s2 += xRangeNorm[i] * xRangeNorm[i];
+        double kd = 2.0 * RFO_PI / (1.1 * sqrt(s2));
+        double kmax = p->kMax[0];
+        for (int i = 1; i < nDim; i++) if (p->kMax[i] > kmax) kmax = p->kMax[i];
+        int nst = 1 + (int)(5.0 * (double)(int64_t)ceil(kmax / kd));
+        p->kNStep[0] = nst;
+        p->kDelta[0] = kmax / (double)(nst - 1);
+        p->kNTotal = nst;
+        p->kN_unsigned = nst;
+        p->nSignBlocks = 1;
+        /* randomFieldND.f90:409 (intended table) */
+        p->isoStep = p->kDelta[0];
+        /* randomFieldND.f90:500-501; FIXED rescales to unit variance:
+         * 3-D by 2*pi, 2-D by sqrt(4*pi) (closed form of the isotropic sum) */
+        double tp = 2.0 * RFO_PI;
+        double tpd = (nDim == 2) ? tp * tp : tp * tp * tp;
+        p->ampMult = sqrt(p->isoStep / tpd);
+        if (mode == RFO_FIXED) p->ampMult *= (nDim == 3) ? 2.0 * RFO_PI : sqrt(4.0 * RFO_PI);
+        return 0;
+    }
+    if (method != RFO_SHINOZUKA && method != RFO_RANDOMIZATION) return -1;
+    if (method == RFO_RANDOMIZATION && ksamp == RFO_KDENSITY) {
+        if (nKDensity < 1) return -1;
+        p->kNTotal = nKDensity;
+        p->kN_unsigned = nKDensity;
+        p->nSignBlocks = 1;
+        p->nKUniform = 2 * (int64_t)nDim * nKDensity;
+        p->ampMult = sqrt(2.0 / (double)nKDensity);
+        return 0;
+    }
+    /* spectra_RF.f90:88-101 */
+    int64_t kNu = 1;
+    for (int i = 0; i < nDim; i++) {
+        double kd = 2.0 * RFO_PI / (1.1 * xRangeNorm[i]);
+        int nst = 1 + (int)(1.0 * (double)(int64_t)ceil(p->kMax[i] / kd));
+        p->kNStep[i] = nst;
+        p->kDelta[i] = p->kMax[i] / (double)(nst - 1);
+        kNu *= nst;
+    }
+    p->nSignBlocks = 1 << (nDim - 1);
+    p->kN_unsigned = kNu;
+    p->kNTotal = kNu * p->nSignBlocks;
+    double tpd = pow(2.0 * RFO_PI, (double)nDim);
+    if (method == RFO_RANDOMIZATION) {
+        p->nKUniform = (int64_t)nDim * kNu;
+        /* randomFieldND.f90:262 */
+        p->ampMult = 2.0 * sqrt(1.0 / ((double)p->kNTotal * tpd));
+        if (mode == RFO_FIXED) {
+            /* uniform draws over the half-space box of volume
+             * 2^(d-1) prod(kMax): the amplitude needs that volume */
+            double vol = (double)p->nSignBlocks;
+            for (int i = 0; i < nDim; i++) vol *= p->kMax[i];
+            p->ampMult = 2.0 * sqrt(vol / ((double)p->kNTotal * tpd));
+        }
+    } else {
+        /* randomFieldND.f90:265 */
+        double prod = 1.0;
+        for (int i = 0; i < nDim; i++) prod *= p->kDelta[i];
+        p->ampMult = 2.0 * sqrt(prod / tpd);
+    }
+    return 0;
+}
+
+void rfo_kpoints(const rfo_plan* p, const double* kUnif, double* kPts)
+{
+    const int d = p->nDim;
+    if (p->method == RFO_ISOTROPIC) {
+        /* intended get_Permutation(i, kMax, kNStep, snapExtremes) value
+         * (math_RF.f90:137-145): ((i - 0.5) - 0.5) * kMax / (kNStep - 1) */
+        double kmax = p->kMax[0];
+        for (int i = 1; i < d; i++) if (p->kMax[i] > kmax) kmax = p->kMax[i];
+        (void)kmax;
+        for (int64_t j = 0; j < p->kNTotal; j++)
+            kPts[j] = ((double)(j + 1) - 0.5 - 1.0 / 2.0) * p->kMax[0]
+                      / ((double)p->kNStep[0] - 1.0);
+        return;
+    }
+    if (p->method == RFO_RANDOMIZATION && p->ksamp == RFO_KDENSITY) {
+        /* build-defined: Box-Muller on uniform pairs, std sqrt(2*pi) in
+         * coordinates normalised by corrL (S(k) = exp(-|k|^2/(4 pi))) */
+        const double sd = sqrt(2.0 * RFO_PI);
+        for (int64_t c = 0; c < d * p->kNTotal; c++) {
+            double u1 = kUnif[2 * c], u2 = kUnif[2 * c + 1];
+            double z = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * RFO_PI * u2);
+            kPts[c] = sd * z;
+        }
+        return;
+    }
+    const int64_t kNu = p->kN_unsigned;
+    double* base = (double*)malloc(sizeof(double) * (size_t)(d * kNu));
+    if (p->method == RFO_SHINOZUKA) {
+        /* setGrid(kPoints_base, 0, kDelta, kNStep) math_RF.f90:70-88:
+         * dimension 1 fastest, value 0 + kDelta(d)*(i-1) */
+        for (int64_t j = 0; j < kNu; j++) {
+            int64_t rem = j;
+            for (int dd = 0; dd < d; dd++) {
+                int64_t idx = rem % p->kNStep[dd];
+                rem /= p->kNStep[dd];
+                base[j * d + dd] = 0.0 + p->kDelta[dd] * (double)idx;
+            }
+        }
+    } else {
+        /* spectra_RF.f90:120-123: random_number(kPoints_base) fills the
+         * d x kN_unsigned array column-major, then scales row i by kMax(i) */
+        for (int64_t j = 0; j < kNu; j++)
+            for (int dd = 0; dd < d; dd++)
+                base[j * d + dd] = kUnif[j * d + dd] * p->kMax[dd];
+    }
+    double kSign[3 * 4];
+    rfo_ksign(d, kSign);
+    const int rows = p->nSignBlocks;
+    /* spectra_RF.f90:127-133: blocks 2..rows are written; block 1 is not
+     * (REF_COMPAT reads it as zero, FIXED writes the base lattice) */
+    for (int b = 0; b < rows; b++) {
+        for (int64_t j = 0; j < kNu; j++) {
+            for (int dd = 0; dd < d; dd++) {
+                double v;
+                if (b == 0 && p->mode == RFO_REF_COMPAT) v = 0.0;
+                else v = base[j * d + dd] * kSign[dd * rows + b];
+                kPts[(b * kNu + j) * d + dd] = v;
+            }
+        }
+    }
+    free(base);
+}
+
+/* spectra_RF.f90:200-221 with corrL = 1 after normalisation */
+void rfo_skvec(const rfo_plan* p, const double* kPts, double* Sk)
+{
+    const int d = p->nDim;
+    if (p->method == RFO_RANDOMIZATION && p->ksamp == RFO_KDENSITY) {
+        /* density sampling: the spectrum is in the sampling, amplitude flat */
+        for (int64_t j = 0; j < p->kNTotal; j++) Sk[j] = 1.0;
+        return;
+    }
+    if (p->corrMod == RFO_CM_COS) {
+        for (int64_t j = 0; j < p->kNTotal; j++) Sk[j] = 0.0;
+        if (p->kNTotal > 6) Sk[5] = 1.0;
+        return;
+    }
+    if (p->method == RFO_ISOTROPIC) {
+        /* intended: S(|k|) = exp(-|k|^2/(4 pi)) on the radial table (the
+         * reference loops nDim rows of a 1-row table, spectra_RF.f90:206) */
+        for (int64_t j = 0; j < p->kNTotal; j++) {
+            double k = kPts[j];
+            Sk[j] = 1.0 * 1.0 * exp(-((k * k) * 1.0) / (4.0 * RFO_PI));
+        }
+        return;
+    }
+    for (int64_t j = 0; j < p->kNTotal; j++) {
+        double s = 1.0;
+        for (int i = 0; i < d; i++) {
+            double k = kPts[j * d + i];
+            s = s * 1.0 * exp(-((k * k) * 1.0) / (4.0 * RFO_PI));
+        }
+        Sk[j] = s;
+    }
+}
+
+/* ------------------------------------------------------- Shinozuka sums */
+
+void rfo_shinozuka(const rfo_plan* p, const double* x, int64_t N,
+                   const double* kPts, const double* Sk, int Nmc,
+                   const double* phaseUnif, const uint8_t* calc,
+                   int64_t xSlab, double* out, int threads)
+{
+    const int d = p->nDim;
+    const int64_t kN = p->kNTotal;
+    /* randomFieldND.f90:237-239 */
+    if (xSlab <= 0) xSlab = (int64_t)((double)d * 1e8 / (double)kN);
+    if (xSlab > N) xSlab = N;
+    if (xSlab < 1) xSlab = 1;
+    int64_t xNSlab = (N + xSlab - 1) / xSlab;
+
+    /* SkMat = sqrt(SkMat) is applied once per processed (n, m) pair
+     * (randomFieldND.f90:312); REF_COMPAT keeps the running power, FIXED
+     * uses sqrt(Sk) throughout.  amp[c] holds the table for the c-th pass. */
+    double* amp = (double*)malloc(sizeof(double) * (size_t)kN);
+    double* phi = (double*)malloc(sizeof(double) * (size_t)kN);
+    double* skm = (double*)malloc(sizeof(double) * (size_t)kN);
+    for (int64_t j = 0; j < kN; j++) skm[j] = Sk[j];
+    for (int64_t i = 0; i < N * Nmc; i++) out[i] = 0.0;    /* :274 */
+#ifdef _OPENMP
+    int nthr = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    for (int n = 0; n < Nmc; n++) {
+        /* :278-279 phiK = (2*pi) * U */
+        for (int64_t j = 0; j < kN; j++) phi[j] = 2.0 * RFO_PI * phaseUnif[(int64_t)n * kN + j];
+        for (int64_t m = 0; m < xNSlab; m++) {
+            int64_t xs = m * xSlab, xe = xs + xSlab;
+            if (xe > N) xe = N;
+            if (calc && !calc[n]) continue;           /* :295 */
+            for (int64_t j = 0; j < kN; j++) skm[j] = sqrt(skm[j]);   /* :312 */
+            const double* a = (p->mode == RFO_REF_COMPAT) ? skm : amp;
+            if (p->mode != RFO_REF_COMPAT)
+                for (int64_t j = 0; j < kN; j++) amp[j] = sqrt(Sk[j]);
+            double* col = out + (int64_t)n * N;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthr)
+#endif
+            for (int64_t i = xs; i < xe; i++) {
+                const double* xi = x + i * d;
+                double acc = 0.0;                      /* dgemm.f90:219-225 */
+                for (int64_t j = 0; j < kN; j++) {
+                    /* dgemm.f90:246-253: TEMP = 0 + sum_l x(l)*k(l), then
+                     * C = 1*TEMP + 1*phi */
+                    const double* kj = kPts + j * d;
+                    double t = 0.0;
+                    for (int l = 0; l < d; l++) t = t + xi[l] * kj[l];
+                    double arg = 1.0 * t + 1.0 * phi[j];
+                    double c = cos(arg);                /* :310 */
+                    /* dgemm.f90:226-234: skip B(l,j) == 0, C += TEMP*A */
+                    if (a[j] != 0.0) acc = acc + (1.0 * a[j]) * c;
+                }
+                col[i] = acc;
+            }
+            if (xe == N) break;                        /* :322 */
+        }
+    }
+    /* :330 */
+    for (int64_t i = 0; i < N * Nmc; i++) out[i] = p->ampMult * out[i];
+    free(amp); free(phi); free(skm);
+}
+
+/* ------------------------------------------------------------ isotropic */
+
+int rfo_iso_nvar(int nDim) { return nDim == 3 ? 4 : 3; }
+
+void rfo_isotropic(const rfo_plan* p, const double* x, int64_t N,
+                   const double* kRad, const double* Sk, int Nmc,
+                   const double* unif, const uint8_t* calc, double* out,
+                   int threads)
+{
+    const int d = p->nDim;
+    const int64_t kN = p->kNTotal;
+    const int nv = rfo_iso_nvar(d);
+    double* rv = (double*)malloc(sizeof(double) * (size_t)(3 * kN));
+    double* am = (double*)malloc(sizeof(double) * (size_t)kN);
+    double* ps = (double*)malloc(sizeof(double) * (size_t)kN);
+#ifdef _OPENMP
+    int nthr = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    /* sqrt(12.0) is a default-real (single precision) literal in the
+     * reference (randomFieldND.f90:468) */
+    const double sq12 = (p->mode == RFO_REF_COMPAT) ? (double)sqrtf(12.0f) : sqrt(12.0);
+    for (int n = 0; n < Nmc; n++) {
+        double* col = out + (int64_t)n * N;
+        for (int64_t i = 0; i < N; i++) col[i] = 0.0;
+        if (calc && !calc[n]) continue;                /* :444-446, 487-489 */
+        const double* U = unif + (int64_t)n * nv * kN;
+        for (int64_t j = 0; j < kN; j++) {
+            double k = kRad[j];
+            if (d == 3) {
+                /* :460-468: draws phi, theta, gamma, psi */
+                double uphi = U[0 * kN + j], uth = U[1 * kN + j];
+                double ugam = U[2 * kN + j], upsi = U[3 * kN + j];
+                double psi = 2.0 * RFO_PI * upsi;
+                double th = (p->mode == RFO_REF_COMPAT) ? 2.0 * RFO_PI * psi
+                                                         : 2.0 * RFO_PI * uth;
+                double ph = RFO_PI * uphi;
+                double gam = sq12 * (ugam - 0.5);
+                /* :474-476 */
+                rv[3 * j + 0] = cos(th) * sin(ph) * k;
+                rv[3 * j + 1] = sin(th) * sin(ph) * k;
+                rv[3 * j + 2] = cos(ph) * k;
+                /* :480 */
+                am[j] = sqrt(Sk[j] * sin(ph) * (k * k)) * gam;
+                ps[j] = psi;
+            } else {
+                /* :424-433: draws psi, theta, gamma */
+                double upsi = U[0 * kN + j], uth = U[1 * kN + j];
+                double ugam = U[2 * kN + j];
+                double psi = 2.0 * RFO_PI * upsi;
+                double th = (p->mode == RFO_REF_COMPAT) ? 2.0 * RFO_PI * psi
+                                                         : 2.0 * RFO_PI * uth;
+                double gam = (p->mode == RFO_REF_COMPAT) ? 2.0 * RFO_PI * ugam
+                                                          : sq12 * (ugam - 0.5);
+                rv[3 * j + 0] = cos(th) * k;
+                rv[3 * j + 1] = sin(th) * k;
+                rv[3 * j + 2] = 0.0;
+                am[j] = sqrt(Sk[j] * k) * gam;
+                ps[j] = psi;
+            }
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthr)
+#endif
+        for (int64_t i = 0; i < N; i++) {
+            const double* xi = x + i * d;
+            double acc = 0.0;
+            for (int64_t j = 0; j < kN; j++) {
+                /* DGEMM T,N with beta = 0: C = 1*(0 + sum x*r) */
+                double t = 0.0;
+                for (int l = 0; l < d; l++) t = t + xi[l] * rv[3 * j + l];
+                acc = am[j] * cos(1.0 * t + ps[j]) + acc;
+            }
+            col[i] = acc;
+        }
+    }
+    /* :500-501 */
+    for (int64_t i = 0; i < N * Nmc; i++) out[i] = p->ampMult * out[i];
+    free(rv); free(am); free(ps);
+}
+
+/* --------------------------------------------------------- marginal map */
+
+int rfo_marginal(int margiFirst, double fieldAvg, double fieldVar,
+                 double* field, int64_t count)
+{
+    double nVar = 1.0, nAvg = 0.0;
+    if (margiFirst == RFO_FOM_GAUSSIAN) {
+        nVar = fieldVar; nAvg = fieldAvg;
+    } else if (margiFirst == RFO_FOM_LOGNORMAL) {
+        if (fieldAvg <= 0.0) return -2;                 /* :162-166 */
+        nVar = log(1.0 + fieldVar / (fieldAvg * fieldAvg));
+        nAvg = log(fieldAvg) - nVar / 2.0;
+    }
+    double s = sqrt(nVar);
+    for (int64_t i = 0; i < count; i++) field[i] = field[i] * s + nAvg;   /* :171 */
+    if (margiFirst == RFO_FOM_LOGNORMAL)
+        for (int64_t i = 0; i < count; i++) field[i] = exp(field[i]);      /* :175 */
+    return 0;
+}
+
+/* --------------------------------------------------- partition of unity */
+
+/* localization_RF.f90:746 */
+void rfo_overlap_npoints(int nDim, const double* overlap, const double* corrL,
+                         const double* xStep, int32_t* nOvl)
+{
+    for (int i = 0; i < nDim; i++)
+        nOvl[i] = (int32_t)lround((overlap[i] * corrL[i] - 2.0 * xStep[i]) / xStep[i]) + 1;
+}
+
+/* weight of tapered face position s = 1..n (s = 1 nearest the interior edge
+ * of the overlap on an upper face; mirrored on a lower face).
+ * REF_COMPAT: localization_RF.f90:800-813, v = s/(n+2), w = (1+cos(pi v))/2.
+ * FIXED: v = s/(n+1) so that the two faces of an overlap sum to one, then
+ * the square root so that the squared weights sum to one. */
+static double face_weight(int s, int n, int upper, int mode)
+{
+    double v;
+    if (mode == RFO_REF_COMPAT) {
+        int q = upper ? s : (n + 1 - s);
+        v = fabs((double)q / (double)(n + 2));
+        return (1.0 + cos(RFO_PI * v)) / 2.0;
+    }
+    int q = upper ? s : (n + 1 - s);
+    v = (double)q / (double)(n + 1);
+    return sqrt((1.0 + cos(RFO_PI * v)) / 2.0);
+}
+
+void rfo_unity_partition(int nDim, const int32_t* xNStep, const int32_t* nOvl,
+                         int faceMask, int mode, double* pu)
+{
+    int64_t tot = 1;
+    for (int i = 0; i < nDim; i++) tot *= xNStep[i];
+    for (int64_t p = 0; p < tot; p++) pu[p] = 1.0;     /* :748 */
+    /* loop order j (face) outer, i (axis) inner: :755, :777 */
+    for (int f = 0; f < 2; f++) {
+        for (int i = 0; i < nDim; i++) {
+            int n = nOvl[i];
+            if (n < 1) continue;                          /* :779 */
+            if (!(faceMask & (1 << (2 * i + f)))) continue;
+            int dlim = (f == 0) ? 1 : xNStep[i] - n + 1;  /* :760-768 */
+            for (int64_t p = 0; p < tot; p++) {
+                int64_t rem = p, idx = 0;
+                for (int a = 0; a <= i; a++) { idx = rem % xNStep[a]; rem /= xNStep[a]; }
+                int s = (int)(idx + 1) - dlim + 1;
+                if (s >= 1 && s <= n) pu[p] = pu[p] * face_weight(s, n, f == 1, mode);
+            }
+        }
+    }
+}
+
+void rfo_blend(int nDim, const int32_t* groupN, int Nmc, int nSubdomains,
+               const int32_t* origins, const int32_t* sizes,
+               const int32_t* nOvl, const int32_t* faceMasks, int mode,
+               const double* const* fields, double* group)
+{
+    int64_t gTot = 1;
+    for (int i = 0; i < nDim; i++) gTot *= groupN[i];
+    for (int64_t p = 0; p < gTot * Nmc; p++) group[p] = 0.0;   /* calls_RF.f90:202 */
+    for (int s = 0; s < nSubdomains; s++) {
+        const int32_t* sz = sizes + s * nDim;
+        const int32_t* org = origins + s * nDim;
+        int64_t sTot = 1;
+        for (int i = 0; i < nDim; i++) sTot *= sz[i];
+        double* pu = (double*)malloc(sizeof(double) * (size_t)sTot);
+        rfo_unity_partition(nDim, sz, nOvl, faceMasks[s], mode, pu);
+        for (int n = 0; n < Nmc; n++) {
+            const double* F = fields[s] + (int64_t)n * sTot;
+            double* G = group + (int64_t)n * gTot;
+            for (int64_t p = 0; p < sTot; p++) {
+                int64_t rem = p, g = 0, stride = 1;
+                for (int i = 0; i < nDim; i++) {
+                    int64_t idx = rem % sz[i]; rem /= sz[i];
+                    g += (idx + org[i]) * stride;
+                    stride *= groupN[i];
+                }
+                /* localization_RF.f90:228 then :253 */
+                double w = F[p] * pu[p];
+                G[g] = w + G[g];
+            }
+        }
+        free(pu);
+    }
+    if (mode == RFO_REF_COMPAT) {
+        /* calls_RF.f90:338-355: reverse taper on every face (no MPI
+         * neighbours on one rank), column 1 only */
+        double* pu1 = (double*)malloc(sizeof(double) * (size_t)gTot);
+        rfo_unity_partition(nDim, groupN, nOvl, (1 << (2 * nDim)) - 1, mode, pu1);
+        for (int64_t p = 0; p < gTot; p++) group[p] = group[p] / pu1[p];
+        free(pu1);
+    }
+}
+
+/* ---------------------------------------------------------- statistics */
+
+void rfo_normalize(double* field, int64_t N, int Nmc)
+{
+    for (int n = 0; n < Nmc; n++) {
+        double* c = field + (int64_t)n * N;
+        double s = 0.0;
+        for (int64_t i = 0; i < N; i++) s += c[i];
+        double avg = s / (double)N;                      /* :869 */
+        for (int64_t i = 0; i < N; i++) c[i] = c[i] - avg;   /* :874 */
+        double s2 = 0.0;
+        for (int64_t i = 0; i < N; i++) s2 += c[i] * c[i];
+        double sd = sqrt(s2 / (double)N);               /* :904 */
+        for (int64_t i = 0; i < N; i++) c[i] = c[i] / sd;    /* :908 */
+    }
+}
+
+void rfo_stats(const double* field, int64_t N, int Nmc, double* avg,
+               double* std, double* gAvg, double* gStd)
+{
+    double ts = 0.0, ts2 = 0.0;
+    for (int n = 0; n < Nmc; n++) {
+        const double* c = field + (int64_t)n * N;
+        double s = 0.0, s2 = 0.0;
+        for (int64_t i = 0; i < N; i++) { s += c[i]; s2 += c[i] * c[i]; }
+        avg[n] = s / (double)N;
+        std[n] = sqrt(s2 / (double)N - avg[n] * avg[n]);
+        ts += s; ts2 += s2;
+    }
+    *gAvg = ts / ((double)N * Nmc);
+    *gStd = sqrt(ts2 / ((double)N * Nmc) - (*gAvg) * (*gAvg));
+}

@@ -1,0 +1,241 @@
+"""randomfield_amd -- Python mirror of the gfx950 spectral random-field path.
+
+The product is the C ABI of ``lib/librf_hip.so`` (include/rf_capi.h) and the
+Fortran facade ``lib/librandomfield_gpu.so``; this module wraps them for
+tests, benchmarks and Python hosts.  Arrays follow the reference's
+column-major layouts seen from C: points (N, nDim) row-major == nDim x N
+column-major; fields (Nmc, N) row-major == N x Nmc column-major.
+
+Mirror of the reference surface:
+    create_random_field(...)  createRandomField / create_RF_Unstruct_Init
+                              (calls_RF.f90:20-22, randomFieldND.f90:27-132)
+    multivariate_transformation  multiVariateTransformation (randomFieldND.f90:139-178)
+    reference_seed(seedStart)    calculate_random_seed (math_RF.f90:538-566)
+    reference_draws(...)         the random_number sequence of one call
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._capi import (  # noqa: F401
+    CM_COS, CM_GAUSSIAN, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
+    FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
+    RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
+    RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, Params, RFError, SpectrumInfo, check, flib, lib,
+    make_params,
+)
+
+__all__ = [
+    "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate",
+    "create_random_field", "multivariate_transformation", "reference_seed", "reference_draws",
+    "blend", "stats", "normalize", "device_uniforms", "RFError",
+]
+
+
+class _Arr:
+    """Keeps a host array alive for the duration of the foreign call."""
+
+    def __init__(self, a: np.ndarray):
+        self.a = a
+        self._as_parameter_ = C.c_void_p(a.ctypes.data)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return _Arr(a)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return C.c_void_p(int(a))
+
+
+def _dtype_code(dt) -> int:
+    s = str(dt)
+    if "float64" in s or s == "f8":
+        return DTYPE_F64
+    if "float32" in s or s == "f4":
+        return DTYPE_F32
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+def plan_spectrum(params: Params) -> SpectrumInfo:
+    info = SpectrumInfo()
+    check(lib().rf_plan_spectrum(C.byref(params), C.byref(info)))
+    return info
+
+
+def spectrum_tables(params: Params, kU: np.ndarray | None = None):
+    """(kPoints (kNTotal, rows), SkVec (kNTotal,)) in normalised coordinates."""
+    info = plan_spectrum(params)
+    rows = 1 if params.method == ISOTROPIC else params.nDim
+    k = np.zeros((info.kNTotal, rows))
+    sk = np.zeros(info.kNTotal)
+    if kU is not None:
+        kU = np.ascontiguousarray(kU, dtype=np.float64)
+    check(lib().rf_spectrum_tables(C.byref(params), _ptr(kU), _ptr(k), _ptr(sk)))
+    return k, sk
+
+
+def reference_seed(seedStart: int, n: int | None = None) -> np.ndarray:
+    """calculate_random_seed: seed(k) = 72 + 18*seedStart + 37*(k-1)."""
+    if n is None:
+        n = flib().rf_fortran_seed_size()
+    return np.array([72 + 18 * seedStart + 37 * k for k in range(n)], dtype=np.int32)
+
+
+def reference_draws(params: Params, seed, calc=None):
+    """Uniform draws of one createRandomField call, made by the Fortran facade
+    with random_number after random_seed(PUT=seed): (kU, phU (Nmc, nPhase))."""
+    info = plan_spectrum(params)
+    seed = np.ascontiguousarray(np.atleast_1d(seed), dtype=np.int32)
+    c = np.ones(params.Nmc, dtype=np.int8) if calc is None else np.ascontiguousarray(calc, dtype=np.int8)
+    kU = np.zeros(max(1, info.nKUniform))
+    phU = np.zeros((params.Nmc, info.nPhaseUniform))
+    flib().rf_fortran_draws(C.byref(params), _ptr(seed), len(seed), _ptr(c), _ptr(kU), _ptr(phU))
+    return (kU[: info.nKUniform] if info.nKUniform > 0 else None), phU
+
+
+class Context:
+    """rf_context: device binding and stream."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().rf_context_create(C.byref(self._h), device))
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def sync(self) -> None:
+        check(lib().rf_context_sync(self._h), self._h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rf_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FieldPlan:
+    """rf_field_plan: device-resident wavevectors, phases and amplitudes."""
+
+    def __init__(self, ctx: Context, params: Params, kU=None, phU=None, calc=None):
+        self.ctx = ctx
+        self.params = params
+        self._h = C.c_void_p()
+        kU = None if kU is None else np.ascontiguousarray(kU, dtype=np.float64)
+        phU = None if phU is None else np.ascontiguousarray(phU, dtype=np.float64)
+        c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
+        check(lib().rf_plan_create(ctx.handle, C.byref(params), _ptr(kU), _ptr(phU), _ptr(c),
+                                   C.byref(self._h)), ctx.handle)
+
+    @property
+    def info(self) -> SpectrumInfo:
+        info = SpectrumInfo()
+        check(lib().rf_plan_info(self._h, C.byref(info)))
+        return info
+
+    def execute(self, x, out, stream=None) -> None:
+        """x: device (N, nDim) float64; out: device (Nmc, N) float32/float64."""
+        N = int(x.shape[0])
+        check(lib().rf_plan_execute(self._h, _ptr(x), N, _ptr(out), _dtype_code(out.dtype),
+                                    _ptr(stream)), self.ctx.handle)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rf_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate(ctx: Context, params: Params, x: np.ndarray, kU=None, phU=None, calc=None,
+             dtype=np.float64) -> np.ndarray:
+    """rf_generate on host arrays: x (N, nDim) -> field (Nmc, N)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((params.Nmc, x.shape[0]), dtype=dtype)
+    kU = None if kU is None else np.ascontiguousarray(kU, dtype=np.float64)
+    phU = None if phU is None else np.ascontiguousarray(phU, dtype=np.float64)
+    c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
+    check(lib().rf_generate(ctx.handle, C.byref(params), _ptr(x), x.shape[0], _ptr(kU), _ptr(phU),
+                            _ptr(c), _ptr(out), _dtype_code(out.dtype)), ctx.handle)
+    return out
+
+
+def create_random_field(ctx: Context, x: np.ndarray, *, method=SHINOZUKA, corrMod=CM_GAUSSIAN,
+                        corrL=(1.0, 1.0, 1.0), xRange=None, Nmc=1, seedStart=0, mode=REF_COMPAT,
+                        form=FORM_FP64, margiFirst=FOM_NONE, fieldAvg=0.0, fieldVar=1.0,
+                        kSampling=KSAMP_BOX, nKDensity=0, calc=None, isoShared=0,
+                        dtype=np.float64) -> np.ndarray:
+    """createRandomField with the reference's draws for seedStart (Fortran
+    random_number); xRange defaults to the points' bounding box."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nDim = x.shape[1]
+    if xRange is None:
+        ext = x.max(axis=0) - x.min(axis=0)
+        xRange = [e if e > 0 else corrL[i] for i, e in enumerate(ext)]
+    p = make_params(nDim=nDim, method=method, corrMod=corrMod, mode=mode, form=form,
+                    kSampling=kSampling, Nmc=Nmc, margiFirst=margiFirst, nKDensity=nKDensity,
+                    corrL=corrL[:nDim], xRange=xRange, fieldAvg=fieldAvg, fieldVar=fieldVar,
+                    isoShared=isoShared)
+    kU, phU = reference_draws(p, reference_seed(seedStart), calc)
+    return generate(ctx, p, x, kU, phU, calc, dtype)
+
+
+def multivariate_transformation(ctx: Context, margiFirst: int, fieldAvg: float, fieldVar: float,
+                                field: np.ndarray) -> None:
+    """In place on a host float64 array (GPU transform)."""
+    assert field.dtype == np.float64 and field.flags.c_contiguous
+    check(lib().rf_marginal_host(ctx.handle, margiFirst, fieldAvg, fieldVar, _ptr(field), field.size),
+          ctx.handle)
+
+
+def blend(ctx: Context, groupN, Nmc, origins, sizes, nOvl, faceMasks, mode, d_fields, d_group,
+          dtype_code, stream=None) -> None:
+    """rf_blend_device over device buffers (d_fields: list of device pointers/tensors)."""
+    nDim = len(groupN)
+    g = np.ascontiguousarray(groupN, dtype=np.int32)
+    o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1)
+    s = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1)
+    n = np.ascontiguousarray(nOvl, dtype=np.int32)
+    m = np.ascontiguousarray(faceMasks, dtype=np.int32)
+    ptrs = (C.c_void_p * len(d_fields))(*[_ptr(f).value for f in d_fields])
+    check(lib().rf_blend_device(ctx.handle, nDim, _ptr(g), Nmc, len(d_fields), _ptr(o), _ptr(s),
+                                _ptr(n), _ptr(m), mode, C.cast(ptrs, C.c_void_p), _ptr(d_group),
+                                dtype_code, _ptr(stream)), ctx.handle)
+
+
+def stats(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int):
+    avg = np.zeros(Nmc)
+    sd = np.zeros(Nmc)
+    g = np.zeros(2)
+    check(lib().rf_stats_device(ctx.handle, _ptr(d_field), N, Nmc, dtype_code, _ptr(avg), _ptr(sd),
+                                _ptr(g)), ctx.handle)
+    return avg, sd, g
+
+
+def normalize(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int, stream=None) -> None:
+    check(lib().rf_normalize_device(ctx.handle, _ptr(d_field), N, Nmc, dtype_code, _ptr(stream)),
+          ctx.handle)
+
+
+def device_uniforms(ctx: Context, seed: int, tag: int, count: int, d_out, stream=None) -> None:
+    check(lib().rf_device_uniforms(ctx.handle, seed, tag, count, _ptr(d_out), _ptr(stream)), ctx.handle)

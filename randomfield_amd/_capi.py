@@ -1,0 +1,133 @@
+"""ctypes binding of librf_hip.so (include/rf_capi.h) and of the Fortran
+facade's draw helpers (librandomfield_gpu.so).
+
+The GPU path has no CPU fallback: if the native library is missing the import
+of the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIBDIR = Path(__file__).resolve().parent / "lib"
+
+RF_OK = 0
+RF_E_ARG, RF_E_LOGNORMAL, RF_E_NOMEM, RF_E_HIP, RF_E_UNSUPPORTED, RF_E_NODEVICE = -1, -2, -3, -4, -5, -6
+ISOTROPIC, SHINOZUKA, RANDOMIZATION = 1, 2, 3
+CM_GAUSSIAN, CM_COS = 1, 2
+FOM_NONE, FOM_GAUSSIAN, FOM_LOGNORMAL = 0, 1, 2
+REF_COMPAT, FIXED = 0, 1
+FORM_FP64, FORM_FP32, FORM_MFMA_F16, FORM_MFMA_BF16 = 0, 1, 2, 3
+KSAMP_BOX, KSAMP_DENSITY = 0, 1
+DTYPE_F64, DTYPE_F32 = 0, 1
+
+# every entry point the header declares
+EXPORTS = (
+    "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
+    "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
+    "rf_plan_execute", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
+    "rf_stats_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
+    "rf_philox4x32_10_host",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nDim", C.c_int32), ("method", C.c_int32), ("corrMod", C.c_int32), ("mode", C.c_int32),
+        ("form", C.c_int32), ("kSampling", C.c_int32), ("Nmc", C.c_int32), ("margiFirst", C.c_int32),
+        ("nKDensity", C.c_int64), ("xSlab", C.c_int64),
+        ("corrL", C.c_double * 3), ("xRange", C.c_double * 3),
+        ("fieldAvg", C.c_double), ("fieldVar", C.c_double),
+        ("seed", C.c_uint64), ("isoShared", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class SpectrumInfo(C.Structure):
+    _fields_ = [
+        ("kNTotal", C.c_int64), ("kN_unsigned", C.c_int64), ("nKUniform", C.c_int64),
+        ("nPhaseUniform", C.c_int64), ("kNStep", C.c_int32 * 3), ("nSignBlocks", C.c_int32),
+        ("nVarIso", C.c_int32), ("pad", C.c_int32), ("kMax", C.c_double * 3),
+        ("kDelta", C.c_double * 3), ("ampMult", C.c_double), ("isoStep", C.c_double),
+    ]
+
+
+def make_params(nDim=3, method=SHINOZUKA, corrMod=CM_GAUSSIAN, mode=REF_COMPAT, form=FORM_FP64,
+                kSampling=KSAMP_BOX, Nmc=1, margiFirst=FOM_NONE, nKDensity=0, xSlab=0,
+                corrL=(1.0, 1.0, 1.0), xRange=(1.0, 1.0, 1.0), fieldAvg=0.0, fieldVar=1.0,
+                seed=0, isoShared=0) -> Params:
+    p = Params()
+    p.nDim, p.method, p.corrMod, p.mode, p.form = nDim, method, corrMod, mode, form
+    p.kSampling, p.Nmc, p.margiFirst, p.nKDensity, p.xSlab = kSampling, Nmc, margiFirst, nKDensity, xSlab
+    cl = list(corrL) + [1.0] * (3 - len(corrL))
+    xr = list(xRange) + [1.0] * (3 - len(xRange))
+    p.corrL = (C.c_double * 3)(*cl[:3])
+    p.xRange = (C.c_double * 3)(*xr[:3])
+    p.fieldAvg, p.fieldVar, p.seed, p.isoShared = fieldAvg, fieldVar, seed, isoShared
+    return p
+
+
+_lib = None
+_flib = None
+
+
+def lib() -> C.CDLL:
+    """librf_hip.so; raises when it is not built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = LIBDIR / "librf_hip.so"
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build() (the GPU path has no fallback)")
+    L = C.CDLL(str(path))
+    vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
+    L.rf_context_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.rf_context_destroy.argtypes = [vp]
+    L.rf_last_error.argtypes = [vp]
+    L.rf_last_error.restype = C.c_char_p
+    L.rf_context_sync.argtypes = [vp]
+    L.rf_plan_spectrum.argtypes = [C.POINTER(Params), C.POINTER(SpectrumInfo)]
+    L.rf_spectrum_tables.argtypes = [C.POINTER(Params), vp, vp, vp]
+    L.rf_plan_create.argtypes = [vp, C.POINTER(Params), vp, vp, vp, C.POINTER(vp)]
+    L.rf_plan_destroy.argtypes = [vp]
+    L.rf_plan_info.argtypes = [vp, C.POINTER(SpectrumInfo)]
+    L.rf_plan_execute.argtypes = [vp, vp, i64, vp, i32, vp]
+    L.rf_generate.argtypes = [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32]
+    L.rf_marginal_device.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp]
+    L.rf_marginal_host.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64]
+    L.rf_blend_device.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]
+    L.rf_stats_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.rf_normalize_device.argtypes = [vp, vp, i64, i32, i32, vp]
+    L.rf_device_uniforms.argtypes = [vp, C.c_uint64, C.c_uint32, i64, vp, vp]
+    L.rf_uniform_host.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+    L.rf_uniform_host.restype = C.c_double
+    L.rf_philox4x32_10_host.argtypes = [vp, vp, vp]
+    _lib = L
+    return L
+
+
+def flib() -> C.CDLL:
+    """The Fortran facade (reference-order draws with the compiler's RNG)."""
+    global _flib
+    if _flib is not None:
+        return _flib
+    lib()  # the facade links librf_hip.so
+    path = LIBDIR / "librandomfield_gpu.so"
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build()")
+    F = C.CDLL(str(path))
+    F.rf_fortran_draws.argtypes = [C.POINTER(Params), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    F.rf_fortran_seed_size.restype = C.c_int
+    _flib = F
+    return F
+
+
+class RFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != RF_OK:
+        msg = lib().rf_last_error(ctx).decode(errors="replace")
+        raise RFError(rc, msg)

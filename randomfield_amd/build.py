@@ -1,0 +1,107 @@
+"""Build the native parts in-tree.
+
+    librf_hip.so          HIP kernels + C ABI (hipcc, gfx950 code objects)
+    librandomfield_gpu.so Fortran facade over the C ABI (amdflang)
+    tests/fortran/bin/*   Fortran drivers of the facade used by the test suite
+    oracle/liboracle_rf.so  the CPU oracle (test infrastructure, gcc)
+
+Run ``python -m randomfield_amd.build`` or call :func:`build_all`.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "randomfield_amd"
+CSRC = PKG / "csrc"
+FSRC = PKG / "fortran"
+LIB = PKG / "lib"
+OBJ = PKG / "lib" / "obj"
+ORACLE = ROOT / "oracle"
+FTESTS = ROOT / "tests" / "fortran"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("RF_FC", "/opt/rocm/bin/amdflang")
+CC = os.environ.get("CC", "gcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+    # no packed FP32 (v_pk_*_f32) code generation
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    proc = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+
+
+def build_hip(jobs: int = 8) -> Path:
+    OBJ.mkdir(parents=True, exist_ok=True)
+    srcs = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+    objs = []
+    cmds = []
+    for s in srcs:
+        o = OBJ / (s.name + ".o")
+        objs.append(o)
+        lang = ["-x", "hip"] if s.suffix == ".hip" else []
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(s), "-o", str(o)])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    out = LIB / "librf_hip.so"
+    tmp = LIB / "librf_hip.so.tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran() -> Path:
+    LIB.mkdir(parents=True, exist_ok=True)
+    mod = LIB / "mod"
+    mod.mkdir(exist_ok=True)
+    out = LIB / "librandomfield_gpu.so"
+    tmp = LIB / "librandomfield_gpu.so.tmp"
+    _run([FC, "-O2", "-fPIC", "-shared", "-module-dir", str(mod),
+          str(FSRC / "randomField_gpu.f90"), "-o", str(tmp),
+          f"-L{LIB}", "-lrf_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    # Fortran drivers of the facade (test programs)
+    bindir = FTESTS / "bin"
+    bindir.mkdir(parents=True, exist_ok=True)
+    for src in sorted(FTESTS.glob("*.f90")):
+        exe = bindir / src.stem
+        _run([FC, "-O2", "-I", str(mod), str(src), "-o", str(exe),
+              f"-L{LIB}", "-lrandomfield_gpu", "-lrf_hip", f"-Wl,-rpath,{LIB}",
+              "-Wl,-rpath,$ORIGIN/../../../randomfield_amd/lib"])
+    return out
+
+
+def build_oracle() -> Path:
+    out = ORACLE / "liboracle_rf.so"
+    tmp = ORACLE / "liboracle_rf.so.tmp"
+    _run([CC, "-O2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
+          str(ORACLE / "rf_oracle.c"), "-o", str(tmp), "-lm"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(jobs: int = 8) -> None:
+    build_oracle()
+    build_hip(jobs)
+    if shutil.which(FC) or Path(FC).exists():
+        build_fortran()
+    else:
+        raise RuntimeError(f"Fortran compiler not found: {FC}")
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built:", LIB / "librf_hip.so", LIB / "librandomfield_gpu.so", ORACLE / "liboracle_rf.so")

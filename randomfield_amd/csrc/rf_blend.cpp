@@ -1,0 +1,196 @@
+// rf_blend.cpp -- host side of rf_blend_device: analyses the subdomain boxes
+// (a tensor-product arrangement, as prepare_Localization's setGrid of
+// subdivision origins produces, localization_RF.f90:151-155), builds the
+// per-axis cover tables and the separable partition-of-unity factors, and
+// launches the gather-form blend kernel.
+//
+// Partition of unity (generateUnityPartition_Matrix, localization_RF.f90:705-857):
+// the weight of a subdomain point is a product over tapered faces of a
+// per-axis factor, so it is stored per (subdomain, axis, position).
+//   REF_COMPAT  lower face position s = 1..n: v = (n+1-s)/(n+2); upper face
+//               (index N-n+s): v = s/(n+2); factor (1 + cos(pi v))/2 (:800-813),
+//               applied linearly (:228).  Afterwards realization 1 of the
+//               group is divided by the same taper on every group face
+//               (calls_RF.f90:338-355; one rank has no MPI neighbours).
+//   FIXED       v = (n+1-s)/(n+1) resp. s/(n+1), factor sqrt((1+cos(pi v))/2):
+//               two faces meeting in an n-point overlap sum their squared
+//               factors to exactly one, so independent unit-variance
+//               subdomains blend to unit variance.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <utility>
+
+#include "rf_blend.h"
+#include "rf_internal.h"
+
+namespace {
+
+double face_factor(int s, int n, bool upper, int mode) {
+    const int q = upper ? s : (n + 1 - s);
+    if (mode == RF_REF_COMPAT) {
+        const double v = std::fabs((double)q / (double)(n + 2));
+        return (1.0 + std::cos(RF_PI_REF * v)) / 2.0;
+    }
+    const double v = (double)q / (double)(n + 1);
+    return std::sqrt((1.0 + std::cos(RF_PI_REF * v)) / 2.0);
+}
+
+// factors of an N-point axis: lower face at positions 0..n-1, upper face at
+// N-n..N-1 (1.0 where untapered)
+void axis_factors(int N, int n, bool lower, bool upper, int mode, double* low, double* up) {
+    for (int p = 0; p < N; p++) { low[p] = 1.0; up[p] = 1.0; }
+    if (n < 1) return;
+    if (lower)
+        for (int s = 1; s <= n; s++) low[s - 1] = face_factor(s, n, false, mode);
+    if (upper)
+        for (int s = 1; s <= n; s++) up[N - n + s - 1] = face_factor(s, n, true, mode);
+}
+
+}  // namespace
+
+extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
+                               int32_t nSub, const int32_t* origins, const int32_t* sizes,
+                               const int32_t* nOvl, const int32_t* faceMasks, int32_t mode,
+                               const void* const* d_fields, void* d_group, int32_t dtype,
+                               void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (nDim < 1 || nDim > 3 || Nmc < 1 || nSub < 1) return rf_set_error(ctx, RF_E_ARG, "bad blend sizes");
+    if (!groupN || !origins || !sizes || !nOvl || !faceMasks || !d_fields || !d_group)
+        return rf_set_error(ctx, RF_E_ARG, "NULL blend argument");
+    if (mode != RF_REF_COMPAT && mode != RF_FIXED) return rf_set_error(ctx, RF_E_ARG, "unknown mode");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    int64_t gTot = 1;
+    for (int a = 0; a < nDim; a++) {
+        if (groupN[a] < 1) return rf_set_error(ctx, RF_E_ARG, "group size must be >= 1");
+        if (nOvl[a] < 0) return rf_set_error(ctx, RF_E_ARG, "overlap points must be >= 0");
+        gTot *= groupN[a];
+    }
+    // distinct intervals per axis
+    std::vector<std::pair<int, int>> ints[3];
+    std::vector<int> subInt((size_t)nSub * 3, 0);
+    int maxSize = 1;
+    for (int s = 0; s < nSub; s++) {
+        for (int a = 0; a < nDim; a++) {
+            const int o = origins[s * nDim + a], z = sizes[s * nDim + a];
+            if (z < 1 || o < 0 || o + z > groupN[a])
+                return rf_set_error(ctx, RF_E_ARG, "subdomain box outside the group grid");
+            for (int f = 0; f < 2; f++)
+                if ((faceMasks[s] >> (2 * a + f)) & 1)
+                    if (nOvl[a] > z)
+                        return rf_set_error(ctx, RF_E_ARG, "subdomain thinner than its tapered overlap");
+            maxSize = std::max(maxSize, z);
+            ints[a].push_back({o, z});
+        }
+    }
+    int nInt[3] = {1, 1, 1};
+    for (int a = 0; a < nDim; a++) {
+        std::sort(ints[a].begin(), ints[a].end());
+        ints[a].erase(std::unique(ints[a].begin(), ints[a].end()), ints[a].end());
+        nInt[a] = (int)ints[a].size();
+    }
+    for (int s = 0; s < nSub; s++)
+        for (int a = 0; a < nDim; a++) {
+            const std::pair<int, int> key{origins[s * nDim + a], sizes[s * nDim + a]};
+            subInt[(size_t)s * 3 + a] = (int)(std::lower_bound(ints[a].begin(), ints[a].end(), key) - ints[a].begin());
+        }
+    const int64_t cells = (int64_t)nInt[0] * nInt[1] * nInt[2];
+    std::vector<int32_t> subOf((size_t)cells, -1);
+    for (int s = 0; s < nSub; s++) {
+        const int64_t c = subInt[(size_t)s * 3] + (int64_t)nInt[0] * (subInt[(size_t)s * 3 + 1] + (int64_t)nInt[1] * subInt[(size_t)s * 3 + 2]);
+        if (subOf[(size_t)c] >= 0) return rf_set_error(ctx, RF_E_UNSUPPORTED, "two subdomains share the same box");
+        subOf[(size_t)c] = s;
+    }
+    // cover tables: <= 2 intervals per group index, ascending
+    std::vector<int32_t> cover[3];
+    for (int a = 0; a < nDim; a++) {
+        cover[a].assign((size_t)groupN[a] * 2, -1);
+        for (int iv = 0; iv < nInt[a]; iv++) {
+            for (int g = ints[a][iv].first; g < ints[a][iv].first + ints[a][iv].second; g++) {
+                int32_t* c = &cover[a][(size_t)g * 2];
+                if (c[0] < 0) c[0] = iv;
+                else if (c[1] < 0) c[1] = iv;
+                else return rf_set_error(ctx, RF_E_UNSUPPORTED, "more than two subdomains overlap along one axis");
+            }
+        }
+    }
+    // taper factor tables
+    std::vector<double> lowF((size_t)nSub * 3 * maxSize, 1.0), upF((size_t)nSub * 3 * maxSize, 1.0);
+    for (int s = 0; s < nSub; s++)
+        for (int a = 0; a < nDim; a++)
+            axis_factors(sizes[s * nDim + a], nOvl[a], (faceMasks[s] >> (2 * a)) & 1,
+                         (faceMasks[s] >> (2 * a + 1)) & 1, mode,
+                         &lowF[((size_t)s * 3 + a) * maxSize], &upF[((size_t)s * 3 + a) * maxSize]);
+    std::vector<double> pu1[3][2];
+    for (int a = 0; a < nDim; a++) {
+        pu1[a][0].assign((size_t)groupN[a], 1.0);
+        pu1[a][1].assign((size_t)groupN[a], 1.0);
+        if (mode == RF_REF_COMPAT)
+            axis_factors(groupN[a], std::min(nOvl[a], groupN[a]), true, true, mode, pu1[a][0].data(), pu1[a][1].data());
+    }
+    std::vector<int64_t> subTot((size_t)nSub);
+    for (int s = 0; s < nSub; s++) {
+        int64_t t = 1;
+        for (int a = 0; a < nDim; a++) t *= sizes[s * nDim + a];
+        subTot[(size_t)s] = t;
+    }
+    std::vector<int32_t> intO[3], intZ[3];
+    for (int a = 0; a < nDim; a++)
+        for (auto& iv : ints[a]) { intO[a].push_back(iv.first); intZ[a].push_back(iv.second); }
+
+    // one device block for every table
+    std::vector<char> blob;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t off = (blob.size() + 15) / 16 * 16;
+        blob.resize(off + bytes);
+        if (bytes) std::memcpy(blob.data() + off, src, bytes);
+        return off;
+    };
+    size_t oCover[3] = {0, 0, 0}, oIO[3] = {0, 0, 0}, oIZ[3] = {0, 0, 0}, oP1[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int a = 0; a < nDim; a++) {
+        oCover[a] = put(cover[a].data(), cover[a].size() * 4);
+        oIO[a] = put(intO[a].data(), intO[a].size() * 4);
+        oIZ[a] = put(intZ[a].data(), intZ[a].size() * 4);
+        oP1[a][0] = put(pu1[a][0].data(), pu1[a][0].size() * 8);
+        oP1[a][1] = put(pu1[a][1].data(), pu1[a][1].size() * 8);
+    }
+    const size_t oSub = put(subOf.data(), subOf.size() * 4);
+    const size_t oTot = put(subTot.data(), subTot.size() * 8);
+    const size_t oLow = put(lowF.data(), lowF.size() * 8);
+    const size_t oUp = put(upF.data(), upF.size() * 8);
+    const size_t oF = put(d_fields, (size_t)nSub * sizeof(void*));
+
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    char* d_blob = nullptr;
+    RF_HIP_TRY(ctx, hipMalloc((void**)&d_blob, blob.size()));
+    hipError_t e = hipMemcpyAsync(d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        rf_blend_dev b{};
+        b.nDim = nDim; b.Nmc = Nmc; b.mode = mode;
+        for (int a = 0; a < 3; a++) {
+            b.groupN[a] = a < nDim ? groupN[a] : 1;
+            b.nInt[a] = nInt[a];
+            if (a < nDim) {
+                b.cover[a] = reinterpret_cast<const int32_t*>(d_blob + oCover[a]);
+                b.intOrigin[a] = reinterpret_cast<const int32_t*>(d_blob + oIO[a]);
+                b.intSize[a] = reinterpret_cast<const int32_t*>(d_blob + oIZ[a]);
+                b.pu1Low[a] = reinterpret_cast<const double*>(d_blob + oP1[a][0]);
+                b.pu1Up[a] = reinterpret_cast<const double*>(d_blob + oP1[a][1]);
+            }
+        }
+        b.groupTot = gTot;
+        b.subOf = reinterpret_cast<const int32_t*>(d_blob + oSub);
+        b.subTot = reinterpret_cast<const int64_t*>(d_blob + oTot);
+        b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
+        b.upF = reinterpret_cast<const double*>(d_blob + oUp);
+        b.maxSize = maxSize;
+        b.fields = reinterpret_cast<const void* const*>(d_blob + oF);
+        e = rf_launch_blend_dev(b, d_group, dtype, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // tables freed below
+    (void)hipFree(d_blob);
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("blend: ") + hipGetErrorString(e));
+    return RF_OK;
+}

@@ -1,0 +1,586 @@
+// rf_capi.cpp -- the C ABI of librf_hip.so (declared in include/rf_capi.h).
+//
+// A plan holds everything of one createRandomField call that does not depend
+// on the points: the wavevector table, phases and amplitudes, resident in HBM
+// in the layout each precision form reads.  rf_plan_execute is the hot path
+// (randomFieldND.f90:184-514): one launch over the points, or one per point
+// slab in REF_COMPAT, where the reference re-applies sqrt(Sk) per slab.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "rf_blend.h"
+#include "rf_internal.h"
+#include "rf_philox.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int plan_fail(rf_context* ctx, int code, const std::string& msg) { return rf_set_error(ctx, code, msg); }
+
+// Marginal-map constants of multiVariateTransformation (randomFieldND.f90:154-169).
+rf_epilogue make_epilogue(int margi, double avg, double var) {
+    rf_epilogue e{};
+    e.margi = margi;
+    e.scale = 1.0;
+    e.shift = 0.0;
+    if (margi == RF_FOM_GAUSSIAN) {
+        e.scale = std::sqrt(var);
+        e.shift = avg;
+    } else if (margi == RF_FOM_LOGNORMAL) {
+        const double nv = std::log(1.0 + var / (avg * avg));
+        e.scale = std::sqrt(nv);
+        e.shift = std::log(avg) - nv / 2.0;
+    }
+    return e;
+}
+
+bool is_iso_direct(const rf_params& p) { return p.method == RF_ISOTROPIC && !p.isoShared; }
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Form-specific operand build from d_phase / d_amp (Nmc x kN, FP64).
+int build_form_operands(rf_field_plan* P, hipStream_t s) {
+    rf_context* ctx = P->ctx;
+    const int64_t kN = P->spec.info.kNTotal;
+    const int Nmc = P->p.Nmc;
+    const double ampMult = P->spec.info.ampMult;
+    if (P->p.form == RF_FORM_FP32) {
+        if (is_iso_direct(P->p)) {
+            if (!P->d_kIsoT) RF_HIP_TRY(ctx, dmalloc(&P->d_kIsoT, (size_t)(kN * P->Rpad * P->p.nDim)));
+            RF_HIP_TRY(ctx, rf_launch_prep_fp32_iso(P->p.nDim, kN, Nmc, P->Rpad, P->d_kIso, P->d_phase,
+                                                    P->d_amp, ampMult, P->d_kIsoT, P->d_phT, P->d_aT, s));
+        } else {
+            RF_HIP_TRY(ctx, rf_launch_prep_fp32(kN, Nmc, P->Rpad, P->d_phase, P->d_amp, ampMult,
+                                                P->d_phT, P->d_aT, s));
+        }
+    } else if (P->p.form == RF_FORM_MFMA_F16 || P->p.form == RF_FORM_MFMA_BF16) {
+        RF_HIP_TRY(ctx, rf_launch_mfma_prep(P->p.form, kN, P->kNpad, Nmc, P->Rpad, P->d_phase,
+                                            P->d_amp, ampMult * P->bScale, P->d_B, s));
+    }
+    return RF_OK;
+}
+
+// Power-of-two scale that brings the largest |amp * ampMult| near 2^14, so the
+// FP16 hi/lo split keeps small amplitudes out of the subnormal range.
+double mfma_bscale(const rf_field_plan* P, double maxAmp) {
+    const double m = maxAmp * P->spec.info.ampMult;
+    if (!(m > 0.0)) return 1.0;
+    int e = 0;
+    std::frexp(m, &e);   // m = f * 2^e, f in [0.5, 1)
+    return std::ldexp(1.0, 14 - e);
+}
+
+}  // namespace
+
+int rf_set_error(rf_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    g_last_error = msg;
+    return code;
+}
+
+extern "C" {
+
+int rf_version(void) { return RF_CAPI_VERSION; }
+
+double rf_uniform_host(uint64_t seed, uint32_t tag, uint64_t e) { return rf_uniform(seed, tag, e); }
+
+void rf_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    rf_u32x4 c;
+    for (int i = 0; i < 4; i++) c.v[i] = ctr[i];
+    const rf_u32x4 o = rf_philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; i++) out[i] = o.v[i];
+}
+
+const char* rf_last_error(const rf_context* ctx) {
+    if (ctx && !ctx->last_error.empty()) return ctx->last_error.c_str();
+    return g_last_error.c_str();
+}
+
+int rf_context_create(rf_context** out, int device) {
+    if (!out) return rf_set_error(nullptr, RF_E_ARG, "ctx out-pointer is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return rf_set_error(nullptr, RF_E_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return rf_set_error(nullptr, RF_E_ARG, "device index out of range");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return rf_set_error(nullptr, RF_E_NODEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return rf_set_error(nullptr, RF_E_NODEVICE,
+                            std::string("librf_hip.so is built for gfx950 only, device is ") + prop.gcnArchName);
+    if (hipSetDevice(device) != hipSuccess) return rf_set_error(nullptr, RF_E_HIP, "hipSetDevice failed");
+    auto* c = new (std::nothrow) rf_context();
+    if (!c) return rf_set_error(nullptr, RF_E_NOMEM, "context allocation failed");
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return rf_set_error(nullptr, RF_E_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return RF_OK;
+}
+
+int rf_context_destroy(rf_context* ctx) {
+    if (!ctx) return RF_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    dfree(ctx->d_scratch);
+    delete ctx;
+    return RF_OK;
+}
+
+int rf_context_sync(rf_context* ctx) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return RF_OK;
+}
+
+int rf_plan_spectrum(const rf_params* p, rf_spectrum_info* info) {
+    if (!info) return rf_set_error(nullptr, RF_E_ARG, "info is NULL");
+    std::string err;
+    const int rc = rf_spectrum_build_info(p, info, &err);
+    if (rc) return rf_set_error(nullptr, rc, err);
+    return RF_OK;
+}
+
+int rf_spectrum_tables(const rf_params* p, const double* kUniform, double* kPoints, double* Sk) {
+    rf_spectrum_info info;
+    std::string err;
+    int rc = rf_spectrum_build_info(p, &info, &err);
+    if (rc) return rf_set_error(nullptr, rc, err);
+    rf_spectrum spec;
+    rc = rf_spectrum_build_tables(p, info, kUniform, &spec, &err);
+    if (rc) return rf_set_error(nullptr, rc, err);
+    if (kPoints) std::memcpy(kPoints, spec.k.data(), spec.k.size() * sizeof(double));
+    if (Sk) std::memcpy(Sk, spec.Sk.data(), spec.Sk.size() * sizeof(double));
+    return RF_OK;
+}
+
+int rf_plan_destroy(rf_field_plan* P) {
+    if (!P) return RF_OK;
+    if (P->ctx) {
+        (void)hipSetDevice(P->ctx->device);
+        (void)hipStreamSynchronize(P->ctx->stream);
+    }
+    dfree(P->d_k); dfree(P->d_kRev); dfree(P->d_kIso); dfree(P->d_kIsoT);
+    dfree(P->d_phase); dfree(P->d_amp); dfree(P->d_uni); dfree(P->d_phT);
+    dfree(P->d_aT); dfree(P->d_B); dfree(P->d_Sk); dfree(P->d_calc);
+    delete P;
+    return RF_OK;
+}
+
+int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
+                   const double* phaseUniform, const uint8_t* calculate, rf_field_plan** out) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!out) return rf_set_error(ctx, RF_E_ARG, "plan out-pointer is NULL");
+    *out = nullptr;
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::string err;
+    rf_spectrum_info info;
+    int rc = rf_spectrum_build_info(p, &info, &err);
+    if (rc) return rf_set_error(ctx, rc, err);
+
+    std::unique_ptr<rf_field_plan, int (*)(rf_field_plan*)> P(new (std::nothrow) rf_field_plan(),
+                                                              rf_plan_destroy);
+    if (!P) return rf_set_error(ctx, RF_E_NOMEM, "plan allocation failed");
+    P->ctx = ctx;
+    P->p = *p;
+    const hipStream_t s = ctx->stream;
+    const int d = p->nDim;
+    const int Nmc = p->Nmc;
+    const int64_t kN = info.kNTotal;
+
+    // wavevector uniforms (method 3): caller's draws or the device stream
+    std::vector<double> kU;
+    if (info.nKUniform > 0 && !kUniform) {
+        double* d_tmp = nullptr;
+        RF_HIP_TRY(ctx, dmalloc(&d_tmp, (size_t)info.nKUniform));
+        hipError_t e = rf_launch_uniforms(p->seed, 1u, info.nKUniform, d_tmp, s);
+        kU.resize((size_t)info.nKUniform);
+        if (e == hipSuccess) e = hipMemcpyAsync(kU.data(), d_tmp, kU.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        dfree(d_tmp);
+        if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("wavevector draws: ") + hipGetErrorString(e));
+        kUniform = kU.data();
+    }
+    rc = rf_spectrum_build_tables(p, info, kUniform, &P->spec, &err);
+    if (rc) return rf_set_error(ctx, rc, err);
+
+    P->calc.assign((size_t)Nmc, 1);
+    if (calculate)
+        for (int r = 0; r < Nmc; r++) P->calc[r] = calculate[r] ? 1 : 0;
+    P->nCalc = (int)std::count(P->calc.begin(), P->calc.end(), (uint8_t)1);
+    P->epi = make_epilogue(p->margiFirst, p->fieldAvg, p->fieldVar);
+
+    // phase uniforms: Nmc x nPhaseUniform, realization-major
+    const int64_t nPU = info.nPhaseUniform * Nmc;
+    RF_HIP_TRY(ctx, dmalloc(&P->d_uni, (size_t)nPU));
+    if (phaseUniform)
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_uni, phaseUniform, (size_t)nPU * sizeof(double), hipMemcpyHostToDevice, s));
+    else
+        RF_HIP_TRY(ctx, rf_launch_uniforms(p->seed, 2u, nPU, P->d_uni, s));
+
+    RF_HIP_TRY(ctx, dmalloc(&P->d_phase, (size_t)(kN * Nmc)));
+    RF_HIP_TRY(ctx, dmalloc(&P->d_amp, (size_t)(kN * Nmc)));
+    RF_HIP_TRY(ctx, dmalloc(&P->d_calc, (size_t)Nmc));
+    RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_calc, P->calc.data(), (size_t)Nmc, hipMemcpyHostToDevice, s));
+
+    const bool mfma = (p->form == RF_FORM_MFMA_F16 || p->form == RF_FORM_MFMA_BF16);
+    P->Rpad = mfma ? rf_mfma_npad(Nmc) : round_up(Nmc, rf_fp32_rchunk());
+    P->kNpad = mfma ? (kN + rf_mfma_kpad(p->form) - 1) / rf_mfma_kpad(p->form) * rf_mfma_kpad(p->form) : kN;
+
+    double maxAmp = 0.0;
+    if (p->method == RF_ISOTROPIC) {
+        RF_HIP_TRY(ctx, dmalloc(&P->d_kIso, (size_t)(kN * Nmc * d)));
+        RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)kN));
+        RF_HIP_TRY(ctx, dmalloc(&P->d_Sk, (size_t)kN));
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->spec.k.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_Sk, P->spec.Sk.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
+        RF_HIP_TRY(ctx, rf_launch_iso_prep(d, p->mode, p->isoShared, kN, Nmc, P->d_k, P->d_Sk, P->d_uni,
+                                           P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
+        if (p->isoShared) {
+            // realization 0's direction table is the shared wavevector table
+            RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // iso_prep read the radii
+            dfree(P->d_k);
+            RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)(kN * d)));
+            RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->d_kIso, (size_t)(kN * d) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        // |amp| <= sqrt(S k^2) * max|gamma|
+        for (int64_t j = 0; j < kN; j++) {
+            const double k = P->spec.k[j];
+            maxAmp = std::max(maxAmp, std::sqrt(P->spec.Sk[j] * (d == 3 ? k * k : k)) * (d == 3 ? 1.7320508075688772 : 2.0 * RF_PI_REF));
+        }
+    } else {
+        RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)(kN * d)));
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->spec.k.data(), (size_t)(kN * d) * sizeof(double), hipMemcpyHostToDevice, s));
+        RF_HIP_TRY(ctx, rf_launch_phase_from_uniform(P->d_uni, kN * Nmc, P->d_phase, s));
+        // REF_COMPAT: the sqrt chain Sk^(2^-c), c = 1, 2, ... until fixed point
+        if (p->mode == RF_REF_COMPAT) {
+            std::vector<double> row(P->spec.Sk);
+            for (int c = 0; c < 4096; c++) {
+                bool changed = false;
+                for (auto& v : row) {
+                    const double nv = std::sqrt(v);
+                    changed |= (nv != v);
+                    v = nv;
+                }
+                P->sqrtChain.push_back(row);
+                if (!changed) break;
+            }
+        } else {
+            std::vector<double> amp((size_t)(kN * Nmc));
+            for (int r = 0; r < Nmc; r++)
+                for (int64_t j = 0; j < kN; j++)
+                    amp[(size_t)(r * kN + j)] = P->calc[r] ? std::sqrt(P->spec.Sk[j]) : 0.0;
+            RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_amp, amp.data(), amp.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // amp leaves scope
+        }
+        for (double v : P->spec.Sk) maxAmp = std::max(maxAmp, std::sqrt(v));
+        if (p->mode == RF_REF_COMPAT) maxAmp = std::max(maxAmp, 1.0);
+    }
+
+    // operand buffers of the chosen form
+    if (p->form == RF_FORM_FP32) {
+        RF_HIP_TRY(ctx, dmalloc(&P->d_phT, (size_t)(kN * P->Rpad)));
+        RF_HIP_TRY(ctx, dmalloc(&P->d_aT, (size_t)(kN * P->Rpad)));
+    }
+    if (p->form != RF_FORM_FP64 && !is_iso_direct(*p)) {
+        // the MFMA form reads whole K-steps: zero wavevectors pad the tail
+        RF_HIP_TRY(ctx, dmalloc(&P->d_kRev, (size_t)(P->kNpad * d)));
+        if (P->kNpad > kN)
+            RF_HIP_TRY(ctx, hipMemsetAsync(P->d_kRev, 0, (size_t)(P->kNpad * d) * sizeof(double), s));
+        RF_HIP_TRY(ctx, rf_launch_krev(kN * d, P->d_k, P->d_kRev, s));
+    }
+    if (mfma) {
+        P->bScale = mfma_bscale(P.get(), maxAmp);
+        RF_HIP_TRY(ctx, hipMalloc(&P->d_B, rf_mfma_bbytes(p->form, P->kNpad, P->Rpad)));
+    }
+    const bool perSlab = (p->mode == RF_REF_COMPAT && p->method != RF_ISOTROPIC);
+    if (!perSlab) {
+        rc = build_form_operands(P.get(), s);
+        if (rc) return rc;
+        P->coeffsReady = true;
+    }
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+    *out = P.release();
+    return RF_OK;
+}
+
+int rf_plan_info(const rf_field_plan* P, rf_spectrum_info* info) {
+    if (!P || !info) return rf_set_error(nullptr, RF_E_ARG, "NULL argument");
+    *info = P->spec.info;
+    return RF_OK;
+}
+
+int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out, int32_t outDtype,
+                    void* stream) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
+    rf_context* ctx = P->ctx;
+    if (N < 0) return rf_set_error(ctx, RF_E_ARG, "N must be >= 0");
+    if (N == 0) return RF_OK;
+    if (!d_x || !d_out) return rf_set_error(ctx, RF_E_ARG, "NULL device buffer");
+    if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32)
+        return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const rf_params& p = P->p;
+    const int64_t kN = P->spec.info.kNTotal;
+
+    rf_sum_args a{};
+    a.x = d_x;
+    a.N = N;
+    a.nDim = p.nDim;
+    a.Nmc = p.Nmc;
+    a.kN = kN;
+    for (int i = 0; i < 3; i++) a.corrL[i] = (i < p.nDim) ? p.corrL[i] : 1.0;
+    a.ampMult = P->spec.info.ampMult;
+    a.epi = P->epi;
+    a.out = d_out;
+    a.outDtype = outDtype;
+
+    auto launch_range = [&](int64_t i0, int64_t i1) -> int {
+        a.i0 = i0;
+        a.i1 = i1;
+        hipError_t e = hipSuccess;
+        switch (p.form) {
+            case RF_FORM_FP64:
+                if (is_iso_direct(p)) e = rf_launch_sum_fp64(a, P->d_kIso, kN * p.nDim, P->d_amp, P->d_phase, s);
+                else e = rf_launch_sum_fp64(a, P->d_k, 0, P->d_amp, P->d_phase, s);
+                break;
+            case RF_FORM_FP32:
+                if (is_iso_direct(p)) e = rf_launch_sum_fp32_iso_t(a, P->d_kIsoT, P->d_phT, P->d_aT, P->Rpad, s);
+                else e = rf_launch_sum_fp32(a, P->d_kRev, P->d_phT, P->d_aT, P->Rpad, s);
+                break;
+            default:
+                e = rf_launch_sum_mfma(p.form, a, P->d_kRev, P->kNpad, P->Rpad, P->d_B, 1.0 / P->bScale, s);
+                break;
+        }
+        if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("sum kernel launch: ") + hipGetErrorString(e));
+        return RF_OK;
+    };
+
+    if (P->coeffsReady) return launch_range(0, N);
+
+    // REF_COMPAT methods 2/3: point slabs (randomFieldND.f90:237-239) and the
+    // per-pass sqrt exponent (:312); pass c = rank(n) * xNSlab + m + 1.
+    int64_t xSlab = p.xSlab > 0 ? p.xSlab : (int64_t)((double)p.nDim * 1e8 / (double)kN);
+    if (xSlab > N) xSlab = N;
+    if (xSlab < 1) xSlab = 1;
+    const int64_t xNSlab = (N + xSlab - 1) / xSlab;
+    std::vector<double> amp((size_t)(kN * p.Nmc));
+    const int64_t lastRow = (int64_t)P->sqrtChain.size() - 1;
+    for (int64_t m = 0; m < xNSlab; m++) {
+        int rank = 0;
+        for (int r = 0; r < p.Nmc; r++) {
+            double* dst = amp.data() + (size_t)r * kN;
+            if (!P->calc[r]) {
+                std::fill(dst, dst + kN, 0.0);
+                continue;
+            }
+            const int64_t c = std::min<int64_t>((int64_t)rank * xNSlab + m, lastRow);
+            std::memcpy(dst, P->sqrtChain[(size_t)c].data(), (size_t)kN * sizeof(double));
+            rank++;
+        }
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_amp, amp.data(), amp.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        int rc = build_form_operands(P, s);
+        if (rc) return rc;
+        const int64_t i0 = m * xSlab;
+        const int64_t i1 = std::min(N, i0 + xSlab);
+        rc = launch_range(i0, i1);
+        if (rc) return rc;
+        RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // amp is reused next slab
+    }
+    return RF_OK;
+}
+
+int rf_generate(rf_context* ctx, const rf_params* p, const double* x, int64_t N,
+                const double* kUniform, const double* phaseUniform, const uint8_t* calculate,
+                void* out, int32_t outDtype) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (N < 0 || (N > 0 && (!x || !out))) return rf_set_error(ctx, RF_E_ARG, "bad point/output buffers");
+    rf_field_plan* P = nullptr;
+    int rc = rf_plan_create(ctx, p, kUniform, phaseUniform, calculate, &P);
+    if (rc) return rc;
+    std::unique_ptr<rf_field_plan, int (*)(rf_field_plan*)> guard(P, rf_plan_destroy);
+    if (N == 0) return RF_OK;
+    const size_t esz = (outDtype == RF_DTYPE_F64) ? 8 : 4;
+    double* d_x = nullptr;
+    void* d_out = nullptr;
+    RF_HIP_TRY(ctx, dmalloc(&d_x, (size_t)(N * p->nDim)));
+    hipError_t e = hipMalloc(&d_out, (size_t)N * p->Nmc * esz);
+    if (e != hipSuccess) {
+        dfree(d_x);
+        return rf_set_error(ctx, RF_E_NOMEM, "output allocation failed");
+    }
+    e = hipMemcpyAsync(d_x, x, (size_t)(N * p->nDim) * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) rc = rf_plan_execute(P, d_x, N, d_out, outDtype, ctx->stream);
+    if (e == hipSuccess && rc == RF_OK)
+        e = hipMemcpyAsync(out, d_out, (size_t)N * p->Nmc * esz, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d_x);
+    dfree(d_out);
+    if (rc) return rc;
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("rf_generate: ") + hipGetErrorString(e));
+    return RF_OK;
+}
+
+int rf_marginal_device(rf_context* ctx, int32_t margiFirst, double fieldAvg, double fieldVar,
+                       void* d_field, int64_t count, int32_t dtype, void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (margiFirst < RF_FOM_NONE || margiFirst > RF_FOM_LOGNORMAL) return rf_set_error(ctx, RF_E_ARG, "unknown margiFirst");
+    if (margiFirst == RF_FOM_LOGNORMAL && !(fieldAvg > 0.0))
+        return rf_set_error(ctx, RF_E_LOGNORMAL, "ERROR - when using lognormal fieldAvg should be a positive number");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    if (count < 0 || (count > 0 && !d_field)) return rf_set_error(ctx, RF_E_ARG, "bad field buffer");
+    if (margiFirst == RF_FOM_NONE || count == 0) return RF_OK;
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const rf_epilogue e = make_epilogue(margiFirst, fieldAvg, fieldVar);
+    RF_HIP_TRY(ctx, rf_launch_marginal(margiFirst, e.scale, e.shift, d_field, count, dtype,
+                                       stream ? (hipStream_t)stream : ctx->stream));
+    return RF_OK;
+}
+
+int rf_marginal_host(rf_context* ctx, int32_t margiFirst, double fieldAvg, double fieldVar,
+                     double* field, int64_t count) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (count < 0 || (count > 0 && !field)) return rf_set_error(ctx, RF_E_ARG, "bad field buffer");
+    if (margiFirst == RF_FOM_LOGNORMAL && !(fieldAvg > 0.0))
+        return rf_set_error(ctx, RF_E_LOGNORMAL, "ERROR - when using lognormal fieldAvg should be a positive number");
+    if (count == 0 || margiFirst == RF_FOM_NONE) return RF_OK;
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* d = nullptr;
+    RF_HIP_TRY(ctx, dmalloc(&d, (size_t)count));
+    hipError_t e = hipMemcpyAsync(d, field, (size_t)count * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    int rc = RF_OK;
+    if (e == hipSuccess) rc = rf_marginal_device(ctx, margiFirst, fieldAvg, fieldVar, d, count, RF_DTYPE_F64, ctx->stream);
+    if (e == hipSuccess && rc == RF_OK)
+        e = hipMemcpyAsync(field, d, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(d);
+    if (rc) return rc;
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("rf_marginal_host: ") + hipGetErrorString(e));
+    return RF_OK;
+}
+
+}  // extern "C"
+
+// ---- statistics, normalisation and blend -----------------------------------
+
+namespace {
+
+int ensure_scratch(rf_context* ctx, size_t bytes) {
+    if (ctx->scratch_bytes >= bytes) return RF_OK;
+    dfree(ctx->d_scratch);
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    RF_HIP_TRY(ctx, hipMalloc(&ctx->d_scratch, bytes));
+    ctx->scratch_bytes = bytes;
+    return RF_OK;
+}
+
+// Sums of u and u^2 per realization into host sums[2*Nmc] (synchronous).
+int moments(rf_context* ctx, const void* d, int64_t N, int Nmc, int dtype, hipStream_t s,
+            std::vector<double>* sums) {
+    const int nPart = rf_moments_parts(N);
+    const size_t bytes = sizeof(double) * ((size_t)nPart * Nmc * 2 + (size_t)Nmc * 2);
+    int rc = ensure_scratch(ctx, bytes);
+    if (rc) return rc;
+    double* part = static_cast<double*>(ctx->d_scratch);
+    double* dsum = part + (size_t)nPart * Nmc * 2;
+    RF_HIP_TRY(ctx, rf_launch_moments(d, N, Nmc, dtype, part, nPart, dsum, s));
+    sums->resize((size_t)Nmc * 2);
+    RF_HIP_TRY(ctx, hipMemcpyAsync(sums->data(), dsum, sizeof(double) * Nmc * 2, hipMemcpyDeviceToHost, s));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rf_stats_device(rf_context* ctx, const void* d_field, int64_t N, int32_t Nmc, int32_t dtype,
+                    double* avg, double* std, double* g) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (N < 1 || Nmc < 1 || !d_field) return rf_set_error(ctx, RF_E_ARG, "bad field");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<double> sums;
+    int rc = moments(ctx, d_field, N, Nmc, dtype, ctx->stream, &sums);
+    if (rc) return rc;
+    // statistics_RF.f90:81-87 (biased, E[u^2] - E[u]^2)
+    double t1 = 0.0, t2 = 0.0;
+    for (int r = 0; r < Nmc; r++) {
+        const double m = sums[2 * r] / (double)N;
+        if (avg) avg[r] = m;
+        if (std) std[r] = std::sqrt(sums[2 * r + 1] / (double)N - m * m);
+        t1 += sums[2 * r];
+        t2 += sums[2 * r + 1];
+    }
+    if (g) {
+        g[0] = t1 / ((double)N * Nmc);
+        g[1] = std::sqrt(t2 / ((double)N * Nmc) - g[0] * g[0]);
+    }
+    return RF_OK;
+}
+
+int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N, int32_t Nmc, int32_t dtype,
+                        void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (N < 1 || Nmc < 1 || !d_field) return rf_set_error(ctx, RF_E_ARG, "bad field");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    std::vector<double> sums;
+    int rc = moments(ctx, d_field, N, Nmc, dtype, s, &sums);
+    if (rc) return rc;
+    // randomFieldND.f90:869-875: subtract the mean
+    std::vector<double> hv((size_t)Nmc * 2);
+    for (int r = 0; r < Nmc; r++) { hv[r] = sums[2 * r] / (double)N; hv[Nmc + r] = 1.0; }
+    double* dv = nullptr;
+    RF_HIP_TRY(ctx, dmalloc(&dv, hv.size()));
+    hipError_t e = hipMemcpyAsync(dv, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = rf_launch_affine(d_field, N, Nmc, dtype, dv, dv + Nmc, s);
+    if (e != hipSuccess) { dfree(dv); return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e)); }
+    rc = moments(ctx, d_field, N, Nmc, dtype, s, &sums);
+    if (rc) { dfree(dv); return rc; }
+    // :904-908: divide by the root mean square
+    for (int r = 0; r < Nmc; r++) { hv[r] = 0.0; hv[Nmc + r] = 1.0 / std::sqrt(sums[2 * r + 1] / (double)N); }
+    e = hipMemcpyAsync(dv, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = rf_launch_affine(d_field, N, Nmc, dtype, dv, dv + Nmc, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(dv);
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    return RF_OK;
+}
+
+int rf_device_uniforms(rf_context* ctx, uint64_t seed, uint32_t tag, int64_t count, double* d_out,
+                       void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (count < 0 || (count > 0 && !d_out)) return rf_set_error(ctx, RF_E_ARG, "bad output");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    RF_HIP_TRY(ctx, rf_launch_uniforms(seed, tag, count, d_out, stream ? (hipStream_t)stream : ctx->stream));
+    return RF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// rf_internal.h -- shared declarations of librf_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rf_capi.h"
+
+// constants_RF.f90:5 -- the reference's pi literal (phases, spectra, tapers).
+#define RF_PI_REF 3.1415926535898
+// exact 2*pi and its reciprocal, for reductions to revolutions
+#define RF_TWO_PI_EXACT 6.283185307179586476925286766559
+#define RF_INV_TWO_PI 0.15915494309189533576888376337251
+
+struct rf_context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int num_cu = 0;
+    void* d_scratch = nullptr;   // reduction partials (grown on demand)
+    size_t scratch_bytes = 0;
+};
+
+// Host-side spectrum: the wavevector table in coordinates normalised by corrL.
+struct rf_spectrum {
+    rf_spectrum_info info{};
+    std::vector<double> k;    // nDim x kNTotal (isotropic: radii)
+    std::vector<double> Sk;   // kNTotal
+};
+
+// Fused epilogue: optional marginal map (randomFieldND.f90:171-176).
+struct rf_epilogue {
+    int margi;      // RF_FOM_NONE / GAUSSIAN / LOGNORMAL
+    double scale;   // sqrt(normalVar)
+    double shift;   // normalAvg
+};
+
+struct rf_field_plan {
+    rf_context* ctx = nullptr;
+    rf_params p{};
+    rf_spectrum spec;
+    std::vector<uint8_t> calc;        // Nmc flags
+    int nCalc = 0;
+    // device buffers (owned)
+    double* d_k = nullptr;       // shared k: kN x nDim (normalised units)
+    double* d_kRev = nullptr;    // shared k / (2 pi)
+    double* d_kIso = nullptr;    // per-realization k: Nmc x kN x nDim
+    double* d_kIsoT = nullptr;   // FP32 isotropic form: kN x Rpad x nDim, revolutions
+    double* d_Sk = nullptr;      // isotropic spectral density
+    uint8_t* d_calc = nullptr;   // Nmc flags
+    double* d_phase = nullptr;   // Nmc x kN radians (2*pi_ref*U or psi)
+    double* d_amp = nullptr;     // Nmc x kN amplitudes (before ampMult)
+    double* d_uni = nullptr;     // device copy of the phase uniforms
+    float* d_phT = nullptr;      // kN x Rpad   FP32 phase in revolutions
+    float* d_aT = nullptr;       // kN x Rpad   FP32 amplitude * ampMult
+    void* d_B = nullptr;         // MFMA coefficient panels (hi, lo)
+    int Rpad = 0;
+    int64_t kNpad = 0;
+    double bScale = 1.0;         // power-of-two scale folded into B
+    // REF_COMPAT: sqrt chain of Sk, row c = Sk^(2^-(c+1)), until it saturates
+    std::vector<std::vector<double>> sqrtChain;
+    bool coeffsReady = false;    // FIXED: coefficients independent of N
+    rf_epilogue epi{};
+};
+
+// ---- host spectrum (rf_spectrum.cpp)
+int rf_spectrum_build_info(const rf_params* p, rf_spectrum_info* info, std::string* err);
+int rf_spectrum_build_tables(const rf_params* p, const rf_spectrum_info& info,
+                             const double* kUniform, rf_spectrum* spec, std::string* err);
+int rf_validate_params(const rf_params* p, std::string* err);
+
+// ---- kernel launchers (rf_sum.hip, rf_prep.hip, rf_mfma.hip, rf_post.hip)
+hipError_t rf_launch_uniforms(uint64_t seed, uint32_t tag, int64_t count, double* d_out,
+                              hipStream_t s);
+hipError_t rf_launch_phase_from_uniform(const double* d_u, int64_t n, double* d_phase,
+                                        hipStream_t s);
+hipError_t rf_launch_iso_prep(int nDim, int mode, int shared, int64_t kN, int Nmc,
+                              const double* d_kRad, const double* d_Sk, const double* d_u,
+                              const uint8_t* d_calc, double* d_kIso, double* d_amp,
+                              double* d_phase, hipStream_t s);
+hipError_t rf_launch_prep_fp32(int64_t kN, int Nmc, int Rpad, const double* d_phase,
+                               const double* d_amp, double ampMult, float* d_phT, float* d_aT,
+                               hipStream_t s);
+hipError_t rf_launch_krev(int64_t count, const double* d_k, double* d_kRev, hipStream_t s);
+
+struct rf_sum_args {
+    const double* x;      // nDim x N caller units
+    int64_t N;
+    int64_t i0, i1;       // point range of this launch
+    double invL[3];       // 1/corrL ... FP64 kernel divides instead (see kernel)
+    double corrL[3];
+    int nDim;
+    int Nmc;
+    int64_t kN;
+    double ampMult;
+    rf_epilogue epi;
+    void* out;
+    int outDtype;
+};
+hipError_t rf_launch_sum_fp64(const rf_sum_args& a, const double* d_k, int64_t kStrideR,
+                              const double* d_amp, const double* d_phase, hipStream_t s);
+hipError_t rf_launch_sum_fp32(const rf_sum_args& a, const double* d_kRev, const float* d_phT,
+                              const float* d_aT, int Rpad, hipStream_t s);
+hipError_t rf_launch_sum_fp32_iso_t(const rf_sum_args& a, const double* d_kT, const float* d_pT,
+                                    const float* d_aT, int Rpad, hipStream_t s);
+hipError_t rf_launch_prep_fp32_iso(int nDim, int64_t kN, int Nmc, int Rpad, const double* d_kIso,
+                                   const double* d_phase, const double* d_amp, double ampMult,
+                                   double* d_kT, float* d_pT, float* d_aT, hipStream_t s);
+int rf_fp32_rchunk();
+
+// MFMA form (rf_mfma.hip)
+int rf_mfma_kpad(int form);
+hipError_t rf_launch_mfma_prep(int form, int64_t kN, int64_t kNpad, int Nmc, int Npad,
+                               const double* d_phase, const double* d_amp, double scale,
+                               void* d_B, hipStream_t s);
+hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, const double* d_kRev,
+                              int64_t kNpad, int Npad, const void* d_B, double outScale,
+                              hipStream_t s);
+int rf_mfma_npad(int Nmc);
+size_t rf_mfma_bbytes(int form, int64_t kNpad, int Npad);
+
+// post-processing (rf_post.hip)
+hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, int64_t count,
+                              int dtype, hipStream_t s);
+
+// utilities
+int rf_set_error(rf_context* ctx, int code, const std::string& msg);
+#define RF_HIP_TRY(ctx, expr)                                                        \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess)                                                        \
+            return rf_set_error((ctx), RF_E_HIP,                                     \
+                                std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)

@@ -1,0 +1,252 @@
+// rf_mfma.hip -- the spectral sum as a dense contraction on the matrix cores.
+//
+//   u[r][i] = sum_j  C[r][j] cos(2 pi t_ij) + S[r][j] sin(2 pi t_ij),
+//   t_ij = (k_j / 2pi) . x_i  (revolutions),  C = A cos(phi), S = -A sin(phi)
+//
+// i.e. D (realizations x points) = P (realizations x 2N_k) . Phi (2N_k x points),
+// the rewrite of randomFieldND.f90:276-330 with the realization loop batched.
+//
+// * Phi, the basis, is never stored: every wave produces the fragments it
+//   consumes.  Lane l owns point l%32 and the wavevectors 4h..4h+3 (h = l/32)
+//   of each 8-wavevector K-step; it forms k.x in FP64, reduces it to
+//   revolutions, takes the hardware sin/cos (v_sin_f32/v_cos_f32 take
+//   revolutions) and splits each value into hi + lo halves.
+// * P, the coefficient panels, is built once per plan in FP64 and stored
+//   pre-split (hi, lo), pre-scaled by a power of two, in exactly the per-lane
+//   order of the MFMA A operand; panels go global -> LDS (double buffered) ->
+//   registers.  K order inside a step is (wavevector q, cos/sin), identical on
+//   both operands, so any hardware K permutation cancels.
+// * D += P_hi Phi_hi + P_hi Phi_lo + P_lo Phi_hi with FP32 accumulation
+//   (v_mfma_f32_32x32x16_{f16,bf16}); the dropped lo.lo term is 2^-22 (FP16)
+//   relative.  A wave holds 32 points x (32*MT) realizations: 16*MT
+//   accumulators per lane.
+#include "rf_internal.h"
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f16x __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;               // waves per workgroup
+constexpr int kThreads = 64 * kWaves;
+constexpr int kPointsPerBlock = 32 * kWaves;
+constexpr int kMaxMT = 8;               // up to 256 realizations per block
+constexpr int kStepK = 8;               // wavevectors per K-step (16 K entries)
+
+template <typename V> struct Elem;
+template <> struct Elem<h8> { typedef _Float16 T; };
+template <> struct Elem<b8> { typedef __bf16 T; };
+
+__device__ inline f16x mfma(const h8& a, const h8& b, const f16x& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+__device__ inline f16x mfma(const b8& a, const b8& b, const f16x& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+template <typename V>
+__device__ inline void split(float v, typename Elem<V>::T* hi, typename Elem<V>::T* lo) {
+    typedef typename Elem<V>::T T;
+    const T h = (T)v;
+    *hi = h;
+    *lo = (T)(v - (float)h);
+}
+
+__device__ inline float epi32(float u, const rf_epilogue& e) {
+    if (e.margi == RF_FOM_NONE) return u;
+    const float y = fmaf(u, (float)e.scale, (float)e.shift);
+    return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
+}
+
+// Panel layout (16-byte units): [step][mtile][hilo][lane] -> 8 elements.
+template <typename V, int D, int MT>
+__global__ __launch_bounds__(kThreads) void k_sum_mfma(rf_sum_args a, const double* __restrict__ kRev,
+                                                       int64_t nSteps, int mtTotal,
+                                                       const u32x4* __restrict__ panels,
+                                                       float outScale) {
+    typedef typename Elem<V>::T T;
+    __shared__ u32x4 lds[2][MT * 2 * 64];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int h = lane >> 5;
+    const int n = lane & 31;
+    const int mt0 = blockIdx.y * kMaxMT;            // first M-tile of this block
+    const int64_t pt = a.i0 + (int64_t)blockIdx.x * kPointsPerBlock + wave * 32 + n;
+    const int64_t ptc = pt < a.i1 ? pt : a.i1 - 1;
+
+    double xn[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xn[d] = a.x[ptc * D + d] / a.corrL[d];
+
+    f16x acc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+
+    // cooperative panel copy: MT*2*64 16-byte units per step
+    constexpr int kUnits = MT * 2 * 64;
+    auto load_step = [&](int64_t t, int buf) {
+        const u32x4* src = panels + ((t * mtTotal + mt0) * 2) * 64;
+        for (int u = tid; u < kUnits; u += kThreads) lds[buf][u] = src[u];
+    };
+
+    load_step(0, 0);
+    for (int64_t t = 0; t < nSteps; t++) {
+        const int buf = (int)(t & 1);
+        // basis fragments of this lane: wavevectors 8t + 4h + q
+        V bh, bl;
+        const double* kt = kRev + (t * kStepK) * D;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double s = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double k0 = kt[q * D + d];
+                const double k1 = kt[(4 + q) * D + d];
+                s = fma(h ? k1 : k0, xn[d], s);
+            }
+            s = s - rint(s);
+            const float f = (float)s;
+            T ch, cl, sh, sl;
+            split<V>(__builtin_amdgcn_cosf(f), &ch, &cl);
+            split<V>(__builtin_amdgcn_sinf(f), &sh, &sl);
+            bh[2 * q] = ch; bh[2 * q + 1] = sh;
+            bl[2 * q] = cl; bl[2 * q + 1] = sl;
+        }
+        __syncthreads();                            // step t resident in lds[buf]
+        if (t + 1 < nSteps) load_step(t + 1, buf ^ 1);
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const u32x4 ph = lds[buf][(m * 2 + 0) * 64 + lane];
+            const u32x4 pl = lds[buf][(m * 2 + 1) * 64 + lane];
+            const V ahi = __builtin_bit_cast(V, ph);
+            const V alo = __builtin_bit_cast(V, pl);
+            acc[m] = mfma(ahi, bh, acc[m]);
+            acc[m] = mfma(ahi, bl, acc[m]);
+            acc[m] = mfma(alo, bh, acc[m]);
+        }
+    }
+    if (pt >= a.i1) return;
+    // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int real = (mt0 + m) * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            if (real < a.Nmc) {
+                const float u = epi32(acc[m][r] * outScale, a.epi);
+                if (a.outDtype == RF_DTYPE_F64)
+                    static_cast<double*>(a.out)[(int64_t)real * a.N + pt] = (double)u;
+                else
+                    static_cast<float*>(a.out)[(int64_t)real * a.N + pt] = u;
+            }
+        }
+    }
+}
+
+// Coefficient panels from FP64 phases and amplitudes.  One thread per
+// (step, mtile, lane, element); both hilo halves written.
+template <typename T>
+__global__ void k_mfma_prep(int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
+                            const double* __restrict__ phase, const double* __restrict__ amp,
+                            double scale, T* __restrict__ panels) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t total = nSteps * mtTotal * 64 * 8;
+    if (e >= total) return;
+    const int el = (int)(e & 7);
+    const int lane = (int)((e >> 3) & 63);
+    const int64_t rest = e >> 9;
+    const int mt = (int)(rest % mtTotal);
+    const int64_t t = rest / mtTotal;
+    const int h = lane >> 5;
+    const int r = mt * 32 + (lane & 31);
+    const int q = el >> 1;
+    const int64_t j = t * kStepK + 4 * h + q;
+    double v = 0.0;
+    if (r < Nmc && j < kN) {
+        const double A = amp[(int64_t)r * kN + j] * scale;
+        const double ph = phase[(int64_t)r * kN + j];
+        v = (el & 1) ? -A * sin(ph) : A * cos(ph);
+    }
+    const float vf = (float)v;
+    const T hi = (T)vf;
+    const T lo = (T)(float)(v - (double)(float)hi);
+    const int64_t base = ((t * mtTotal + mt) * 2) * 64 * 8;
+    panels[base + (int64_t)lane * 8 + el] = hi;
+    panels[base + 64 * 8 + (int64_t)lane * 8 + el] = lo;
+}
+
+template <typename V, int D>
+hipError_t launch_mt(int mt, const rf_sum_args& a, const double* kRev, int64_t nSteps, int mtTotal,
+                     const void* B, float outScale, dim3 grid, hipStream_t s) {
+    const u32x4* P = static_cast<const u32x4*>(B);
+    switch (mt) {
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_form(const rf_sum_args& a, const double* kRev, int64_t kNpad, int Npad,
+                       const void* B, double outScale, hipStream_t s) {
+    const int mtTotal = Npad / 32;
+    const int mt = mtTotal >= kMaxMT ? kMaxMT : mtTotal;
+    const int64_t n = a.i1 - a.i0;
+    dim3 grid((unsigned)((n + kPointsPerBlock - 1) / kPointsPerBlock),
+              (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
+    const int64_t nSteps = kNpad / kStepK;
+    switch (a.nDim) {
+        case 1: return launch_mt<V, 1>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
+        case 2: return launch_mt<V, 2>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
+        default: return launch_mt<V, 3>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
+    }
+}
+
+}  // namespace
+
+int rf_mfma_kpad(int) { return kStepK; }
+
+// Realizations are padded to whole 32-row tiles; above 256 to whole blocks of
+// 8 tiles, and below that to a power-of-two tile count (1, 2, 4, 8).
+int rf_mfma_npad(int Nmc) {
+    int mt = (Nmc + 31) / 32;
+    if (mt > kMaxMT) return ((mt + kMaxMT - 1) / kMaxMT) * kMaxMT * 32;
+    int p = 1;
+    while (p < mt) p <<= 1;
+    return p * 32;
+}
+
+size_t rf_mfma_bbytes(int, int64_t kNpad, int Npad) {
+    return (size_t)(kNpad / kStepK) * (size_t)(Npad / 32) * 2 * 64 * 16;
+}
+
+hipError_t rf_launch_mfma_prep(int form, int64_t kN, int64_t kNpad, int Nmc, int Npad,
+                               const double* d_phase, const double* d_amp, double scale,
+                               void* d_B, hipStream_t s) {
+    const int64_t nSteps = kNpad / kStepK;
+    const int mtTotal = Npad / 32;
+    const int64_t total = nSteps * mtTotal * 64 * 8;
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (form == RF_FORM_MFMA_BF16)
+        hipLaunchKernelGGL(k_mfma_prep<__bf16>, dim3(grid), dim3(256), 0, s, kN, nSteps, Nmc, mtTotal,
+                           d_phase, d_amp, scale, static_cast<__bf16*>(d_B));
+    else
+        hipLaunchKernelGGL(k_mfma_prep<_Float16>, dim3(grid), dim3(256), 0, s, kN, nSteps, Nmc, mtTotal,
+                           d_phase, d_amp, scale, static_cast<_Float16*>(d_B));
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, const double* d_kRev, int64_t kNpad,
+                              int Npad, const void* d_B, double outScale, hipStream_t s) {
+    if (a.i1 <= a.i0) return hipSuccess;
+    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(a, d_kRev, kNpad, Npad, d_B, outScale, s);
+    return launch_form<h8>(a, d_kRev, kNpad, Npad, d_B, outScale, s);
+}

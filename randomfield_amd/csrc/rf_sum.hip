@@ -1,0 +1,195 @@
+// rf_sum.hip -- direct spectral sums u_r(x) = ampMult * sum_j a_rj cos(k_j.x + phi_rj)
+// (randomFieldND.f90:184-337 methods 2/3, :368-514 method 1).
+//
+// Two precision forms:
+//  * FP64: one thread per (point, realization), sequential sum over j in the
+//    reference's order with FMA contraction off -- the on-GPU oracle form.
+//  * FP32: one thread per point holding R realization accumulators.  The
+//    argument k.x is formed in FP64, reduced to revolutions in [-1/2, 1/2)
+//    once per (point, k) and reused for all R realizations; each term is then
+//    one FP32 add, one hardware cosine (v_cos_f32 takes revolutions) and one
+//    FMA.  Wavevector, phase and amplitude operands are wave-uniform and come
+//    through scalar loads, so the inner loop is pure VALU/transcendental work.
+#include "rf_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T>
+__device__ inline void store_out(void* out, int64_t idx, T v, int dtype) {
+    if (dtype == RF_DTYPE_F64) static_cast<double*>(out)[idx] = (double)v;
+    else static_cast<float*>(out)[idx] = (float)v;
+}
+
+// randomFieldND.f90:171-176 fused after the sum.
+__device__ inline double epilogue64(double u, const rf_epilogue& e) {
+#pragma clang fp contract(off)
+    if (e.margi == RF_FOM_NONE) return u;
+    const double y = u * e.scale + e.shift;
+    return (e.margi == RF_FOM_LOGNORMAL) ? exp(y) : y;
+}
+__device__ inline float epilogue32(float u, const rf_epilogue& e) {
+    if (e.margi == RF_FOM_NONE) return u;
+    const float y = fmaf(u, (float)e.scale, (float)e.shift);
+    return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
+}
+
+// ---------------------------------------------------------------- FP64 form
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sum_fp64(rf_sum_args a, const double* __restrict__ k,
+                                                     int64_t kStrideR,
+                                                     const double* __restrict__ amp,
+                                                     const double* __restrict__ phase) {
+#pragma clang fp contract(off)
+    const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= a.i1) return;
+    double xn[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xn[d] = a.x[i * D + d] / a.corrL[d];   // :59
+    const double* kr = k + (int64_t)r * kStrideR;
+    const double* am = amp + (int64_t)r * a.kN;
+    const double* ph = phase + (int64_t)r * a.kN;
+    double acc = 0.0;
+    for (int64_t j = 0; j < a.kN; j++) {
+        // dgemm.f90:246-253 (k.x then + phi), :310 cos, dgemm.f90:226-234
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) t = t + xn[d] * kr[j * D + d];
+        const double c = cos(t + ph[j]);
+        const double aj = am[j];
+        if (aj != 0.0) acc = acc + aj * c;
+    }
+    const double u = epilogue64(a.ampMult * acc, a.epi);               // :330
+    store_out(a.out, (int64_t)r * a.N + i, u, a.outDtype);
+}
+
+// ---------------------------------------------------------------- FP32 form
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double* __restrict__ kRev,
+                                                     const float* __restrict__ phT,
+                                                     const float* __restrict__ aT, int Rpad) {
+    const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r0 = blockIdx.y * R;
+    if (i >= a.i1) return;
+    double xn[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xn[d] = a.x[i * D + d] / a.corrL[d];
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
+    const float* ph = phT + r0;
+    const float* am = aT + r0;
+    for (int64_t j = 0; j < a.kN; j++) {
+        const double* kj = kRev + j * D;
+        double t = kj[0] * xn[0];
+#pragma unroll
+        for (int d = 1; d < D; d++) t = fma(kj[d], xn[d], t);
+        t = t - rint(t);                 // k.x mod 2pi, in revolutions
+        const float tf = (float)t;
+        const float* phj = ph + j * Rpad;
+        const float* amj = am + j * Rpad;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            acc[r] = fmaf(amj[r], __builtin_amdgcn_cosf(tf + phj[r]), acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (r0 + r < a.Nmc)
+            store_out(a.out, (int64_t)(r0 + r) * a.N + i, epilogue32(acc[r], a.epi), a.outDtype);
+    }
+}
+
+// FP32 form of the isotropic sum with per-realization directions: the
+// argument cannot be shared across realizations, so each term forms its own
+// FP64 k.x.  kT[j][r][D] in revolutions, aT/pT[j][r] as in k_sum_fp32.
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const double* __restrict__ kT,
+                                                         const float* __restrict__ pT,
+                                                         const float* __restrict__ aT, int Rpad) {
+    const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r0 = blockIdx.y * R;
+    if (i >= a.i1) return;
+    double xn[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xn[d] = a.x[i * D + d] / a.corrL[d];
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0f;
+    for (int64_t j = 0; j < a.kN; j++) {
+        const double* kj = kT + (j * Rpad + r0) * D;
+        const float* pj = pT + j * Rpad + r0;
+        const float* aj = aT + j * Rpad + r0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double t = kj[r * D] * xn[0];
+#pragma unroll
+            for (int d = 1; d < D; d++) t = fma(kj[r * D + d], xn[d], t);
+            t = t - rint(t);
+            acc[r] = fmaf(aj[r], __builtin_amdgcn_cosf((float)t + pj[r]), acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (r0 + r < a.Nmc)
+            store_out(a.out, (int64_t)(r0 + r) * a.N + i, epilogue32(acc[r], a.epi), a.outDtype);
+    }
+}
+
+template <int D>
+hipError_t launch_fp64(const rf_sum_args& a, const double* k, int64_t ks, const double* amp,
+                       const double* ph, hipStream_t s) {
+    const int64_t n = a.i1 - a.i0;
+    dim3 grid((unsigned)((n + kBlock - 1) / kBlock), (unsigned)a.Nmc);
+    hipLaunchKernelGGL(k_sum_fp64<D>, grid, dim3(kBlock), 0, s, a, k, ks, amp, ph);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Realization chunk of the FP32 kernels (accumulators per thread).
+constexpr int kRChunk = 16;
+
+hipError_t rf_launch_sum_fp64(const rf_sum_args& a, const double* d_k, int64_t kStrideR,
+                              const double* d_amp, const double* d_phase, hipStream_t s) {
+    if (a.i1 <= a.i0) return hipSuccess;
+    switch (a.nDim) {
+        case 1: return launch_fp64<1>(a, d_k, kStrideR, d_amp, d_phase, s);
+        case 2: return launch_fp64<2>(a, d_k, kStrideR, d_amp, d_phase, s);
+        default: return launch_fp64<3>(a, d_k, kStrideR, d_amp, d_phase, s);
+    }
+}
+
+hipError_t rf_launch_sum_fp32(const rf_sum_args& a, const double* d_kRev, const float* d_phT,
+                              const float* d_aT, int Rpad, hipStream_t s) {
+    if (a.i1 <= a.i0) return hipSuccess;
+    const int64_t n = a.i1 - a.i0;
+    dim3 grid((unsigned)((n + kBlock - 1) / kBlock), (unsigned)(Rpad / kRChunk));
+    switch (a.nDim) {
+        case 1:
+            hipLaunchKernelGGL((k_sum_fp32<1, kRChunk>), grid, dim3(kBlock), 0, s, a, d_kRev, d_phT, d_aT, Rpad);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_sum_fp32<2, kRChunk>), grid, dim3(kBlock), 0, s, a, d_kRev, d_phT, d_aT, Rpad);
+            break;
+        default:
+            hipLaunchKernelGGL((k_sum_fp32<3, kRChunk>), grid, dim3(kBlock), 0, s, a, d_kRev, d_phT, d_aT, Rpad);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_sum_fp32_iso_t(const rf_sum_args& a, const double* d_kT, const float* d_pT,
+                                    const float* d_aT, int Rpad, hipStream_t s) {
+    if (a.i1 <= a.i0) return hipSuccess;
+    const int64_t n = a.i1 - a.i0;
+    dim3 grid((unsigned)((n + kBlock - 1) / kBlock), (unsigned)(Rpad / kRChunk));
+    if (a.nDim == 2)
+        hipLaunchKernelGGL((k_sum_fp32_iso<2, kRChunk>), grid, dim3(kBlock), 0, s, a, d_kT, d_pT, d_aT, Rpad);
+    else
+        hipLaunchKernelGGL((k_sum_fp32_iso<3, kRChunk>), grid, dim3(kBlock), 0, s, a, d_kT, d_pT, d_aT, Rpad);
+    return hipGetLastError();
+}
+
+int rf_fp32_rchunk() { return kRChunk; }

@@ -1,0 +1,572 @@
+!> randomField_gpu -- Fortran calling surface of the gfx950 spectral random-field
+!> path (librf_hip.so), for hosts written against randomField.
+!>
+!> Replace `use calls_RF` (or `use randomFieldND`) by `use randomField_gpu`:
+!>   * type(RF) / type(MESH) carry the components the generation path reads
+!>     and writes (type_RF.f90:11-55, type_MESH.f90:9-54);
+!>   * createRandomField(RDF, MSH) is the generic of create_RF_Unstruct_Init
+!>     (calls_RF.f90:20-22, randomFieldND.f90:27-132);
+!>   * createStandardGaussianFieldUnstruct is the README's front door
+!>     (README.md:58-88), which the reference documents but does not define;
+!>   * multiVariateTransformation (randomFieldND.f90:139-178),
+!>     calculate_random_seed / init_random_seed (math_RF.f90:475-566).
+!>
+!> The uniforms are drawn here, on the host, with the compiler's random_number
+!> after random_seed(PUT=RDF%seed), in the reference's order (wavevector
+!> draws of method 3 first, spectra_RF.f90:120; then per realization:
+!> randomFieldND.f90:278 for methods 2/3, :424-426 / :460-463 for method 1),
+!> so a given seed gives the reference's phases with the same compiler.
+!> The sums, the marginal map and everything O(N) run on the GPU.
+!> Failures stop with the library's message, as the reference stops.
+module randomField_gpu
+    use iso_c_binding
+    implicit none
+    private
+
+    ! constants_RF.f90:5-21
+    double precision, parameter, public :: PI = 3.1415926535898d0
+    integer, parameter, public :: ISOTROPIC = 1, SHINOZUKA = 2, RANDOMIZATION = 3, FFT = 4
+    integer, parameter, public :: cm_GAUSSIAN = 1, cm_COS = 2
+    integer, parameter, public :: fom_GAUSSIAN = 1, fom_LOGNORMAL = 2
+    ! GPU path controls (no counterpart in the reference)
+    integer, parameter, public :: RF_REF_COMPAT = 0, RF_FIXED = 1
+    integer, parameter, public :: RF_FORM_FP64 = 0, RF_FORM_FP32 = 1, &
+                                  RF_FORM_MFMA_F16 = 2, RF_FORM_MFMA_BF16 = 3
+    integer, parameter, public :: RF_KSAMP_BOX = 0, RF_KSAMP_DENSITY = 1
+
+    type, bind(C), public :: rf_params_c
+        integer(c_int32_t) :: nDim = 0, method = 0, corrMod = 0, mode = 0
+        integer(c_int32_t) :: form = 0, kSampling = 0, Nmc = 0, margiFirst = 0
+        integer(c_int64_t) :: nKDensity = 0, xSlab = 0
+        real(c_double)     :: corrL(3) = 1d0, xRange(3) = 1d0
+        real(c_double)     :: fieldAvg = 0d0, fieldVar = 1d0
+        integer(c_int64_t) :: seed = 0
+        integer(c_int32_t) :: isoShared = 0
+        integer(c_int32_t) :: reserved(7) = 0
+    end type rf_params_c
+
+    type, bind(C), public :: rf_spectrum_info_c
+        integer(c_int64_t) :: kNTotal, kN_unsigned, nKUniform, nPhaseUniform
+        integer(c_int32_t) :: kNStep(3), nSignBlocks, nVarIso, pad
+        real(c_double)     :: kMax(3), kDelta(3), ampMult, isoStep
+    end type rf_spectrum_info_c
+
+    !> type_RF.f90:11-55 (generation components)
+    type, public :: RF
+        integer :: nDim = -1
+        integer(kind=8) :: kNTotal = -1
+        integer, dimension(:), allocatable :: seed
+        integer :: seedStart = -1
+        double precision :: fieldAvg = -1, fieldVar = -1
+        integer :: corrMod = -1
+        integer :: margiFirst = -1
+        integer :: method = -1
+        integer :: Nmc = -1
+        logical :: init = .false.
+        double precision, dimension(:), allocatable :: corrL, kMax, kDelta
+        double precision, dimension(:, :), allocatable :: kPoints
+        double precision, dimension(:), allocatable :: SkVec
+        double precision, dimension(:), allocatable :: xRange
+        integer, dimension(:), allocatable :: kNStep
+        double precision, pointer :: xPoints(:,:) => null()
+        double precision, pointer :: randField(:,:) => null()
+        double precision, dimension(:,:), pointer :: RF_2D => null()
+        double precision, dimension(:,:,:), pointer :: RF_3D => null()
+        logical, dimension(:), allocatable :: calculate
+        ! GPU path controls
+        integer :: compatMode = RF_REF_COMPAT   !< RF_REF_COMPAT or RF_FIXED
+        integer :: form = RF_FORM_FP64          !< precision form of the sum
+        integer :: kSampling = RF_KSAMP_BOX     !< method 3 wavevector sampling
+        integer(kind=8) :: nKDensity = 0        !< N_k for RF_KSAMP_DENSITY
+        logical :: isoShared = .false.          !< method 1: shared directions
+        logical :: deviceDraws = .false.        !< device generator, not random_number
+        integer(kind=8) :: deviceSeed = 0
+        integer :: device = 0
+    end type RF
+
+    !> type_MESH.f90:9-54 (the components gen_Std_Gauss reads and rescales)
+    type, public :: MESH
+        integer :: nDim = -1
+        integer(kind=8) :: xNTotal = -1
+        integer, dimension(:), allocatable :: xNStep
+        double precision, dimension(:), allocatable :: xStep, xMinGlob, xMaxGlob
+        double precision, dimension(:), allocatable :: xMinInt, xMaxInt, xMinExt, xMaxExt
+        double precision, dimension(:), allocatable :: xMinBound, xMaxBound
+        logical :: init = .false.
+    end type MESH
+
+    interface createRandomField
+        module procedure create_RF_Unstruct_Init
+    end interface createRandomField
+
+    interface createStandardGaussianFieldUnstruct
+        module procedure createStdGaussUnstruct_str, createStdGaussUnstruct_int
+    end interface createStandardGaussianFieldUnstruct
+
+    public :: createRandomField, create_RF_Unstruct_Init, createStandardGaussianFieldUnstruct
+    public :: multiVariateTransformation, calculate_random_seed, init_random_seed
+    public :: init_RF, finalize_RF, init_MESH, finalize_MESH, rf_draw_sequence
+    public :: rf_c_params, rf_check, rf_ctx
+
+    interface
+        integer(c_int) function c_rf_context_create(ctx, device) bind(C, name="rf_context_create")
+            import :: c_int, c_ptr
+            type(c_ptr) :: ctx
+            integer(c_int), value :: device
+        end function
+        function c_rf_last_error(ctx) bind(C, name="rf_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+            type(c_ptr) :: c_rf_last_error
+        end function
+        integer(c_int) function c_rf_plan_spectrum(p, info) bind(C, name="rf_plan_spectrum")
+            import :: c_int, rf_params_c, rf_spectrum_info_c
+            type(rf_params_c), intent(in) :: p
+            type(rf_spectrum_info_c), intent(out) :: info
+        end function
+        integer(c_int) function c_rf_spectrum_tables(p, kU, kPts, Sk) bind(C, name="rf_spectrum_tables")
+            import :: c_int, c_ptr, rf_params_c
+            type(rf_params_c), intent(in) :: p
+            type(c_ptr), value :: kU, kPts, Sk
+        end function
+        integer(c_int) function c_rf_generate(ctx, p, x, N, kU, phU, calc, out, dtype) &
+                bind(C, name="rf_generate")
+            import :: c_int, c_int32_t, c_int64_t, c_ptr, rf_params_c
+            type(c_ptr), value :: ctx
+            type(rf_params_c), intent(in) :: p
+            type(c_ptr), value :: x, kU, phU, calc, out
+            integer(c_int64_t), value :: N
+            integer(c_int32_t), value :: dtype
+        end function
+        integer(c_int) function c_rf_marginal_host(ctx, margi, avg, var, field, count) &
+                bind(C, name="rf_marginal_host")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: margi
+            real(c_double), value :: avg, var
+            type(c_ptr), value :: field
+            integer(c_int64_t), value :: count
+        end function
+    end interface
+
+    type(c_ptr), save :: g_ctx = c_null_ptr
+    integer, save :: g_ctx_device = -1
+
+contains
+
+    !> The process-wide context on `device` (created on first use).
+    function rf_ctx(device) result(ctx)
+        integer, intent(in) :: device
+        type(c_ptr) :: ctx
+        integer(c_int) :: rc
+        if (.not. c_associated(g_ctx) .or. g_ctx_device /= device) then
+            rc = c_rf_context_create(g_ctx, int(device, c_int))
+            call rf_check(rc, "rf_context_create")
+            g_ctx_device = device
+        end if
+        ctx = g_ctx
+    end function rf_ctx
+
+    !> Stops with the library's message on a nonzero status.
+    subroutine rf_check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        type(c_ptr) :: msg
+        character(kind=c_char), pointer :: chars(:)
+        character(len=512) :: text
+        integer :: i
+        if (rc == 0) return
+        msg = c_rf_last_error(g_ctx)
+        text = ""
+        if (c_associated(msg)) then
+            call c_f_pointer(msg, chars, [512])
+            do i = 1, 512
+                if (chars(i) == c_null_char) exit
+                text(i:i) = chars(i)
+            end do
+        end if
+        write(*,*) "ERROR in ", trim(what), " (status ", rc, "): ", trim(text)
+        error stop 1
+    end subroutine rf_check
+
+    !> init_RF (type_RF.f90:62-126) without the IPT_RF input record.
+    subroutine init_RF(RDF, nDim, Nmc, method, corrMod, margiFirst, corrL, fieldAvg, fieldVar, seedStart)
+        type(RF), intent(inout) :: RDF
+        integer, intent(in) :: nDim, Nmc, method, corrMod, margiFirst
+        double precision, intent(in) :: corrL(:), fieldAvg, fieldVar
+        integer, intent(in), optional :: seedStart
+        integer :: n
+        call finalize_RF(RDF)
+        allocate(RDF%corrL(nDim), RDF%kMax(nDim), RDF%kNStep(nDim), RDF%kDelta(nDim))
+        allocate(RDF%calculate(Nmc), RDF%xRange(nDim))
+        call random_seed(size = n)
+        allocate(RDF%seed(n))
+        RDF%nDim = nDim
+        RDF%Nmc = Nmc
+        RDF%method = method
+        RDF%corrMod = corrMod
+        RDF%margiFirst = margiFirst
+        RDF%corrL = corrL(1:nDim)
+        RDF%fieldAvg = fieldAvg
+        RDF%fieldVar = fieldVar
+        RDF%seedStart = -1
+        if (present(seedStart)) RDF%seedStart = seedStart
+        RDF%xRange = -1.0D0
+        RDF%kMax = -1
+        RDF%kDelta = -1
+        RDF%seed = -1
+        RDF%kNStep = -1
+        RDF%calculate(:) = .true.
+        RDF%init = .true.
+        nullify(RDF%xPoints, RDF%randField, RDF%RF_2D, RDF%RF_3D)
+    end subroutine init_RF
+
+    subroutine finalize_RF(RDF)
+        type(RF), intent(inout) :: RDF
+        if (allocated(RDF%corrL)) deallocate(RDF%corrL)
+        if (allocated(RDF%kMax)) deallocate(RDF%kMax)
+        if (allocated(RDF%kNStep)) deallocate(RDF%kNStep)
+        if (allocated(RDF%kDelta)) deallocate(RDF%kDelta)
+        if (allocated(RDF%calculate)) deallocate(RDF%calculate)
+        if (allocated(RDF%xRange)) deallocate(RDF%xRange)
+        if (allocated(RDF%seed)) deallocate(RDF%seed)
+        if (allocated(RDF%kPoints)) deallocate(RDF%kPoints)
+        if (allocated(RDF%SkVec)) deallocate(RDF%SkVec)
+        nullify(RDF%xPoints, RDF%randField, RDF%RF_2D, RDF%RF_3D)
+        RDF%init = .false.
+    end subroutine finalize_RF
+
+    !> init_MESH (type_MESH.f90:61-142) without the IPT_RF input record.
+    subroutine init_MESH(MSH, nDim)
+        type(MESH), intent(inout) :: MSH
+        integer, intent(in) :: nDim
+        call finalize_MESH(MSH)
+        allocate(MSH%xNStep(nDim), MSH%xStep(nDim), MSH%xMinGlob(nDim), MSH%xMaxGlob(nDim))
+        allocate(MSH%xMinInt(nDim), MSH%xMaxInt(nDim), MSH%xMinExt(nDim), MSH%xMaxExt(nDim))
+        allocate(MSH%xMinBound(nDim), MSH%xMaxBound(nDim))
+        MSH%nDim = nDim
+        MSH%xNStep = -1
+        MSH%xStep = 1.0D0
+        MSH%xMinGlob = 0.0D0; MSH%xMaxGlob = 0.0D0
+        MSH%xMinInt = -1; MSH%xMaxInt = -1; MSH%xMinExt = -1; MSH%xMaxExt = -1
+        MSH%xMinBound = -1; MSH%xMaxBound = -1
+        MSH%init = .true.
+    end subroutine init_MESH
+
+    subroutine finalize_MESH(MSH)
+        type(MESH), intent(inout) :: MSH
+        if (allocated(MSH%xNStep)) deallocate(MSH%xNStep)
+        if (allocated(MSH%xStep)) deallocate(MSH%xStep)
+        if (allocated(MSH%xMinGlob)) deallocate(MSH%xMinGlob)
+        if (allocated(MSH%xMaxGlob)) deallocate(MSH%xMaxGlob)
+        if (allocated(MSH%xMinInt)) deallocate(MSH%xMinInt)
+        if (allocated(MSH%xMaxInt)) deallocate(MSH%xMaxInt)
+        if (allocated(MSH%xMinExt)) deallocate(MSH%xMinExt)
+        if (allocated(MSH%xMaxExt)) deallocate(MSH%xMaxExt)
+        if (allocated(MSH%xMinBound)) deallocate(MSH%xMinBound)
+        if (allocated(MSH%xMaxBound)) deallocate(MSH%xMaxBound)
+        MSH%init = .false.
+    end subroutine finalize_MESH
+
+    !> calculate_random_seed (math_RF.f90:538-566) for a deterministic start:
+    !> seed(k) = 72 + 18*seedStart + 37*(k-1); a negative start takes the
+    !> system clock as the reference does.
+    subroutine calculate_random_seed(seed, seedStart)
+        integer, dimension(:), allocatable, intent(out) :: seed
+        integer, optional, intent(in) :: seedStart
+        integer :: i, n, clock
+        call random_seed(size = n)
+        allocate(seed(n))
+        call system_clock(count = clock)
+        if (present(seedStart)) then
+            if (seedStart >= 0) then
+                seed = 72 + seedStart*18 + 37*(/ (i - 1, i = 1, n) /)
+                return
+            end if
+        end if
+        seed = clock + 37*(/ (i - 1, i = 1, n) /)
+    end subroutine calculate_random_seed
+
+    !> init_random_seed (math_RF.f90:475-497)
+    subroutine init_random_seed(seedIn, seedStart)
+        integer, dimension(1:), optional, intent(in) :: seedIn
+        integer, optional, intent(in) :: seedStart
+        integer, dimension(:), allocatable :: seed
+        if (present(seedIn)) then
+            call random_seed(put = seedIn)
+        else
+            call calculate_random_seed(seed, seedStart)
+            call random_seed(put = seed)
+            deallocate(seed)
+        end if
+    end subroutine init_random_seed
+
+    !> The C parameter block of RDF (coordinates normalised inside the library).
+    function rf_c_params(RDF) result(p)
+        type(RF), intent(in) :: RDF
+        type(rf_params_c) :: p
+        integer :: i
+        p%nDim = RDF%nDim
+        p%method = RDF%method
+        p%corrMod = RDF%corrMod
+        p%mode = RDF%compatMode
+        p%form = RDF%form
+        p%kSampling = RDF%kSampling
+        p%Nmc = RDF%Nmc
+        p%margiFirst = 0
+        p%nKDensity = RDF%nKDensity
+        p%xSlab = 0
+        p%corrL = 1d0
+        p%xRange = 1d0
+        do i = 1, RDF%nDim
+            p%corrL(i) = RDF%corrL(i)
+            p%xRange(i) = RDF%xRange(i)
+        end do
+        p%fieldAvg = RDF%fieldAvg
+        p%fieldVar = RDF%fieldVar
+        p%seed = RDF%deviceSeed
+        p%isoShared = merge(1, 0, RDF%isoShared)
+    end function rf_c_params
+
+    !> The uniforms of one generation call, drawn in the reference's order
+    !> after random_seed(PUT=seed): kU (method 3 wavevectors, :120) then
+    !> phU(:, n) per realization (methods 2/3 always, :276-279; method 1 only
+    !> when calculate(n), :420-426 / :456-463).
+    subroutine rf_draw_sequence(p, info, seed, calc, kU, phU)
+        type(rf_params_c), intent(in) :: p
+        type(rf_spectrum_info_c), intent(in) :: info
+        integer, dimension(:), intent(in) :: seed
+        logical, dimension(:), intent(in) :: calc
+        double precision, dimension(:), intent(out) :: kU
+        double precision, dimension(:, :), intent(out) :: phU
+        integer :: n, v, nv
+        integer(kind=8) :: kN
+        call random_seed(put = seed)
+        if (info%nKUniform > 0) call random_number(kU(1:info%nKUniform))
+        kN = info%kNTotal
+        do n = 1, p%Nmc
+            if (p%method == ISOTROPIC) then
+                if (.not. calc(n)) then
+                    phU(:, n) = 0d0
+                    cycle
+                end if
+                nv = info%nVarIso
+                do v = 1, nv
+                    call random_number(phU((v - 1)*kN + 1:v*kN, n))
+                end do
+            else
+                call random_number(phU(1:kN, n))
+            end if
+        end do
+    end subroutine rf_draw_sequence
+
+    !> C entry for test harnesses: the same draw sequence as
+    !> create_RF_Unstruct_Init for parameter block p and seed vector seed.
+    subroutine rf_fortran_draws(p, seed, nseed, calc, kU, phU) bind(C, name="rf_fortran_draws")
+        type(rf_params_c), intent(in) :: p
+        integer(c_int), value :: nseed
+        integer(c_int), intent(in) :: seed(nseed)
+        integer(c_int8_t), intent(in) :: calc(*)
+        real(c_double), intent(out) :: kU(*)
+        real(c_double), intent(out) :: phU(*)
+        type(rf_spectrum_info_c) :: info
+        integer(c_int) :: rc
+        integer :: n, ns
+        integer, allocatable :: s(:)
+        logical, allocatable :: lc(:)
+        double precision, allocatable :: ku_l(:), ph_l(:,:)
+        rc = c_rf_plan_spectrum(p, info)
+        call rf_check(rc, "rf_plan_spectrum")
+        call random_seed(size = ns)
+        allocate(s(ns))
+        s = seed(1)
+        do n = 1, min(ns, int(nseed))
+            s(n) = seed(n)
+        end do
+        allocate(lc(p%Nmc))
+        do n = 1, p%Nmc
+            lc(n) = calc(n) /= 0
+        end do
+        allocate(ku_l(max(1_8, info%nKUniform)), ph_l(info%nPhaseUniform, p%Nmc))
+        call rf_draw_sequence(p, info, s, lc, ku_l, ph_l)
+        if (info%nKUniform > 0) kU(1:info%nKUniform) = ku_l(1:info%nKUniform)
+        phU(1:info%nPhaseUniform*p%Nmc) = reshape(ph_l, [info%nPhaseUniform*p%Nmc])
+    end subroutine rf_fortran_draws
+
+    !> Number of integers random_seed takes (the compiler's seed size).
+    integer(c_int) function rf_fortran_seed_size() bind(C, name="rf_fortran_seed_size")
+        integer :: n
+        call random_seed(size = n)
+        rf_fortran_seed_size = n
+    end function rf_fortran_seed_size
+
+    !> create_RF_Unstruct_Init (randomFieldND.f90:27-132) on the GPU.
+    !> Preconditions as in the reference: RDF initialised, RDF%xPoints =>
+    !> nDim x N points, RDF%randField => N x Nmc storage, RDF%xRange set,
+    !> product(MSH%xNStep) = N (randomFieldND.f90:209).
+    subroutine create_RF_Unstruct_Init(RDF, MSH)
+        type(RF), intent(inout), target :: RDF
+        type(MESH), intent(inout) :: MSH
+        type(rf_params_c) :: p
+        type(rf_spectrum_info_c) :: info
+        integer(kind=8) :: N
+        integer(c_int) :: rc
+        integer :: i
+        double precision, allocatable, target :: kU(:), phU(:,:)
+        integer(c_int8_t), allocatable, target :: calc(:)
+        type(c_ptr) :: kU_p, phU_p
+
+        if (.not. RDF%init) call fail("RDF is not initialised (init_RF)")
+        if (.not. associated(RDF%xPoints)) call fail("RDF%xPoints is not associated")
+        if (.not. associated(RDF%randField)) call fail("RDF%randField is not associated")
+        if (size(RDF%xPoints, 1) /= RDF%nDim) call fail("size(xPoints,1) /= nDim")
+        N = size(RDF%xPoints, 2, kind=8)
+        if (.not. allocated(MSH%xNStep)) call fail("MSH%xNStep is not allocated (init_MESH)")
+        if (product(int(MSH%xNStep, 8)) /= N) call fail("product(MSH%xNStep) /= number of points")
+        if (size(RDF%randField, 1, kind=8) /= N .or. size(RDF%randField, 2) /= RDF%Nmc) &
+            call fail("randField must be N x Nmc")
+        do i = 1, RDF%nDim
+            if (.not. (RDF%xRange(i) > 0d0)) call fail("RDF%xRange must be set and > 0")
+        end do
+        MSH%xNTotal = N
+
+        p = rf_c_params(RDF)
+        rc = c_rf_plan_spectrum(p, info)
+        call rf_check(rc, "rf_plan_spectrum")
+
+        allocate(calc(RDF%Nmc))
+        do i = 1, RDF%Nmc
+            calc(i) = merge(1_c_int8_t, 0_c_int8_t, RDF%calculate(i))
+        end do
+        allocate(kU(max(1_8, info%nKUniform)), phU(info%nPhaseUniform, RDF%Nmc))
+        kU_p = c_null_ptr
+        phU_p = c_null_ptr
+        if (.not. RDF%deviceDraws) then
+            if (.not. allocated(RDF%seed)) call fail("RDF%seed is not allocated")
+            call rf_draw_sequence(p, info, RDF%seed, RDF%calculate, kU, phU)
+            if (info%nKUniform > 0) kU_p = c_loc(kU)
+            phU_p = c_loc(phU)
+        end if
+
+        if (is_contiguous(RDF%xPoints) .and. is_contiguous(RDF%randField)) then
+            rc = c_rf_generate(rf_ctx(RDF%device), p, c_loc(RDF%xPoints), int(N, c_int64_t), &
+                               kU_p, phU_p, c_loc(calc), c_loc(RDF%randField), 0_c_int32_t)
+        else
+            ! pointer sections: stage through contiguous copies
+            block
+                double precision, allocatable, target :: xs(:,:), fs(:,:)
+                allocate(xs(RDF%nDim, N), fs(N, RDF%Nmc))
+                xs = RDF%xPoints
+                rc = c_rf_generate(rf_ctx(RDF%device), p, c_loc(xs), int(N, c_int64_t), &
+                                   kU_p, phU_p, c_loc(calc), c_loc(fs), 0_c_int32_t)
+                if (rc == 0) RDF%randField = fs
+            end block
+        end if
+        call rf_check(rc, "rf_generate")
+
+        ! spectral state the reference leaves in RDF (spectra_RF.f90:68-133)
+        RDF%kNTotal = info%kNTotal
+        RDF%kMax = info%kMax(1:RDF%nDim)
+        RDF%kDelta = info%kDelta(1:RDF%nDim)
+        RDF%kNStep = info%kNStep(1:RDF%nDim)
+        if (allocated(RDF%kPoints)) deallocate(RDF%kPoints)
+        if (allocated(RDF%SkVec)) deallocate(RDF%SkVec)
+        if (RDF%method == ISOTROPIC) then
+            allocate(RDF%kPoints(1, info%kNTotal))
+        else
+            allocate(RDF%kPoints(RDF%nDim, info%kNTotal))
+        end if
+        allocate(RDF%SkVec(info%kNTotal))
+        if (.not. RDF%deviceDraws .or. info%nKUniform == 0) then
+            rc = c_rf_spectrum_tables(p, merge(c_loc(kU), c_null_ptr, info%nKUniform > 0), &
+                                      c_loc(RDF%kPoints), c_loc(RDF%SkVec))
+            call rf_check(rc, "rf_spectrum_tables")
+        end if
+    contains
+        subroutine fail(msg)
+            character(len=*), intent(in) :: msg
+            write(*,*) "ERROR in create_RF_Unstruct_Init: ", msg
+            error stop 1
+        end subroutine fail
+    end subroutine create_RF_Unstruct_Init
+
+    !> multiVariateTransformation (randomFieldND.f90:139-178), on the GPU.
+    subroutine multiVariateTransformation(margiFirst, fieldAvg, fieldVar, randField)
+        integer, intent(in) :: margiFirst
+        double precision, intent(in) :: fieldAvg, fieldVar
+        double precision, dimension(1:, 1:), intent(inout), target, contiguous :: randField
+        integer(c_int) :: rc
+        rc = c_rf_marginal_host(rf_ctx(0), int(margiFirst, c_int32_t), fieldAvg, fieldVar, &
+                                c_loc(randField), size(randField, kind=c_int64_t))
+        call rf_check(rc, "multiVariateTransformation")
+    end subroutine multiVariateTransformation
+
+    !> README.md:58-88 with string model names ("gaussian"; "gaussian" or
+    !> "lognormal").  The box extent of the points sets the wavevector lattice
+    !> (xRange); seedStart (default 0) gives the reference seed vector
+    !> 72 + 18*seedStart + 37*(k-1).
+    subroutine createStdGaussUnstruct_str(xPoints, corrMod, margiFirst, corrL, fieldAvg, fieldVar, &
+                                          Nmc, method, randField, seedStart, mode, form)
+        double precision, dimension(:, :), intent(in), target, contiguous :: xPoints
+        character(len=*), intent(in) :: corrMod, margiFirst
+        double precision, dimension(:), intent(in) :: corrL
+        double precision, intent(in) :: fieldAvg, fieldVar
+        integer, intent(in) :: Nmc, method
+        double precision, dimension(:, :), intent(inout), target, contiguous :: randField
+        integer, intent(in), optional :: seedStart, mode, form
+        integer :: cm, mf
+        select case (trim(adjustl(corrMod)))
+            case ("gaussian"); cm = cm_GAUSSIAN
+            case ("cos", "cosinus"); cm = cm_COS
+            case default
+                write(*,*) "ERROR: unknown corrMod '", trim(corrMod), "'"
+                error stop 1
+        end select
+        select case (trim(adjustl(margiFirst)))
+            case ("gaussian"); mf = fom_GAUSSIAN
+            case ("lognormal"); mf = fom_LOGNORMAL
+            case default
+                write(*,*) "ERROR: unknown margiFirst '", trim(margiFirst), "'"
+                error stop 1
+        end select
+        call createStdGaussUnstruct_int(xPoints, cm, mf, corrL, fieldAvg, fieldVar, Nmc, method, &
+                                        randField, seedStart, mode, form)
+    end subroutine createStdGaussUnstruct_str
+
+    subroutine createStdGaussUnstruct_int(xPoints, corrMod, margiFirst, corrL, fieldAvg, fieldVar, &
+                                          Nmc, method, randField, seedStart, mode, form)
+        double precision, dimension(:, :), intent(in), target, contiguous :: xPoints
+        integer, intent(in) :: corrMod, margiFirst
+        double precision, dimension(:), intent(in) :: corrL
+        double precision, intent(in) :: fieldAvg, fieldVar
+        integer, intent(in) :: Nmc, method
+        double precision, dimension(:, :), intent(inout), target, contiguous :: randField
+        integer, intent(in), optional :: seedStart, mode, form
+        type(RF) :: RDF
+        type(MESH) :: MSH
+        integer :: nDim, i, s
+        integer, allocatable :: seed(:)
+        nDim = size(xPoints, 1)
+        call init_RF(RDF, nDim, Nmc, method, corrMod, margiFirst, corrL, fieldAvg, fieldVar)
+        call init_MESH(MSH, nDim)
+        MSH%xNStep = 1
+        MSH%xNStep(1) = size(xPoints, 2)
+        do i = 1, nDim
+            RDF%xRange(i) = maxval(xPoints(i, :)) - minval(xPoints(i, :))
+            if (.not. (RDF%xRange(i) > 0d0)) RDF%xRange(i) = corrL(i)
+        end do
+        s = 0
+        if (present(seedStart)) s = seedStart
+        call calculate_random_seed(seed, s)
+        RDF%seed = seed
+        if (present(mode)) RDF%compatMode = mode
+        if (present(form)) RDF%form = form
+        RDF%xPoints => xPoints
+        RDF%randField => randField
+        call create_RF_Unstruct_Init(RDF, MSH)
+        call multiVariateTransformation(margiFirst, fieldAvg, fieldVar, randField)
+        call finalize_RF(RDF)
+        call finalize_MESH(MSH)
+    end subroutine createStdGaussUnstruct_int
+
+end module randomField_gpu

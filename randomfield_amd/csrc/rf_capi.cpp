@@ -69,8 +69,9 @@ int build_form_operands(rf_field_plan* P, hipStream_t s) {
                                                 P->d_phT, P->d_aT, s));
         }
     } else if (P->p.form == RF_FORM_MFMA_F16 || P->p.form == RF_FORM_MFMA_BF16) {
-        RF_HIP_TRY(ctx, rf_launch_mfma_prep(P->p.form, kN, P->kNpad, Nmc, P->Rpad, P->d_phase,
-                                            P->d_amp, ampMult * P->bScale, P->d_B, s));
+        RF_HIP_TRY(ctx, rf_launch_mfma_prep(P->p.form, P->p.nDim, kN, P->kNpad, Nmc, P->Rpad,
+                                            P->d_kRev, P->d_phase, P->d_amp, ampMult * P->bScale,
+                                            P->d_B, s));
     }
     return RF_OK;
 }
@@ -373,7 +374,7 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
                 else e = rf_launch_sum_fp32(a, P->d_kRev, P->d_phT, P->d_aT, P->Rpad, s);
                 break;
             default:
-                e = rf_launch_sum_mfma(p.form, a, P->d_kRev, P->kNpad, P->Rpad, P->d_B, 1.0 / P->bScale, s);
+                e = rf_launch_sum_mfma(p.form, a, P->kNpad, P->Rpad, P->d_B, 1.0 / P->bScale, s);
                 break;
         }
         if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("sum kernel launch: ") + hipGetErrorString(e));

@@ -113,12 +113,11 @@ int rf_fp32_rchunk();
 
 // MFMA form (rf_mfma.hip)
 int rf_mfma_kpad(int form);
-hipError_t rf_launch_mfma_prep(int form, int64_t kN, int64_t kNpad, int Nmc, int Npad,
-                               const double* d_phase, const double* d_amp, double scale,
-                               void* d_B, hipStream_t s);
-hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, const double* d_kRev,
-                              int64_t kNpad, int Npad, const void* d_B, double outScale,
-                              hipStream_t s);
+hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, int Nmc, int Npad,
+                               const double* d_kRev, const double* d_phase, const double* d_amp,
+                               double scale, void* d_B, hipStream_t s);
+hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, int64_t kNpad, int Npad,
+                              const void* d_B, double outScale, hipStream_t s);
 int rf_mfma_npad(int Nmc);
 size_t rf_mfma_bbytes(int form, int64_t kNpad, int Npad);
 

@@ -9,13 +9,16 @@
 // * Phi, the basis, is never stored: every wave produces the fragments it
 //   consumes.  Lane l owns point l%32 and the wavevectors 4h..4h+3 (h = l/32)
 //   of each 8-wavevector K-step; it forms k.x in FP64, reduces it to
-//   revolutions, takes the hardware sin/cos (v_sin_f32/v_cos_f32 take
-//   revolutions) and splits each value into hi + lo halves.
+//   revolutions (v_fract_f64), takes the hardware sin/cos (v_sin_f32 and
+//   v_cos_f32 take revolutions) and splits each value into hi + lo halves.
 // * P, the coefficient panels, is built once per plan in FP64 and stored
 //   pre-split (hi, lo), pre-scaled by a power of two, in exactly the per-lane
-//   order of the MFMA A operand; panels go global -> LDS (double buffered) ->
-//   registers.  K order inside a step is (wavevector q, cos/sin), identical on
-//   both operands, so any hardware K permutation cancels.
+//   order of the MFMA A operand.  Each K-step's block -- its 8 wavevectors
+//   and the panels -- is contiguous in HBM and goes to LDS by direct
+//   global->LDS loads (global_load_lds_dwordx4) one step ahead of use, into a
+//   double buffer; one barrier per step.  K order inside a step is
+//   (wavevector q, cos/sin), identical on both operands, so any hardware K
+//   permutation cancels.
 // * D += P_hi Phi_hi + P_hi Phi_lo + P_lo Phi_hi with FP32 accumulation
 //   (v_mfma_f32_32x32x16_{f16,bf16}); the dropped lo.lo term is 2^-22 (FP16)
 //   relative.  A wave holds 32 points x (32*MT) realizations: 16*MT
@@ -28,12 +31,14 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f16x __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 constexpr int kWaves = 4;               // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
 constexpr int kPointsPerBlock = 32 * kWaves;
 constexpr int kMaxMT = 8;               // up to 256 realizations per block
 constexpr int kStepK = 8;               // wavevectors per K-step (16 K entries)
+constexpr int kKUnits = 16;             // 8 wavevectors x 4 doubles = 16 x 16 B
 
 template <typename V> struct Elem;
 template <> struct Elem<h8> { typedef _Float16 T; };
@@ -46,28 +51,24 @@ __device__ inline f16x mfma(const b8& a, const b8& b, const f16x& c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-template <typename V>
-__device__ inline void split(float v, typename Elem<V>::T* hi, typename Elem<V>::T* lo) {
-    typedef typename Elem<V>::T T;
-    const T h = (T)v;
-    *hi = h;
-    *lo = (T)(v - (float)h);
-}
-
 __device__ inline float epi32(float u, const rf_epilogue& e) {
     if (e.margi == RF_FOM_NONE) return u;
     const float y = fmaf(u, (float)e.scale, (float)e.shift);
     return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
 }
 
-// Panel layout (16-byte units): [step][mtile][hilo][lane] -> 8 elements.
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Step block in HBM (16-byte units): [t][ k (16 units) | panels mtTotal x 2 x 64 ]
+// LDS buffer: [ panels MT x 2 x 64 | k 16 ].
 template <typename V, int D, int MT>
-__global__ __launch_bounds__(kThreads) void k_sum_mfma(rf_sum_args a, const double* __restrict__ kRev,
-                                                       int64_t nSteps, int mtTotal,
-                                                       const u32x4* __restrict__ panels,
-                                                       float outScale) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ blocks,
+                float outScale) {
     typedef typename Elem<V>::T T;
-    __shared__ u32x4 lds[2][MT * 2 * 64];
+    constexpr int kPanelUnits = MT * 2 * 64;
+    constexpr int kBufUnits = kPanelUnits + kKUnits;
+    __shared__ u32x4 lds[2][kBufUnits];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(kThreads) void k_sum_mfma(rf_sum_args a, const doub
     const int h = lane >> 5;
     const int n = lane & 31;
     const int mt0 = blockIdx.y * kMaxMT;            // first M-tile of this block
+    const int64_t stepUnits = kKUnits + (int64_t)mtTotal * 128;
     const int64_t pt = a.i0 + (int64_t)blockIdx.x * kPointsPerBlock + wave * 32 + n;
     const int64_t ptc = pt < a.i1 ? pt : a.i1 - 1;
 
@@ -88,48 +90,62 @@ __global__ __launch_bounds__(kThreads) void k_sum_mfma(rf_sum_args a, const doub
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
 
-    // cooperative panel copy: MT*2*64 16-byte units per step
-    constexpr int kUnits = MT * 2 * 64;
-    auto load_step = [&](int64_t t, int buf) {
-        const u32x4* src = panels + ((t * mtTotal + mt0) * 2) * 64;
-        for (int u = tid; u < kUnits; u += kThreads) lds[buf][u] = src[u];
+    // direct global->LDS copy of step t into buffer buf: 2*MT panel chunks
+    // of 64 units spread over the waves, plus the 16 wavevector units
+    auto dma_step = [&](int64_t t, int buf) {
+        const u32x4* src = blocks + t * stepUnits;
+        for (int c = wave; c < 2 * MT; c += kWaves) {
+            const u32x4* g = src + kKUnits + (int64_t)mt0 * 128 + c * 64 + lane;
+            __builtin_amdgcn_global_load_lds((const void*)g, (lds_void*)&lds[buf][c * 64], 16, 0, 0);
+        }
+        if (wave == kWaves - 1 && lane < kKUnits)
+            __builtin_amdgcn_global_load_lds((const void*)(src + lane), (lds_void*)&lds[buf][kPanelUnits], 16, 0, 0);
     };
 
-    load_step(0, 0);
+    dma_step(0, 0);
+    __syncthreads();
     for (int64_t t = 0; t < nSteps; t++) {
         const int buf = (int)(t & 1);
+        if (t + 1 < nSteps) dma_step(t + 1, buf ^ 1);
         // basis fragments of this lane: wavevectors 8t + 4h + q
         V bh, bl;
-        const double* kt = kRev + (t * kStepK) * D;
+        const d2* kq = reinterpret_cast<const d2*>(&lds[buf][kPanelUnits]);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            double s = 0.0;
+            const d2 k01 = kq[(4 * h + q) * 2 + 0];
+            double s = k01.x * xn[0];
+            if (D > 1) s = fma(k01.y, xn[1], s);
+            if (D > 2) s = fma(kq[(4 * h + q) * 2 + 1].x, xn[2], s);
+            const float f = (float)__builtin_amdgcn_fract(s);
+            const float c = __builtin_amdgcn_cosf(f);
+            const float sn = __builtin_amdgcn_sinf(f);
+            const T ch = (T)c, sh = (T)sn;
+            bh[2 * q] = ch;
+            bh[2 * q + 1] = sh;
+            bl[2 * q] = (T)(c - (float)ch);
+            bl[2 * q + 1] = (T)(sn - (float)sh);
+        }
+        const u32x4* pan = &lds[buf][0];
 #pragma unroll
-            for (int d = 0; d < D; d++) {
-                const double k0 = kt[q * D + d];
-                const double k1 = kt[(4 + q) * D + d];
-                s = fma(h ? k1 : k0, xn[d], s);
+        for (int m = 0; m < MT; m += 2) {
+            const V ah0 = __builtin_bit_cast(V, pan[(m * 2 + 0) * 64 + lane]);
+            const V al0 = __builtin_bit_cast(V, pan[(m * 2 + 1) * 64 + lane]);
+            if (MT > 1) {
+                const V ah1 = __builtin_bit_cast(V, pan[(m * 2 + 2) * 64 + lane]);
+                const V al1 = __builtin_bit_cast(V, pan[(m * 2 + 3) * 64 + lane]);
+                acc[m] = mfma(ah0, bh, acc[m]);
+                acc[m + 1] = mfma(ah1, bh, acc[m + 1]);
+                acc[m] = mfma(ah0, bl, acc[m]);
+                acc[m + 1] = mfma(ah1, bl, acc[m + 1]);
+                acc[m] = mfma(al0, bh, acc[m]);
+                acc[m + 1] = mfma(al1, bh, acc[m + 1]);
+            } else {
+                acc[m] = mfma(ah0, bh, acc[m]);
+                acc[m] = mfma(ah0, bl, acc[m]);
+                acc[m] = mfma(al0, bh, acc[m]);
             }
-            s = s - rint(s);
-            const float f = (float)s;
-            T ch, cl, sh, sl;
-            split<V>(__builtin_amdgcn_cosf(f), &ch, &cl);
-            split<V>(__builtin_amdgcn_sinf(f), &sh, &sl);
-            bh[2 * q] = ch; bh[2 * q + 1] = sh;
-            bl[2 * q] = cl; bl[2 * q + 1] = sl;
         }
-        __syncthreads();                            // step t resident in lds[buf]
-        if (t + 1 < nSteps) load_step(t + 1, buf ^ 1);
-#pragma unroll
-        for (int m = 0; m < MT; m++) {
-            const u32x4 ph = lds[buf][(m * 2 + 0) * 64 + lane];
-            const u32x4 pl = lds[buf][(m * 2 + 1) * 64 + lane];
-            const V ahi = __builtin_bit_cast(V, ph);
-            const V alo = __builtin_bit_cast(V, pl);
-            acc[m] = mfma(ahi, bh, acc[m]);
-            acc[m] = mfma(ahi, bl, acc[m]);
-            acc[m] = mfma(alo, bh, acc[m]);
-        }
+        __syncthreads();   // step t+1 landed in LDS; step t's buffer is free
     }
     if (pt >= a.i1) return;
     // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
@@ -149,20 +165,31 @@ __global__ __launch_bounds__(kThreads) void k_sum_mfma(rf_sum_args a, const doub
     }
 }
 
-// Coefficient panels from FP64 phases and amplitudes.  One thread per
-// (step, mtile, lane, element); both hilo halves written.
+// Step blocks from FP64 wavevectors (revolutions), phases and amplitudes.
+// Threads cover (step, mtile, lane, element) of the panels; thread 0..31 of
+// each step also write its 8 wavevectors.
 template <typename T>
-__global__ void k_mfma_prep(int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
-                            const double* __restrict__ phase, const double* __restrict__ amp,
-                            double scale, T* __restrict__ panels) {
+__global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
+                            const double* __restrict__ kRev, const double* __restrict__ phase,
+                            const double* __restrict__ amp, double scale,
+                            unsigned char* __restrict__ blocks) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t total = nSteps * mtTotal * 64 * 8;
-    if (e >= total) return;
-    const int el = (int)(e & 7);
-    const int lane = (int)((e >> 3) & 63);
-    const int64_t rest = e >> 9;
-    const int mt = (int)(rest % mtTotal);
-    const int64_t t = rest / mtTotal;
+    const int64_t perStep = (int64_t)mtTotal * 64 * 8;
+    const int64_t stepBytes = (kKUnits + (int64_t)mtTotal * 128) * 16;
+    if (e >= nSteps * perStep) return;
+    const int64_t t = e / perStep;
+    const int64_t rem = e - t * perStep;
+    unsigned char* blk = blocks + t * stepBytes;
+    if (rem < kStepK * 4) {
+        const int q = (int)(rem >> 2), d = (int)(rem & 3);
+        const int64_t j = t * kStepK + q;
+        double v = 0.0;
+        if (j < kN && d < nDim) v = kRev[j * nDim + d];
+        reinterpret_cast<double*>(blk)[q * 4 + d] = v;
+    }
+    const int el = (int)(rem & 7);
+    const int lane = (int)((rem >> 3) & 63);
+    const int mt = (int)(rem >> 9);
     const int h = lane >> 5;
     const int r = mt * 32 + (lane & 31);
     const int q = el >> 1;
@@ -173,30 +200,29 @@ __global__ void k_mfma_prep(int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
         const double ph = phase[(int64_t)r * kN + j];
         v = (el & 1) ? -A * sin(ph) : A * cos(ph);
     }
-    const float vf = (float)v;
-    const T hi = (T)vf;
+    const T hi = (T)(float)v;
     const T lo = (T)(float)(v - (double)(float)hi);
-    const int64_t base = ((t * mtTotal + mt) * 2) * 64 * 8;
-    panels[base + (int64_t)lane * 8 + el] = hi;
-    panels[base + 64 * 8 + (int64_t)lane * 8 + el] = lo;
+    T* pan = reinterpret_cast<T*>(blk + kKUnits * 16) + (int64_t)mt * 2 * 64 * 8;
+    pan[(int64_t)lane * 8 + el] = hi;
+    pan[64 * 8 + (int64_t)lane * 8 + el] = lo;
 }
 
 template <typename V, int D>
-hipError_t launch_mt(int mt, const rf_sum_args& a, const double* kRev, int64_t nSteps, int mtTotal,
-                     const void* B, float outScale, dim3 grid, hipStream_t s) {
+hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, const void* B,
+                     float outScale, dim3 grid, hipStream_t s) {
     const u32x4* P = static_cast<const u32x4*>(B);
     switch (mt) {
-        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
-        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
-        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
-        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, kRev, nSteps, mtTotal, P, outScale); break;
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
     }
     return hipGetLastError();
 }
 
 template <typename V>
-hipError_t launch_form(const rf_sum_args& a, const double* kRev, int64_t kNpad, int Npad,
-                       const void* B, double outScale, hipStream_t s) {
+hipError_t launch_form(const rf_sum_args& a, int64_t kNpad, int Npad, const void* B,
+                       double outScale, hipStream_t s) {
     const int mtTotal = Npad / 32;
     const int mt = mtTotal >= kMaxMT ? kMaxMT : mtTotal;
     const int64_t n = a.i1 - a.i0;
@@ -204,9 +230,9 @@ hipError_t launch_form(const rf_sum_args& a, const double* kRev, int64_t kNpad, 
               (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
     const int64_t nSteps = kNpad / kStepK;
     switch (a.nDim) {
-        case 1: return launch_mt<V, 1>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
-        case 2: return launch_mt<V, 2>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
-        default: return launch_mt<V, 3>(mt, a, kRev, nSteps, mtTotal, B, (float)outScale, grid, s);
+        case 1: return launch_mt<V, 1>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
+        case 2: return launch_mt<V, 2>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
+        default: return launch_mt<V, 3>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
     }
 }
 
@@ -225,28 +251,28 @@ int rf_mfma_npad(int Nmc) {
 }
 
 size_t rf_mfma_bbytes(int, int64_t kNpad, int Npad) {
-    return (size_t)(kNpad / kStepK) * (size_t)(Npad / 32) * 2 * 64 * 16;
+    return (size_t)(kNpad / kStepK) * (size_t)(kKUnits + (Npad / 32) * 128) * 16;
 }
 
-hipError_t rf_launch_mfma_prep(int form, int64_t kN, int64_t kNpad, int Nmc, int Npad,
-                               const double* d_phase, const double* d_amp, double scale,
-                               void* d_B, hipStream_t s) {
+hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, int Nmc, int Npad,
+                               const double* d_kRev, const double* d_phase, const double* d_amp,
+                               double scale, void* d_B, hipStream_t s) {
     const int64_t nSteps = kNpad / kStepK;
     const int mtTotal = Npad / 32;
     const int64_t total = nSteps * mtTotal * 64 * 8;
     const unsigned grid = (unsigned)((total + 255) / 256);
     if (form == RF_FORM_MFMA_BF16)
-        hipLaunchKernelGGL(k_mfma_prep<__bf16>, dim3(grid), dim3(256), 0, s, kN, nSteps, Nmc, mtTotal,
-                           d_phase, d_amp, scale, static_cast<__bf16*>(d_B));
+        hipLaunchKernelGGL(k_mfma_prep<__bf16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
+                           d_kRev, d_phase, d_amp, scale, static_cast<unsigned char*>(d_B));
     else
-        hipLaunchKernelGGL(k_mfma_prep<_Float16>, dim3(grid), dim3(256), 0, s, kN, nSteps, Nmc, mtTotal,
-                           d_phase, d_amp, scale, static_cast<_Float16*>(d_B));
+        hipLaunchKernelGGL(k_mfma_prep<_Float16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
+                           d_kRev, d_phase, d_amp, scale, static_cast<unsigned char*>(d_B));
     return hipGetLastError();
 }
 
-hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, const double* d_kRev, int64_t kNpad,
-                              int Npad, const void* d_B, double outScale, hipStream_t s) {
+hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, int64_t kNpad, int Npad,
+                              const void* d_B, double outScale, hipStream_t s) {
     if (a.i1 <= a.i0) return hipSuccess;
-    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(a, d_kRev, kNpad, Npad, d_B, outScale, s);
-    return launch_form<h8>(a, d_kRev, kNpad, Npad, d_B, outScale, s);
+    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(a, kNpad, Npad, d_B, outScale, s);
+    return launch_form<h8>(a, kNpad, Npad, d_B, outScale, s);
 }

@@ -38,7 +38,6 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int kPointsPerBlock = 32 * kWaves;
 constexpr int kMaxMT = 8;               // up to 256 realizations per block
 constexpr int kStepK = 8;               // wavevectors per K-step (16 K entries)
-constexpr int kKUnits = 16;             // 8 wavevectors x 4 doubles = 16 x 16 B
 
 template <typename V> struct Elem;
 template <> struct Elem<h8> { typedef _Float16 T; };
@@ -59,16 +58,53 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Step block in HBM (16-byte units): [t][ k (16 units) | panels mtTotal x 2 x 64 ]
-// LDS buffer: [ panels MT x 2 x 64 | k 16 ].
+// The 4 wavevectors (x,y,z in revolutions) of one lane-half for one K-step:
+// 12 doubles, 96 contiguous bytes of the wavevector stream.
+struct KQuad { u32x4 v[6]; };
+
+__device__ inline KQuad load_kquad(const u32x4* __restrict__ kq, int64_t t, int h) {
+    KQuad r;
+    const u32x4* p = kq + (t * 2 + h) * 6;
+#pragma unroll
+    for (int i = 0; i < 6; i++) r.v[i] = p[i];
+    return r;
+}
+
+// Basis fragments of a lane: point x, the 4 wavevectors of its half.
+template <typename V, int D>
+__device__ inline void basis(const KQuad& kk, const double* xn, V* bh, V* bl) {
+    typedef typename Elem<V>::T T;
+    const double* k = reinterpret_cast<const double*>(&kk.v[0]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double s = k[3 * q] * xn[0];
+        if (D > 1) s = fma(k[3 * q + 1], xn[1], s);
+        if (D > 2) s = fma(k[3 * q + 2], xn[2], s);
+        const float f = (float)__builtin_amdgcn_fract(s);
+        const float c = __builtin_amdgcn_cosf(f);
+        const float sn = __builtin_amdgcn_sinf(f);
+        const T ch = (T)c, sh = (T)sn;
+        (*bh)[2 * q] = ch;
+        (*bh)[2 * q + 1] = sh;
+        (*bl)[2 * q] = (T)(c - (float)ch);
+        (*bl)[2 * q + 1] = (T)(sn - (float)sh);
+    }
+}
+
+// Panels in HBM (16-byte units): [t][mtile][hilo][lane]; LDS double buffer of
+// MT x 2 x 64 units.  Wavevectors: separate stream [t][half][4][3] doubles,
+// read through the vector memory path two steps ahead, so the LDS serves only
+// the coefficient fragments.  The basis of step t+1 is formed while the MFMAs
+// of step t run.
 template <typename V, int D, int MT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ blocks,
-                float outScale) {
-    typedef typename Elem<V>::T T;
+void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
+                const u32x4* __restrict__ kq, float outScale) {
     constexpr int kPanelUnits = MT * 2 * 64;
-    constexpr int kBufUnits = kPanelUnits + kKUnits;
-    __shared__ u32x4 lds[2][kBufUnits];
+    // two distinct objects: the waitcnt pass can tell the DMA target of the
+    // next step from the buffer being read
+    __shared__ u32x4 ldsA[kPanelUnits];
+    __shared__ u32x4 ldsB[kPanelUnits];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -76,7 +112,7 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
     const int h = lane >> 5;
     const int n = lane & 31;
     const int mt0 = blockIdx.y * kMaxMT;            // first M-tile of this block
-    const int64_t stepUnits = kKUnits + (int64_t)mtTotal * 128;
+    const int64_t stepUnits = (int64_t)mtTotal * 128;
     const int64_t pt = a.i0 + (int64_t)blockIdx.x * kPointsPerBlock + wave * 32 + n;
     const int64_t ptc = pt < a.i1 ? pt : a.i1 - 1;
 
@@ -90,49 +126,39 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
 
-    // direct global->LDS copy of step t into buffer buf: 2*MT panel chunks
-    // of 64 units spread over the waves, plus the 16 wavevector units
-    auto dma_step = [&](int64_t t, int buf) {
-        const u32x4* src = blocks + t * stepUnits;
-        for (int c = wave; c < 2 * MT; c += kWaves) {
-            const u32x4* g = src + kKUnits + (int64_t)mt0 * 128 + c * 64 + lane;
-            __builtin_amdgcn_global_load_lds((const void*)g, (lds_void*)&lds[buf][c * 64], 16, 0, 0);
+    // direct global->LDS copy of step t's panels into buffer buf: 2*MT
+    // chunks of 64 units spread over the waves
+    // (static trip count, so waits on older loads need not drain the DMA)
+    constexpr int kDmaPerWave = (2 * MT + kWaves - 1) / kWaves;
+    auto dma_step = [&](int64_t t, u32x4* dst) {
+        const u32x4* src = panels + t * stepUnits + (int64_t)mt0 * 128;
+#pragma unroll
+        for (int i = 0; i < kDmaPerWave; i++) {
+            const int c = wave + i * kWaves;
+            if (2 * MT >= kWaves * (i + 1) || c < 2 * MT)
+                __builtin_amdgcn_global_load_lds((const void*)(src + c * 64 + lane),
+                                                 (lds_void*)&dst[c * 64], 16, 0, 0);
         }
-        if (wave == kWaves - 1 && lane < kKUnits)
-            __builtin_amdgcn_global_load_lds((const void*)(src + lane), (lds_void*)&lds[buf][kPanelUnits], 16, 0, 0);
     };
 
-    dma_step(0, 0);
-    __syncthreads();
-    for (int64_t t = 0; t < nSteps; t++) {
-        const int buf = (int)(t & 1);
-        if (t + 1 < nSteps) dma_step(t + 1, buf ^ 1);
-        // basis fragments of this lane: wavevectors 8t + 4h + q
-        V bh, bl;
-        const d2* kq = reinterpret_cast<const d2*>(&lds[buf][kPanelUnits]);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const d2 k01 = kq[(4 * h + q) * 2 + 0];
-            double s = k01.x * xn[0];
-            if (D > 1) s = fma(k01.y, xn[1], s);
-            if (D > 2) s = fma(kq[(4 * h + q) * 2 + 1].x, xn[2], s);
-            const float f = (float)__builtin_amdgcn_fract(s);
-            const float c = __builtin_amdgcn_cosf(f);
-            const float sn = __builtin_amdgcn_sinf(f);
-            const T ch = (T)c, sh = (T)sn;
-            bh[2 * q] = ch;
-            bh[2 * q + 1] = sh;
-            bl[2 * q] = (T)(c - (float)ch);
-            bl[2 * q + 1] = (T)(sn - (float)sh);
-        }
-        const u32x4* pan = &lds[buf][0];
+    V bh, bl;
+    KQuad kk;
+    // one K-step: prefetch step t+1's panels into nxt, form the basis of
+    // step t+1 while the MFMAs of step t read cur
+    auto step = [&](int64_t t, const u32x4* cur, u32x4* nxt) {
+        // unconditional (the last step re-reads itself into the unused
+        // buffer), so every path leaves the same number of loads in flight
+        dma_step(t + 1 < nSteps ? t + 1 : t, nxt);
+        V nbh, nbl;
+        basis<V, D>(kk, xn, &nbh, &nbl);
+        kk = load_kquad(kq, t + 2 < nSteps ? t + 2 : t, h);
 #pragma unroll
         for (int m = 0; m < MT; m += 2) {
-            const V ah0 = __builtin_bit_cast(V, pan[(m * 2 + 0) * 64 + lane]);
-            const V al0 = __builtin_bit_cast(V, pan[(m * 2 + 1) * 64 + lane]);
+            const V ah0 = __builtin_bit_cast(V, cur[(m * 2 + 0) * 64 + lane]);
+            const V al0 = __builtin_bit_cast(V, cur[(m * 2 + 1) * 64 + lane]);
             if (MT > 1) {
-                const V ah1 = __builtin_bit_cast(V, pan[(m * 2 + 2) * 64 + lane]);
-                const V al1 = __builtin_bit_cast(V, pan[(m * 2 + 3) * 64 + lane]);
+                const V ah1 = __builtin_bit_cast(V, cur[(m * 2 + 2) * 64 + lane]);
+                const V al1 = __builtin_bit_cast(V, cur[(m * 2 + 3) * 64 + lane]);
                 acc[m] = mfma(ah0, bh, acc[m]);
                 acc[m + 1] = mfma(ah1, bh, acc[m + 1]);
                 acc[m] = mfma(ah0, bl, acc[m]);
@@ -145,7 +171,21 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
                 acc[m] = mfma(al0, bh, acc[m]);
             }
         }
+        bh = nbh;
+        bl = nbl;
+        // this wave's global->LDS copies are complete (vmcnt(0); expcnt and
+        // lgkmcnt untouched) before the barrier publishes them to the others
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();   // step t+1 landed in LDS; step t's buffer is free
+    };
+
+    dma_step(0, ldsA);
+    basis<V, D>(load_kquad(kq, 0, h), xn, &bh, &bl);
+    kk = load_kquad(kq, nSteps > 1 ? 1 : 0, h);           // wavevectors of step 1
+    __syncthreads();
+    for (int64_t t = 0; t < nSteps; t += 2) {
+        step(t, ldsA, ldsB);
+        if (t + 1 < nSteps) step(t + 1, ldsB, ldsA);
     }
     if (pt >= a.i1) return;
     // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
@@ -165,27 +205,24 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
     }
 }
 
-// Step blocks from FP64 wavevectors (revolutions), phases and amplitudes.
-// Threads cover (step, mtile, lane, element) of the panels; thread 0..31 of
-// each step also write its 8 wavevectors.
+// Coefficient panels from FP64 phases and amplitudes, one thread per (step,
+// mtile, lane, element) writing both halves; the first 24 threads of each
+// step also write its 8 wavevectors (revolutions, 3 components, zero padded)
+// to the wavevector stream kqd[j*3 + d].
 template <typename T>
 __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
                             const double* __restrict__ kRev, const double* __restrict__ phase,
-                            const double* __restrict__ amp, double scale,
-                            unsigned char* __restrict__ blocks) {
+                            const double* __restrict__ amp, double scale, T* __restrict__ panels,
+                            double* __restrict__ kqd) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t perStep = (int64_t)mtTotal * 64 * 8;
-    const int64_t stepBytes = (kKUnits + (int64_t)mtTotal * 128) * 16;
     if (e >= nSteps * perStep) return;
     const int64_t t = e / perStep;
     const int64_t rem = e - t * perStep;
-    unsigned char* blk = blocks + t * stepBytes;
-    if (rem < kStepK * 4) {
-        const int q = (int)(rem >> 2), d = (int)(rem & 3);
-        const int64_t j = t * kStepK + q;
-        double v = 0.0;
-        if (j < kN && d < nDim) v = kRev[j * nDim + d];
-        reinterpret_cast<double*>(blk)[q * 4 + d] = v;
+    if (rem < kStepK * 3) {
+        const int64_t j = t * kStepK + rem / 3;
+        const int d = (int)(rem % 3);
+        kqd[j * 3 + d] = (j < kN && d < nDim) ? kRev[j * nDim + d] : 0.0;
     }
     const int el = (int)(rem & 7);
     const int lane = (int)((rem >> 3) & 63);
@@ -202,7 +239,7 @@ __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int m
     }
     const T hi = (T)(float)v;
     const T lo = (T)(float)(v - (double)(float)hi);
-    T* pan = reinterpret_cast<T*>(blk + kKUnits * 16) + (int64_t)mt * 2 * 64 * 8;
+    T* pan = panels + ((t * mtTotal + mt) * 2) * 64 * 8;
     pan[(int64_t)lane * 8 + el] = hi;
     pan[64 * 8 + (int64_t)lane * 8 + el] = lo;
 }
@@ -211,11 +248,12 @@ template <typename V, int D>
 hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, const void* B,
                      float outScale, dim3 grid, hipStream_t s) {
     const u32x4* P = static_cast<const u32x4*>(B);
+    const u32x4* K = P + nSteps * mtTotal * 128;
     switch (mt) {
-        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
-        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
-        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
-        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, outScale); break;
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
     }
     return hipGetLastError();
 }
@@ -250,8 +288,10 @@ int rf_mfma_npad(int Nmc) {
     return p * 32;
 }
 
+// panels [steps][mtiles][hilo][64] x 16 B, then the wavevector stream
+// (3 doubles per padded wavevector)
 size_t rf_mfma_bbytes(int, int64_t kNpad, int Npad) {
-    return (size_t)(kNpad / kStepK) * (size_t)(kKUnits + (Npad / 32) * 128) * 16;
+    return (size_t)(kNpad / kStepK) * (size_t)(Npad / 32) * 128 * 16 + (size_t)kNpad * 3 * 8;
 }
 
 hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, int Nmc, int Npad,
@@ -261,12 +301,14 @@ hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, in
     const int mtTotal = Npad / 32;
     const int64_t total = nSteps * mtTotal * 64 * 8;
     const unsigned grid = (unsigned)((total + 255) / 256);
+    char* base = static_cast<char*>(d_B);
+    double* kqd = reinterpret_cast<double*>(base + (size_t)nSteps * mtTotal * 128 * 16);
     if (form == RF_FORM_MFMA_BF16)
         hipLaunchKernelGGL(k_mfma_prep<__bf16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
-                           d_kRev, d_phase, d_amp, scale, static_cast<unsigned char*>(d_B));
+                           d_kRev, d_phase, d_amp, scale, reinterpret_cast<__bf16*>(base), kqd);
     else
         hipLaunchKernelGGL(k_mfma_prep<_Float16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
-                           d_kRev, d_phase, d_amp, scale, static_cast<unsigned char*>(d_B));
+                           d_kRev, d_phase, d_amp, scale, reinterpret_cast<_Float16*>(base), kqd);
     return hipGetLastError();
 }
 

@@ -58,28 +58,27 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// The 4 wavevectors (x,y,z in revolutions) of one lane-half for one K-step:
-// 12 doubles, 96 contiguous bytes of the wavevector stream.
-struct KQuad { u32x4 v[6]; };
-
-__device__ inline KQuad load_kquad(const u32x4* __restrict__ kq, int64_t t, int h) {
-    KQuad r;
-    const u32x4* p = kq + (t * 2 + h) * 6;
-#pragma unroll
-    for (int i = 0; i < 6; i++) r.v[i] = p[i];
-    return r;
-}
-
-// Basis fragments of a lane: point x, the 4 wavevectors of its half.
+// Basis fragments of a lane: point x, the wavevectors 4h..4h+3 of the step
+// whose 8 wavevectors (x,y,z in revolutions, 24 doubles) start at k.  The
+// wavevectors are wave-uniform (scalar loads); both halves' dot products are
+// formed and the lane's half selected, which is cheaper than selecting the
+// 64-bit operands.
 template <typename V, int D>
-__device__ inline void basis(const KQuad& kk, const double* xn, V* bh, V* bl) {
+__device__ inline void basis(const double* __restrict__ k, int h, const double* xn, V* bh, V* bl) {
     typedef typename Elem<V>::T T;
-    const double* k = reinterpret_cast<const double*>(&kk.v[0]);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        double s = k[3 * q] * xn[0];
-        if (D > 1) s = fma(k[3 * q + 1], xn[1], s);
-        if (D > 2) s = fma(k[3 * q + 2], xn[2], s);
+        double s0 = k[3 * q] * xn[0];
+        double s1 = k[3 * (q + 4)] * xn[0];
+        if (D > 1) {
+            s0 = fma(k[3 * q + 1], xn[1], s0);
+            s1 = fma(k[3 * (q + 4) + 1], xn[1], s1);
+        }
+        if (D > 2) {
+            s0 = fma(k[3 * q + 2], xn[2], s0);
+            s1 = fma(k[3 * (q + 4) + 2], xn[2], s1);
+        }
+        const double s = h ? s1 : s0;
         const float f = (float)__builtin_amdgcn_fract(s);
         const float c = __builtin_amdgcn_cosf(f);
         const float sn = __builtin_amdgcn_sinf(f);
@@ -92,10 +91,10 @@ __device__ inline void basis(const KQuad& kk, const double* xn, V* bh, V* bl) {
 }
 
 // Panels in HBM (16-byte units): [t][mtile][hilo][lane]; LDS double buffer of
-// MT x 2 x 64 units.  Wavevectors: separate stream [t][half][4][3] doubles,
-// read through the vector memory path two steps ahead, so the LDS serves only
-// the coefficient fragments.  The basis of step t+1 is formed while the MFMAs
-// of step t run.
+// MT x 2 x 64 units.  Wavevectors: separate stream [t][8][3] doubles read
+// with scalar loads, so the LDS serves only the coefficient fragments and the
+// only vector-memory traffic in the loop is the panel DMA.  The basis of step
+// t+1 is formed while the MFMAs of step t run.
 template <typename V, int D, int MT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
@@ -142,7 +141,7 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
     };
 
     V bh, bl;
-    KQuad kk;
+    const double* kqd = reinterpret_cast<const double*>(kq);
     // one K-step: prefetch step t+1's panels into nxt, form the basis of
     // step t+1 while the MFMAs of step t read cur
     auto step = [&](int64_t t, const u32x4* cur, u32x4* nxt) {
@@ -150,8 +149,7 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
         // buffer), so every path leaves the same number of loads in flight
         dma_step(t + 1 < nSteps ? t + 1 : t, nxt);
         V nbh, nbl;
-        basis<V, D>(kk, xn, &nbh, &nbl);
-        kk = load_kquad(kq, t + 2 < nSteps ? t + 2 : t, h);
+        basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
 #pragma unroll
         for (int m = 0; m < MT; m += 2) {
             const V ah0 = __builtin_bit_cast(V, cur[(m * 2 + 0) * 64 + lane]);
@@ -180,8 +178,7 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
     };
 
     dma_step(0, ldsA);
-    basis<V, D>(load_kquad(kq, 0, h), xn, &bh, &bl);
-    kk = load_kquad(kq, nSteps > 1 ? 1 : 0, h);           // wavevectors of step 1
+    basis<V, D>(kqd, h, xn, &bh, &bl);
     __syncthreads();
     for (int64_t t = 0; t < nSteps; t += 2) {
         step(t, ldsA, ldsB);

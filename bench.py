@@ -30,6 +30,229 @@ N_POINTS, N_MC, N_K = 2_000_000, 256, 4096
 CORR_L = (2.0, 2.0, 0.5)
 
 
+def _threads() -> int:
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+
+
+def _timed(fn, steps: int, warmup: int):
+    """Wall ms per call (events on torch's current stream) after warmup."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(st)
+        fn()
+        b.record(st)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ms = np.array([a.elapsed_time(b) for a, b in ev])
+    return wall, ms
+
+
+def _err(got, ref):
+    import numpy as np
+    return {"rel_rms": float(np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))),
+            "max_abs": float(np.max(np.abs(got - ref)))}
+
+
+def _line(metric, value, steps, warmup, ms, dtype, workload, **extra):
+    d = {"metric": metric, "value": value, "unit": "points*realizations/s", "n_gpus": 1,
+         "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
+         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+         "config": {"workload": workload}}
+    d.update(extra)
+    return d
+
+
+def run_cfg1(args):
+    """cfg 1: Shinozuka on 20,000 points in [0,10]^3, Nmc 4, seed 1, REF_COMPAT,
+    the reference's own CPU case; FP64 output compared point-wise."""
+    import numpy as np
+    import torch
+
+    import randomfield_amd as R
+    from oracle import oracle as O
+    from randomfield_amd import workloads as W
+    x = W.cfg1_points(20000, 1)
+    ctx = R.Context(0)
+    sptr = torch.cuda.current_stream().cuda_stream
+    base = dict(nDim=3, method=R.SHINOZUKA, mode=R.REF_COMPAT, Nmc=4, xRange=(10, 10, 10))
+    _, phU = R.reference_draws(R.make_params(**base), R.reference_seed(1))
+    po = O.make_plan(3, O.SHINOZUKA, O.REF_COMPAT, [10, 10, 10])
+    k = O.kpoints(po)
+    sk = O.skvec(po, k)
+    t = time.perf_counter()
+    ref = O.shinozuka(po, x, k, sk, phU, threads=_threads())
+    cpu_s = time.perf_counter() - t
+    dx = torch.from_numpy(x).cuda()
+    work = 20000 * 4
+    forms = {}
+    for name, form, dt in (("fp64", R.FORM_FP64, torch.float64), ("fp32", R.FORM_FP32, torch.float32),
+                           ("mfma_f16", R.FORM_MFMA_F16, torch.float32)):
+        plan = R.FieldPlan(ctx, R.make_params(form=form, **base), None, phU)
+        out = torch.empty((4, 20000), dtype=dt, device="cuda")
+        wall, ms = _timed(lambda: plan.execute(dx, out, stream=sptr), args.steps, args.warmup)
+        forms[name] = {"value": work / (wall * 1e-3), "ms_per_step": wall,
+                       "kernel_ms": float(ms.mean()), **_err(out.cpu().numpy().astype(np.float64), ref)}
+        plan.close()
+    ctx.close()
+    return _line("points x realizations / s (cfg1 Shinozuka, FP64 form, point-wise vs FP64 oracle)",
+                 forms["fp64"]["value"], args.steps, args.warmup, forms["fp64"]["ms_per_step"], "f64",
+                 "cfg1: Shinozuka, 20,000 uniform points in [0,10]^3, N_k 10,976, Nmc 4, REF_COMPAT",
+                 forms=forms,
+                 cpu_baseline={"value": work / cpu_s, "unit": "points*realizations/s", "cores": _threads(),
+                               "kind": "port", "seconds": cpu_s,
+                               "sample": "the whole cfg1 workload (FP64 oracle, reference order)"})
+
+
+def run_cfg2(args):
+    """cfg 2: isotropic, 128^3 grid spacing 0.1, corrL 0.5, lognormal (avg 1,
+    var 0.25), Nmc 64, seed 7; FP32 direct (per-realization directions) and
+    BF16 hi/lo MFMA (shared directions); FP32 output with the fused map."""
+    import numpy as np
+    import torch
+
+    import randomfield_amd as R
+    from oracle import oracle as O
+    from randomfield_amd import workloads as W
+    x = W.grid_points((128, 128, 128), (0.1, 0.1, 0.1))
+    N, Nmc = x.shape[0], 64
+    ctx = R.Context(0)
+    sptr = torch.cuda.current_stream().cuda_stream
+    base = dict(nDim=3, method=R.ISOTROPIC, mode=R.FIXED, Nmc=Nmc, corrL=(0.5, 0.5, 0.5),
+                xRange=(12.7, 12.7, 12.7), margiFirst=R.FOM_LOGNORMAL, fieldAvg=1.0, fieldVar=0.25)
+    _, U = R.reference_draws(R.make_params(**base), R.reference_seed(7))
+    dx = torch.from_numpy(x).cuda()
+    work = N * Nmc
+    sel = np.random.default_rng(0).choice(N, 32, replace=False)
+    po = O.make_plan(3, O.ISOTROPIC, O.FIXED, [25.4] * 3)
+    k = O.kpoints(po)
+    sk = O.skvec(po, k)
+    kN = po.kNTotal
+    Us = U.copy()
+    Us[:, : 2 * kN] = U[0, : 2 * kN]
+    forms = {}
+    for name, form, shared, uref in (("fp32_direct", R.FORM_FP32, 0, U),
+                                     ("mfma_bf16_shared", R.FORM_MFMA_BF16, 1, Us),
+                                     ("mfma_f16_shared", R.FORM_MFMA_F16, 1, Us)):
+        plan = R.FieldPlan(ctx, R.make_params(form=form, isoShared=shared, **base), None, U)
+        out = torch.empty((Nmc, N), dtype=torch.float32, device="cuda")
+        wall, ms = _timed(lambda: plan.execute(dx, out, stream=sptr), args.steps, args.warmup)
+        ref = O.marginal(2, 1.0, 0.25, O.isotropic(po, x[sel] / 0.5, k, sk, uref))
+        got = out[:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
+        forms[name] = {"value": work / (wall * 1e-3), "ms_per_step": wall, "kernel_ms": float(ms.mean()),
+                       **_err(got, ref)}
+        plan.close()
+    # the marginal map alone: 8 B per FP32 value (read + write), HBM-bound
+    out = torch.randn((Nmc, N), dtype=torch.float32, device="cuda") * 0.1
+    wall, ms = _timed(lambda: R.check(R.lib().rf_marginal_device(ctx.handle, R.FOM_LOGNORMAL, 1.0, 0.25,
+                                                                 R._ptr(out), out.numel(), R.DTYPE_F32,
+                                                                 R.C.c_void_p(sptr))), 5, 1)
+    gbs = 8.0 * out.numel() / (ms.mean() * 1e-3) / 1e9
+    t = time.perf_counter()
+    O.isotropic(po, x[:2000] / 0.5, k, sk, U, threads=_threads())
+    cpu_s = time.perf_counter() - t
+    ctx.close()
+    best = max(forms.values(), key=lambda f: f["value"])
+    return _line("points x realizations / s (cfg2 isotropic + lognormal)", best["value"], args.steps,
+                 args.warmup, best["ms_per_step"], "f32",
+                 "cfg2: isotropic, 128^3 grid spacing 0.1, corrL 0.5, 286 shells, Nmc 64, lognormal",
+                 forms=forms,
+                 marginal_map={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
+                               "frac": gbs / 8000.0, "kernel_ms": float(ms.mean())},
+                 cpu_baseline={"value": 2000 * Nmc / cpu_s, "unit": "points*realizations/s",
+                               "cores": _threads(), "kind": "port", "seconds": cpu_s,
+                               "sample": "first 2,000 grid points, all realizations (FP64 oracle)"})
+
+
+def run_cfg4(args):
+    """cfg 4: 512x512x256 grid (spacing 0.25), 4x4x2 uneven subdomains with
+    7-point overlaps, Shinozuka per subdomain on its own lattice (seed 100 +
+    field index), Nmc 16, FP32; grid-form generation then the
+    variance-preserving blend.  A step is all 32 generations + the blend."""
+    import numpy as np
+    import torch
+
+    import randomfield_amd as R
+    from oracle import oracle as O
+    from randomfield_amd import workloads as W
+    groupN, parts, nOvl, step, Nmc = (512, 512, 256), (4, 4, 2), 7, (0.25, 0.25, 0.25), 16
+    origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
+    ctx = R.Context(0)
+    sptr = torch.cuda.current_stream().cuda_stream
+    plans, fields, geo = [], [], []
+    for s in range(len(sizes)):
+        n = [int(v) for v in sizes[s]]
+        o = [origins[s][a] * step[a] for a in range(3)]
+        ext = [(n[a] - 1) * step[a] for a in range(3)]
+        plans.append(R.FieldPlan(ctx, R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.FIXED,
+                                                    form=R.FORM_FP32, Nmc=Nmc, xRange=ext,
+                                                    seed=100 + s + 1)))
+        fields.append(torch.empty((Nmc, int(np.prod(n))), dtype=torch.float32, device="cuda"))
+        geo.append((o, n, ext))
+    g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float32, device="cuda")
+
+    def generate():
+        for s, pl in enumerate(plans):
+            pl.execute_grid(geo[s][0], step, geo[s][1], fields[s], stream=sptr)
+
+    def blend():
+        R.blend(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields, g, R.DTYPE_F32,
+                stream=sptr)
+
+    wall_g, ms_g = _timed(generate, args.steps, args.warmup)
+    wall_b, ms_b = _timed(blend, args.steps, args.warmup)
+    nglob = float(np.prod(groupN))
+    nsub = float(sum(np.prod(s) for s in sizes))
+    work = nglob * Nmc
+    bytes_blend = (nsub + nglob) * Nmc * 4.0
+    gbs = bytes_blend / (ms_b.mean() * 1e-3) / 1e9
+    # accuracy: 24 points of subdomain 0 against the FP64 oracle
+    info = plans[0].info
+    du = torch.empty(Nmc * info.kNTotal, dtype=torch.float64, device="cuda")
+    R.device_uniforms(ctx, 101, 2, Nmc * info.kNTotal, du)
+    torch.cuda.synchronize()
+    phU = du.cpu().numpy().reshape(Nmc, info.kNTotal)
+    o, n, ext = geo[0]
+    po = O.make_plan(3, O.SHINOZUKA, O.FIXED, ext)
+    k = O.kpoints(po)
+    sk = O.skvec(po, k)
+    xs = W.grid_points(n, step, o)
+    sel = np.random.default_rng(0).choice(xs.shape[0], 24, replace=False)
+    generate()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ref = O.shinozuka(po, xs[sel], k, sk, phU, threads=_threads())
+    cpu_s = time.perf_counter() - t
+    got = fields[0][:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
+    for pl in plans:
+        pl.close()
+    ctx.close()
+    step_ms = wall_g + wall_b
+    return _line("points x realizations / s (cfg4 localization: generation + blend)", work / (step_ms * 1e-3),
+                 args.steps, args.warmup, step_ms, "f32",
+                 "cfg4: 512x512x256 grid, 4x4x2 uneven subdomains, 7-point overlap, Nmc 16, "
+                 f"N_k {info.kNTotal} (subdomain 0), grid form + variance-preserving blend",
+                 generation_ms=wall_g, blend_ms=wall_b,
+                 blend_roofline={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
+                                 "frac": gbs / 8000.0, "kernel_ms": float(ms_b.mean()),
+                                 "bytes_per_launch": bytes_blend},
+                 accuracy={**_err(got, ref), "sample": "24 points of subdomain 0 x 16 realizations"},
+                 cpu_baseline={"value": 24 * Nmc / cpu_s, "unit": "points*realizations/s",
+                               "cores": _threads(), "kind": "port", "seconds": cpu_s,
+                               "sample": f"24 points x 16 realizations x {info.kNTotal} wavevectors"})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,7 +263,19 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--config", type=int, default=3,
+                    help="benchmark configuration 1..5 (3, the headline workload, by default)")
     args = ap.parse_args()
+    if args.config != 3 and not args.calibrate_copy:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        runners = {1: run_cfg1, 2: run_cfg2, 4: run_cfg4}
+        if args.config not in runners:
+            print(f"unknown configuration {args.config}", file=sys.stderr)
+            return 2
+        print(json.dumps(runners[args.config](args)), flush=True)
+        return 0
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4

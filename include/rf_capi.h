@@ -154,6 +154,16 @@ int rf_plan_info(const rf_field_plan* plan, rf_spectrum_info* info);
  * d_out: device N x Nmc in outDtype.  Stream-ordered, asynchronous. */
 int rf_plan_execute(rf_field_plan* plan, const double* d_x, int64_t N,
                     void* d_out, int32_t outDtype, void* stream);
+/* The same sum on a structured grid x = origin + (i1,i2,i3)*step (caller
+ * units, x fastest; the point order of set_XPoints/setGrid, mesh_RF.f90:44,
+ * math_RF.f90:70-88), evaluated axis by axis: the Shinozuka lattice is a
+ * tensor product per sign block, so sum_j C_j exp(i k.x) factorises into
+ * three small contractions.  Method 2, FIXED mode, nDim 2 or 3 (otherwise
+ * RF_E_UNSUPPORTED).  d_out: prod(nstep) x Nmc.  The plan keeps a scratch
+ * buffer, grown on the first call with a larger grid. */
+int rf_plan_execute_grid(rf_field_plan* plan, const double origin[3],
+                         const double step[3], const int32_t nstep[3],
+                         void* d_out, int32_t outDtype, void* stream);
 /* Host-buffer convenience: create, upload, execute, download, destroy.
  * x: nDim x N, out: N x Nmc (outDtype). */
 int rf_generate(rf_context* ctx, const rf_params* p, const double* x,

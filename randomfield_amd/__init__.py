@@ -155,6 +155,14 @@ class FieldPlan:
         check(lib().rf_plan_execute(self._h, _ptr(x), N, _ptr(out), _dtype_code(out.dtype),
                                     _ptr(stream)), self.ctx.handle)
 
+    def execute_grid(self, origin, step, nstep, out, stream=None) -> None:
+        """Structured grid (x fastest): out device (Nmc, prod(nstep))."""
+        o = np.zeros(3); o[: len(origin)] = origin
+        st = np.ones(3); st[: len(step)] = step
+        ns = np.ones(3, dtype=np.int32); ns[: len(nstep)] = nstep
+        check(lib().rf_plan_execute_grid(self._h, _ptr(o), _ptr(st), _ptr(ns), _ptr(out),
+                                         _dtype_code(out.dtype), _ptr(stream)), self.ctx.handle)
+
     def close(self) -> None:
         if self._h:
             lib().rf_plan_destroy(self._h)

@@ -25,7 +25,7 @@ DTYPE_F64, DTYPE_F32 = 0, 1
 EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
-    "rf_plan_execute", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
+    "rf_plan_execute", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
     "rf_stats_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
 )
@@ -91,6 +91,7 @@ def lib() -> C.CDLL:
     L.rf_plan_destroy.argtypes = [vp]
     L.rf_plan_info.argtypes = [vp, C.POINTER(SpectrumInfo)]
     L.rf_plan_execute.argtypes = [vp, vp, i64, vp, i32, vp]
+    L.rf_plan_execute_grid.argtypes = [vp, vp, vp, vp, vp, i32, vp]
     L.rf_generate.argtypes = [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32]
     L.rf_marginal_device.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp]
     L.rf_marginal_host.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64]

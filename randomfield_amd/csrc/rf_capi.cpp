@@ -186,7 +186,7 @@ int rf_plan_destroy(rf_field_plan* P) {
     }
     dfree(P->d_k); dfree(P->d_kRev); dfree(P->d_kIso); dfree(P->d_kIsoT);
     dfree(P->d_phase); dfree(P->d_amp); dfree(P->d_uni); dfree(P->d_phT);
-    dfree(P->d_aT); dfree(P->d_B); dfree(P->d_Sk); dfree(P->d_calc);
+    dfree(P->d_aT); dfree(P->d_B); dfree(P->d_Sk); dfree(P->d_calc); dfree(P->d_gridScratch);
     delete P;
     return RF_OK;
 }
@@ -412,6 +412,41 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
         if (rc) return rc;
         RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // amp is reused next slab
     }
+    return RF_OK;
+}
+
+int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* step,
+                         const int32_t* nstep, void* d_out, int32_t outDtype, void* stream) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
+    rf_context* ctx = P->ctx;
+    const rf_params& p = P->p;
+    if (!origin || !step || !nstep || !d_out) return rf_set_error(ctx, RF_E_ARG, "NULL grid argument");
+    if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32)
+        return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
+    if (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2)
+        return rf_set_error(ctx, RF_E_UNSUPPORTED,
+                            "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3");
+    for (int a = 0; a < p.nDim; a++) {
+        if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
+        if (!(step[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid steps must be > 0");
+    }
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const rf_spectrum_info& in = P->spec.info;
+    const size_t bytes = rf_grid_scratch_elems(p.nDim, in.kNStep, in.nSignBlocks, p.Nmc, nstep) * sizeof(float2);
+    if (bytes > P->gridScratchBytes) {
+        RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+        dfree(P->d_gridScratch);
+        P->d_gridScratch = nullptr;
+        P->gridScratchBytes = 0;
+        RF_HIP_TRY(ctx, hipMalloc(&P->d_gridScratch, bytes));
+        P->gridScratchBytes = bytes;
+    }
+    double corrL[3] = {1.0, 1.0, 1.0};
+    for (int a = 0; a < p.nDim; a++) corrL[a] = p.corrL[a];
+    RF_HIP_TRY(ctx, rf_launch_grid_sum(p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
+                                       nstep, corrL, P->d_amp, P->d_phase, in.ampMult, P->epi, d_out,
+                                       outDtype, static_cast<float2*>(P->d_gridScratch), s));
     return RF_OK;
 }
 

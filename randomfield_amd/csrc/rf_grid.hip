@@ -1,0 +1,235 @@
+// rf_grid.hip -- Shinozuka sums on structured grids by axis-wise contraction.
+//
+// On a grid x = o + (i1, i2, i3) * step (x fastest, setGrid order,
+// math_RF.f90:70-88) and the Shinozuka lattice -- per sign block b the tensor
+// product k = (s1 k1_j1, s2 k2_j2, s3 k3_j3) of 1-D lattices k_a,j = j dk_a
+// (spectra_RF.f90:88-133) -- the sum of randomFieldND.f90:276-330 factorises:
+//
+//   u_r(x) = Re sum_b sum_j3 E3^{s3}[j3][x3] sum_j2 E2^{s2}[j2][x2]
+//                        sum_j1 E1[j1][x1] C_r[b][j3][j2][j1],
+//   E_a[j][x] = exp(i k_a,j x_a),  C = ampMult A exp(i phi),
+//
+// three small batched complex GEMMs instead of N * N_k cosines
+// (O(n N_k + n^2 N_k^(2/3) + n^3 N_k^(1/3)) per realization and block).
+// One tiled FP32 kernel does every stage: D[m][n] = sum_k A[m][k] B[k][n]
+// with per-stage strides; A may come in per-block variants (conjugated
+// tables of the negative-sign blocks).  The phase tables are formed in FP64
+// and rounded to FP32; accumulation is FP32 over <= 4*N_k^(1/3) terms.
+#include "rf_internal.h"
+
+namespace {
+
+constexpr int kTM = 64, kTN = 64, kTK = 16;   // block tile; 256 threads, 4x4 each
+
+struct GemmDesc {
+    const float2* A; int64_t aSm, aSk, aStrideB; int aDiv, aMod;
+    const float2* B; int64_t bSk, bSn, bStrideB;
+    void* D; int64_t dSm, dSn, dStrideB;
+    int M, N, K, batches;
+    int realOut;          // 0: complex D, 1: Re(D) with the epilogue below
+    int outDtype;
+    rf_epilogue epi;
+};
+
+__device__ inline float2 cmul_add(float2 a, float2 b, float2 c) {
+    c.x = fmaf(a.x, b.x, c.x);
+    c.x = fmaf(-a.y, b.y, c.x);
+    c.y = fmaf(a.x, b.y, c.y);
+    c.y = fmaf(a.y, b.x, c.y);
+    return c;
+}
+
+__device__ inline float epi32(float u, const rf_epilogue& e) {
+    if (e.margi == RF_FOM_NONE) return u;
+    const float y = fmaf(u, (float)e.scale, (float)e.shift);
+    return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
+}
+
+__global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
+    __shared__ float2 sA[kTK][kTM + 1];
+    __shared__ float2 sB[kTK][kTN + 1];
+    const int batch = blockIdx.z;
+    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+    const int tid = threadIdx.x;
+    const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
+    const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB;
+    const float2* B = g.B + (int64_t)batch * g.bStrideB;
+    float2 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = make_float2(0.f, 0.f);
+
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        // stage A (kTM x kTK) and B (kTK x kTN), zero outside the matrices
+        for (int e = tid; e < kTM * kTK; e += 256) {
+            const int mm = e / kTK, kk = e % kTK;
+            const int m = m0 + mm, k = k0 + kk;
+            sA[kk][mm] = (m < g.M && k < g.K) ? A[m * g.aSm + k * g.aSk] : make_float2(0.f, 0.f);
+        }
+        for (int e = tid; e < kTK * kTN; e += 256) {
+            const int kk = e / kTN, nn = e % kTN;
+            const int n = n0 + nn, k = k0 + kk;
+            sB[kk][nn] = (n < g.N && k < g.K) ? B[k * g.bSk + n * g.bSn] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kTK; kk++) {
+            float2 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = sA[kk][tm + i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = sB[kk][tn + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = cmul_add(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = m0 + tm + i;
+        if (m >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + tn + j;
+            if (n >= g.N) continue;
+            const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
+            if (!g.realOut) {
+                static_cast<float2*>(g.D)[o] = acc[i][j];
+            } else {
+                const float u = epi32(acc[i][j].x, g.epi);
+                if (g.outDtype == RF_DTYPE_F64) static_cast<double*>(g.D)[o] = (double)u;
+                else static_cast<float*>(g.D)[o] = u;
+            }
+        }
+    }
+}
+
+// E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
+// variant v = 0 plain, v = 1 conjugate (negative sign).
+__global__ void k_phase_table(int nk, int nx, double dk, double o, double step, double invL,
+                              int variants, float2* __restrict__ E) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)nk * nx) return;
+    const int j = (int)(e / nx), i = (int)(e % nx);
+    const double x = (o + step * (double)i) * invL;
+    const double t = (dk * (double)j) * x;
+    double s, c;
+    sincos(t, &s, &c);
+    E[e] = make_float2((float)c, (float)s);
+    if (variants > 1) E[(int64_t)nk * nx + e] = make_float2((float)c, (float)-s);
+}
+
+// Per-block table for the last axis: Ext[b][j][x] = E^{s_b}[j][x], so the
+// last stage contracts over (b, j) with a plain stride.
+__global__ void k_block_table(int nb, int nk, int nx, int axisBit, const float2* __restrict__ E,
+                              float2* __restrict__ Ext) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t per = (int64_t)nk * nx;
+    if (e >= (int64_t)nb * per) return;
+    const int b = (int)(e / per);
+    const int64_t r = e - b * per;
+    const bool neg = (b >> axisBit) & 1;
+    Ext[e] = neg ? E[per + r] : E[r];
+}
+
+// C[r][j] = ampMult amp[r][j] exp(i phase[r][j]) (FP64 -> FP32 complex).
+__global__ void k_coeffs(int64_t n, const double* __restrict__ amp, const double* __restrict__ ph,
+                         double ampMult, float2* __restrict__ C) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double s, c;
+    sincos(ph[e], &s, &c);
+    const double a = amp[e] * ampMult;
+    C[e] = make_float2((float)(a * c), (float)(a * s));
+}
+
+hipError_t gemm(const GemmDesc& g, hipStream_t s) {
+    dim3 grid((unsigned)((g.N + kTN - 1) / kTN), (unsigned)((g.M + kTM - 1) / kTM), (unsigned)g.batches);
+    hipLaunchKernelGGL(k_cgemm, grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// Scratch for one grid execution (float2 elements).
+size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n) {
+    const int64_t R = (int64_t)nb * Nmc;
+    int64_t tables = 0;
+    for (int a = 0; a < nDim; a++) tables += 2LL * kNStep[a] * n[a] + (int64_t)nb * kNStep[a] * n[a];
+    int64_t coeff = R;
+    for (int a = 0; a < nDim; a++) coeff *= kNStep[a];
+    int64_t t1 = R * n[0];
+    for (int a = 1; a < nDim; a++) t1 *= kNStep[a];
+    int64_t t2 = (nDim == 3) ? R * n[0] * n[1] * kNStep[2] : 0;
+    return (size_t)(tables + coeff + t1 + t2);
+}
+
+hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta,
+                              int nb, const double* origin, const double* step, const int32_t* n,
+                              const double* corrL, const double* d_amp, const double* d_phase,
+                              double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                              float2* scratch, hipStream_t s) {
+    const int64_t R = (int64_t)nb * Nmc;
+    float2* p = scratch;
+    float2* E[3];
+    float2* Ext[3];
+    for (int a = 0; a < nDim; a++) {
+        E[a] = p; p += 2LL * kNStep[a] * n[a];
+        Ext[a] = p; p += (int64_t)nb * kNStep[a] * n[a];
+        const int64_t cnt = (int64_t)kNStep[a] * n[a];
+        hipLaunchKernelGGL(k_phase_table, dim3(blocks_for(cnt)), dim3(256), 0, s, kNStep[a], n[a],
+                           kDelta[a], origin[a], step[a], 1.0 / corrL[a], 2, E[a]);
+        // sign of axis a in block b: kSign column a flips on bit (nDim-1-a) of b
+        if (a > 0)
+            hipLaunchKernelGGL(k_block_table, dim3(blocks_for(nb * cnt)), dim3(256), 0, s, nb, kNStep[a],
+                               n[a], nDim - 1 - a, E[a], Ext[a]);
+    }
+    int64_t nkAll = 1;
+    for (int a = 0; a < nDim; a++) nkAll *= kNStep[a];
+    float2* C = p; p += R * nkAll;
+    hipLaunchKernelGGL(k_coeffs, dim3(blocks_for(R * nkAll)), dim3(256), 0, s, R * nkAll, d_amp, d_phase,
+                       ampMult, C);
+    // stage 1: rows (r, b, j_d..j_2) x j1 -> x1
+    const int64_t rows1 = R * nkAll / kNStep[0];
+    float2* T1 = p; p += rows1 * n[0];
+    GemmDesc g{};
+    g.A = C; g.aSm = kNStep[0]; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
+    g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;
+    g.D = T1; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
+    g.M = (int)rows1; g.N = n[0]; g.K = kNStep[0]; g.batches = 1; g.realOut = 0;
+    hipError_t e = gemm(g, s);
+    if (e != hipSuccess) return e;
+    const float2* last = T1;
+    int64_t plane = n[0];                      // points already resolved
+    if (nDim == 3) {
+        // stage 2: per (r, b, j3): x2 x j2 (E2^{s2}) times j2 x x1
+        float2* T2 = p; p += R * kNStep[2] * n[1] * n[0];
+        GemmDesc h{};
+        h.A = E[1]; h.aSm = 1; h.aSk = n[1]; h.aStrideB = (int64_t)kNStep[1] * n[1];
+        h.B = T1; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)kNStep[1] * n[0];
+        h.D = T2; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
+        h.M = n[1]; h.N = n[0]; h.K = kNStep[1]; h.batches = (int)(R * kNStep[2]); h.realOut = 0;
+        // batch index = (r*nb + b)*nk3 + j3; the axis-2 sign of block b is
+        // bit 1 of b (3-D, nb = 4), i.e. (batch / (2 nk3)) % 2
+        h.aDiv = kNStep[2] * 2;
+        h.aMod = 2;
+        e = gemm(h, s);
+        if (e != hipSuccess) return e;
+        last = T2;
+        plane = (int64_t)n[0] * n[1];
+    }
+    // last stage: per r, x_d x (b, j_d) times (b, j_d) x plane, real part
+    const int ad = nDim - 1;
+    GemmDesc f{};
+    f.A = Ext[ad]; f.aSm = 1; f.aSk = n[ad]; f.aStrideB = 0; f.aDiv = 1; f.aMod = 1;
+    f.B = last; f.bSk = plane; f.bSn = 1; f.bStrideB = (int64_t)nb * kNStep[ad] * plane;
+    f.D = d_out; f.dSm = plane; f.dSn = 1; f.dStrideB = plane * n[ad];
+    f.M = n[ad]; f.N = (int)plane; f.K = nb * kNStep[ad]; f.batches = Nmc; f.realOut = 1;
+    f.outDtype = outDtype; f.epi = epi;
+    return gemm(f, s);
+}

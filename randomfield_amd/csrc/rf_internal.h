@@ -64,6 +64,8 @@ struct rf_field_plan {
     std::vector<std::vector<double>> sqrtChain;
     bool coeffsReady = false;    // FIXED: coefficients independent of N
     rf_epilogue epi{};
+    void* d_gridScratch = nullptr;   // structured-grid form (grown on demand)
+    size_t gridScratchBytes = 0;
 };
 
 // ---- host spectrum (rf_spectrum.cpp)
@@ -120,6 +122,14 @@ hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, int64_t kNpad, int
                               const void* d_B, double outScale, hipStream_t s);
 int rf_mfma_npad(int Nmc);
 size_t rf_mfma_bbytes(int form, int64_t kNpad, int Npad);
+
+// structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice
+size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
+hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta,
+                              int nb, const double* origin, const double* step, const int32_t* n,
+                              const double* corrL, const double* d_amp, const double* d_phase,
+                              double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                              float2* scratch, hipStream_t s);
 
 // post-processing (rf_post.hip)
 hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, int64_t count,

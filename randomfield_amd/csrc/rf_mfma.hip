@@ -58,6 +58,33 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+// (c, s) -> hi and lo halves at positions 2q, 2q+1 of the basis fragments.
+// FP16: one packed conversion for hi, the residuals v - hi by mixed-precision
+// FMAs reading the f16 halves in place (v_fma_mix_f32), one packed conversion
+// for lo.  BF16: plain conversions.
+template <typename V>
+__device__ inline void split_pair(float c, float s, int q, V* bh, V* bl) {
+    typedef typename Elem<V>::T T;
+    const T ch = (T)c, sh = (T)s;
+    (*bh)[2 * q] = ch;
+    (*bh)[2 * q + 1] = sh;
+    (*bl)[2 * q] = (T)(c - (float)ch);
+    (*bl)[2 * q + 1] = (T)(s - (float)sh);
+}
+template <>
+__device__ inline void split_pair<h8>(float c, float s, int q, h8* bh, h8* bl) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 hi = {(_Float16)c, (_Float16)s};        // v_cvt_pk_f16_f32 (RNE)
+    const unsigned hb = __builtin_bit_cast(unsigned, hi);
+    float lc, ls;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[0,0,0] op_sel_hi:[1,0,0]" : "=v"(lc) : "v"(hb), "v"(c));
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ls) : "v"(hb), "v"(s));
+    (*bh)[2 * q] = hi.x;
+    (*bh)[2 * q + 1] = hi.y;
+    (*bl)[2 * q] = (_Float16)lc;
+    (*bl)[2 * q + 1] = (_Float16)ls;
+}
+
 // Basis fragments of a lane: point x, the wavevectors 4h..4h+3 of the step
 // whose 8 wavevectors (x,y,z in revolutions, 24 doubles) start at k.  The
 // wavevectors are wave-uniform (scalar loads); both halves' dot products are
@@ -82,11 +109,7 @@ __device__ inline void basis(const double* __restrict__ k, int h, const double* 
         const float f = (float)__builtin_amdgcn_fract(s);
         const float c = __builtin_amdgcn_cosf(f);
         const float sn = __builtin_amdgcn_sinf(f);
-        const T ch = (T)c, sh = (T)sn;
-        (*bh)[2 * q] = ch;
-        (*bh)[2 * q + 1] = sh;
-        (*bl)[2 * q] = (T)(c - (float)ch);
-        (*bl)[2 * q + 1] = (T)(sn - (float)sh);
+        split_pair<V>(c, sn, q, bh, bl);
     }
 }
 

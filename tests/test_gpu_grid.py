@@ -1,0 +1,114 @@
+"""GPU parity of the structured-grid form (axis-wise contraction of the
+Shinozuka sum) and of the localization pipeline built on it: subdomains
+generated on their own grids and lattices, then blended.  The oracle sums the
+same wavevectors point by point in FP64 (randomFieldND.f90:276-330 order)."""
+import numpy as np
+import pytest
+
+import randomfield_amd as R
+from oracle import oracle as O
+from randomfield_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+def grid_case(ctx, nstep, step, origin, corrL, Nmc, seed, dtype=np.float32, margi=R.FOM_NONE):
+    import torch
+    d = len(nstep)
+    ext = [(nstep[a] - 1) * step[a] for a in range(d)]
+    p = R.make_params(nDim=d, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=Nmc,
+                      corrL=corrL, xRange=ext, margiFirst=margi, fieldAvg=1.0, fieldVar=0.25)
+    _, phU = R.reference_draws(p, R.reference_seed(seed))
+    plan = R.FieldPlan(ctx, p, None, phU)
+    out = torch.empty((Nmc, int(np.prod(nstep))), dtype=torch.float32 if dtype == np.float32 else torch.float64,
+                      device="cuda")
+    plan.execute_grid(origin, step, nstep, out)
+    ctx.sync()
+    x = W.grid_points(nstep, step, origin)
+    po = O.make_plan(d, O.SHINOZUKA, O.FIXED, [ext[a] / corrL[a] for a in range(d)])
+    k = O.kpoints(po)
+    ref = O.shinozuka(po, x / np.array(corrL), k, O.skvec(po, k), phU)
+    if margi != R.FOM_NONE:
+        ref = O.marginal(margi, 1.0, 0.25, ref)
+    got = out.cpu().numpy().astype(np.float64)
+    plan.close()
+    return got, ref
+
+
+@pytest.mark.parametrize("nstep,step,corrL", [
+    ((20, 18, 16), (0.25, 0.25, 0.25), (1.0, 1.2, 0.8)),
+    ((33, 7, 40), (0.3, 0.5, 0.2), (1.0, 1.0, 1.0)),
+    ((41, 29), (0.2, 0.3), (0.7, 1.3)),
+])
+def test_grid_form_matches_oracle(ctx, nstep, step, corrL):
+    d = len(nstep)
+    got, ref = grid_case(ctx, nstep, step, (1.5, -2.0, 0.25)[:d], corrL[:d], 3, 5)
+    e = rel_rms(got, ref)
+    print(f"[parity] grid {nstep}: rel_rms={e:.3e} max_abs={np.max(np.abs(got - ref)):.3e}", flush=True)
+    assert e <= 1e-5
+
+
+def test_grid_form_lognormal_and_fp64_out(ctx):
+    got, ref = grid_case(ctx, (12, 11, 10), (0.25, 0.25, 0.25), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2, 9,
+                         dtype=np.float64, margi=R.FOM_LOGNORMAL)
+    assert rel_rms(got, ref) <= 1e-5
+
+
+def test_grid_form_rejects_other_methods(ctx):
+    import torch
+    p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.REF_COMPAT, Nmc=1, xRange=(2, 2, 2))
+    plan = R.FieldPlan(ctx, p)
+    out = torch.empty((1, 27), dtype=torch.float32, device="cuda")
+    with pytest.raises(R.RFError) as e:
+        plan.execute_grid((0, 0, 0), (1, 1, 1), (3, 3, 3), out)
+    assert e.value.code == R.RF_E_UNSUPPORTED
+    plan.close()
+
+
+def test_localization_pipeline_small(ctx):
+    """Uneven tensor-product subdomains (seed 100 + field index, last axis
+    fastest) generated on the grid form and blended with variance-preserving
+    weights, against the oracle doing the same per point."""
+    import torch
+    groupN = (40, 36, 20)
+    parts = (3, 2, 2)
+    nOvl = 5
+    step = (0.25, 0.25, 0.25)
+    Nmc = 2
+    origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
+    fields_d, fields_h = [], []
+    for s in range(len(sizes)):
+        n = [int(v) for v in sizes[s]]
+        o = [origins[s][a] * step[a] for a in range(3)]
+        ext = [(n[a] - 1) * step[a] for a in range(3)]
+        p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=Nmc,
+                          xRange=ext, seed=100 + s + 1)
+        plan = R.FieldPlan(ctx, p)
+        f = torch.empty((Nmc, int(np.prod(n))), dtype=torch.float32, device="cuda")
+        plan.execute_grid(o, step, n, f)
+        ctx.sync()
+        # the oracle, same device uniforms (tag 2 stream of the subdomain seed)
+        info = plan.info
+        du = torch.empty(Nmc * info.kNTotal, dtype=torch.float64, device="cuda")
+        R.device_uniforms(ctx, 100 + s + 1, 2, Nmc * info.kNTotal, du)
+        ctx.sync()
+        phU = du.cpu().numpy().reshape(Nmc, info.kNTotal)
+        po = O.make_plan(3, O.SHINOZUKA, O.FIXED, ext)
+        k = O.kpoints(po)
+        x = W.grid_points(n, step, o)
+        ref = O.shinozuka(po, x, k, O.skvec(po, k), phU)
+        assert rel_rms(f.cpu().numpy().astype(np.float64), ref) <= 1e-5
+        fields_d.append(f)
+        fields_h.append(ref)
+        plan.close()
+    g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float32, device="cuda")
+    R.blend(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields_d, g, R.DTYPE_F32)
+    ctx.sync()
+    gref = O.blend(groupN, Nmc, origins, sizes, [nOvl] * 3, masks, O.FIXED, fields_h)
+    e = rel_rms(g.cpu().numpy().astype(np.float64), gref)
+    print(f"[parity] localization pipeline: rel_rms={e:.3e}", flush=True)
+    assert e <= 1e-5

@@ -253,6 +253,103 @@ def run_cfg4(args):
                                "sample": f"24 points x 16 realizations x {info.kNTotal} wavevectors"})
 
 
+def run_cfg5(args):
+    """cfg 5: serving stream.  2,000 requests (seed 23) arriving as a Poisson
+    process at args.rate per second: n log-uniform in [1e3, 1e6] points
+    uniform in a box of side 10 corrL, corrL U[0.2, 3]^3, Nmc U{1..128},
+    method 1 or 2 (50/50), marginal gaussian or lognormal (avg 1, var 0.25).
+    Points are resident on the device before the clock starts; the server
+    admits by memory budget and batches continuously; a consumer releases
+    each field once it is ready."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import randomfield_amd as R
+    from oracle import oracle as O
+    from randomfield_amd.serving import Server
+    n_req = args.requests
+    rng = np.random.default_rng(23)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_req))
+    ns = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_req)).astype(np.int64)
+    corrLs = rng.uniform(0.2, 3.0, (n_req, 3))
+    nmcs = rng.integers(1, 129, n_req)
+    methods = np.where(rng.random(n_req) < 0.5, R.ISOTROPIC, R.SHINOZUKA)
+    margis = np.where(rng.random(n_req) < 0.5, R.FOM_GAUSSIAN, R.FOM_LOGNORMAL)
+    seeds = rng.integers(1, 1 << 40, n_req)
+    ctx = R.Context(0)
+    xs = []
+    for i in range(n_req):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(int(seeds[i]))
+        xs.append(torch.rand((int(ns[i]), 3), dtype=torch.float64, device="cuda", generator=g)
+                  * torch.tensor(10.0 * corrLs[i], dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+    srv = Server(ctx, hbm_budget=args.budget_gib << 30, max_inflight=3, max_batch=64)
+    # warm-up: one request of each method outside the trace
+    for m in (R.SHINOZUKA, R.ISOTROPIC):
+        rid = srv.submit(xs[0][:2000], m, 8, (1.0, 1.0, 1.0), 1)
+        srv.wait(rid)
+        srv.release(rid)
+    ids = [None] * n_req
+    lat = np.zeros(n_req)
+    done = np.zeros(n_req, dtype=bool)
+
+    def consumer():
+        for i in range(n_req):
+            while ids[i] is None:
+                time.sleep(20e-6)
+            res = srv.wait(ids[i])
+            lat[i] = res.tDone - res.tSubmit
+            done[i] = res.status == 0
+            srv.release(ids[i])
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(n_req):
+        while time.perf_counter() - t0 < arrivals[i]:
+            time.sleep(20e-6)
+        ids[i] = srv.submit(xs[i], int(methods[i]), int(nmcs[i]), corrLs[i], int(seeds[i]), int(margis[i]),
+                            1.0, 0.25)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    work = float(np.sum(ns * nmcs))
+    # CPU baseline: the FP64 oracle on the first 500 points of the first
+    # request of each method (all its realizations)
+    cpu_t, cpu_w = 0.0, 0.0
+    for m in (R.SHINOZUKA, R.ISOTROPIC):
+        i = int(np.nonzero(methods == m)[0][0])
+        xh = xs[i][:500].cpu().numpy() / corrLs[i]
+        po = O.make_plan(3, m, O.FIXED, [10.0, 10.0, 10.0])
+        k = O.kpoints(po)
+        U = np.random.default_rng(0).random((int(nmcs[i]), po.kNTotal * (4 if m == R.ISOTROPIC else 1)))
+        t = time.perf_counter()
+        if m == R.SHINOZUKA:
+            O.shinozuka(po, xh, k, O.skvec(po, k), U, threads=_threads())
+        else:
+            O.isotropic(po, xh, k, O.skvec(po, k), U, threads=_threads())
+        cpu_t += time.perf_counter() - t
+        cpu_w += 500.0 * int(nmcs[i])
+    ctx.close()
+    return _line("points x realizations / s (cfg5 serving stream, p50/p99 latency)", work / wall, 1, 0,
+                 wall * 1e3, "f16+f32",
+                 f"cfg5: {n_req} Poisson requests at {args.rate}/s, n 1e3..1e6, Nmc 1..128, methods 1/2, "
+                 "device-resident points, continuous batching",
+                 latency_ms={"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3),
+                             "max": float(lat.max() * 1e3)},
+                 requests={"completed": int(done.sum()), "batches": int(st.batches),
+                           "max_batch": int(st.maxBatchSeen), "max_inflight": int(st.maxInflightSeen),
+                           "arena_peak_GiB": st.arenaPeak / 2**30, "arena_GiB": st.arenaCapacity / 2**30,
+                           "offered_s": float(arrivals[-1]), "makespan_s": wall},
+                 cpu_baseline={"value": cpu_w / cpu_t, "unit": "points*realizations/s", "cores": _threads(),
+                               "kind": "port", "seconds": cpu_t,
+                               "sample": "500 points of the first request of each method, all its realizations"})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +360,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--rate", type=float, default=1000.0, help="cfg 5: arrivals per second")
+    ap.add_argument("--requests", type=int, default=2000, help="cfg 5: requests in the stream")
+    ap.add_argument("--budget-gib", type=int, default=64, help="cfg 5: server memory budget")
     ap.add_argument("--config", type=int, default=3,
                     help="benchmark configuration 1..5 (3, the headline workload, by default)")
     args = ap.parse_args()
@@ -270,7 +370,7 @@ def main() -> int:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             return 2
-        runners = {1: run_cfg1, 2: run_cfg2, 4: run_cfg4}
+        runners = {1: run_cfg1, 2: run_cfg2, 4: run_cfg4, 5: run_cfg5}
         if args.config not in runners:
             print(f"unknown configuration {args.config}", file=sys.stderr)
             return 2
@@ -307,6 +407,16 @@ def main() -> int:
     plan = R.FieldPlan(ctx, p)
     dx = torch.from_numpy(x).cuda()
     out = torch.empty((N_MC, N_POINTS), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    # launch shapes: each timed briefly, then the library's own autotuner
+    # picks the one used below
+    variants_ms = []
+    for v in range(4):
+        plan.set_variant(v)
+        _, vms = _timed(lambda: plan.execute(dx, out, stream=sptr), 3, 1)
+        variants_ms.append(float(vms.mean()))
+    chosen = plan.autotune(dx, out, stream=sptr)
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):
@@ -413,7 +523,8 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_F16_DENSE / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_F16_DENSE, "traffic": traffic,
                      "kernel_ms": float(kms.mean()),
-                     "flop_per_launch": flop_alg, "executed_mfma_flop_per_launch": 3 * flop_alg},
+                     "flop_per_launch": flop_alg, "executed_mfma_flop_per_launch": 3 * flop_alg,
+                     "variant": chosen, "variants_kernel_ms": variants_ms},
         "latency_ms": {"p50": float(np.percentile(kms, 50)), "p99": float(np.percentile(kms, 99))},
         "direct_fp32": direct,
         "accuracy": acc,

@@ -154,6 +154,17 @@ int rf_plan_info(const rf_field_plan* plan, rf_spectrum_info* info);
  * d_out: device N x Nmc in outDtype.  Stream-ordered, asynchronous. */
 int rf_plan_execute(rf_field_plan* plan, const double* d_x, int64_t N,
                     void* d_out, int32_t outDtype, void* stream);
+/* Launch shape of the MFMA forms (no counterpart in the reference, whose
+ * sum has one code path): variant 0..3 = {4 waves, 1 K-step per LDS stage},
+ * {4, 2}, {8, 1}, {8, 2}.  Results are identical across variants (same
+ * per-point operation order); only the speed differs.  rf_plan_autotune
+ * times every variant on the caller's points (each runs three times, the
+ * last run's result stays in d_out), keeps the fastest in the plan and
+ * returns it in *chosen; for the FP64/FP32 forms it does nothing. */
+int rf_plan_set_variant(rf_field_plan* plan, int32_t variant);
+int rf_plan_autotune(rf_field_plan* plan, const double* d_x, int64_t N,
+                     void* d_out, int32_t outDtype, void* stream,
+                     int32_t* chosen);
 /* The same sum on a structured grid x = origin + (i1,i2,i3)*step (caller
  * units, x fastest; the point order of set_XPoints/setGrid, mesh_RF.f90:44,
  * math_RF.f90:70-88), evaluated axis by axis: the Shinozuka lattice is a
@@ -218,6 +229,85 @@ void   rf_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2],
                              uint32_t out[4]);
 /* Streams of the device generator: wavevector draws use tag 1, phase draws
  * tag 2 (element = realization * nPhaseUniform + index). */
+
+/* ---- serving: field requests with continuous batching ----------------------
+ * No counterpart in the reference, whose unit of work is one
+ * createRandomField call per MPI rank; a server answers many such calls
+ * (single_realization, sample_RF.f90:309-372, as requests) on one GPU.
+ *
+ * Every request is a 3-D FIXED-mode field on the server's lattice: the
+ * Shinozuka lattice (method 2, MFMA form of the server) or the isotropic
+ * shells (method 1, FP32 direct form with per-realization directions) of the
+ * normalised extent cfg.xRangeNorm (corrL 1).  A request's points are
+ * divided by its own corrL, so a request whose box is xRangeNorm * corrL
+ * gets exactly the field rf_plan_create would build for it.  Phases come
+ * from the device generator (seed, tag 2), as in rf_plan_create with
+ * phaseUniform NULL.
+ *
+ * A worker thread admits queued requests in arrival order while their
+ * device memory fits the server's budget (one pre-allocated block, HBM
+ * admission), groups them into batches (one grouped launch per method),
+ * and keeps up to cfg.maxInflight batches on the device at once, each on
+ * its own stream: requests that arrive while others run are admitted as
+ * memory and a stream slot free up, and their work overlaps.  A request's
+ * result depends only on the request (per-point operation order is fixed),
+ * not on what it was batched with.  Outputs stay in the server's memory
+ * until rf_server_release. */
+typedef struct rf_server rf_server;
+
+typedef struct rf_server_config {
+    int64_t hbmBudget;     /* bytes the server may hold; 0 = half the free memory */
+    int32_t maxInflight;   /* batches on the device at once, 1..8; 0 = 3 */
+    int32_t maxBatch;      /* requests per batch; 0 = 64 */
+    int32_t form;          /* method-2 form: RF_FORM_MFMA_F16 (0 = default) or _BF16 */
+    int32_t maxNmc;        /* largest Nmc accepted; 0 = 1024 */
+    double  xRangeNorm[3]; /* lattice extent in correlation lengths; 0 = 10 */
+    double  batchTerms;    /* batch closes above this many N*Nmc*N_k terms; 0 = 5e11 */
+    int32_t reserved[8];
+} rf_server_config;
+
+typedef struct rf_request {
+    int32_t method;        /* RF_SHINOZUKA or RF_ISOTROPIC */
+    int32_t Nmc;
+    int32_t margiFirst;    /* RF_FOM_* (lognormal needs fieldAvg > 0) */
+    int32_t xOnDevice;     /* 1: x is a device pointer, 0: host (copied at launch) */
+    int64_t N;
+    const double* x;       /* 3 x N (point-major), valid until the request completes */
+    double corrL[3];
+    double fieldAvg, fieldVar;
+    uint64_t seed;
+    int32_t reserved[6];
+} rf_request;
+
+typedef struct rf_request_result {
+    int32_t status;        /* 0 done, 1 queued/running, < 0 failed (RF_E_*) */
+    int32_t Nmc;
+    int64_t N;
+    const float* d_out;    /* device, N x Nmc FP32 (realization-major rows of N) */
+    double tSubmit, tLaunch, tDone;   /* steady-clock seconds */
+    int64_t batch;         /* batch the request ran in */
+} rf_request_result;
+
+typedef struct rf_server_stats {
+    int64_t submitted, completed, failed, batches;
+    int64_t arenaCapacity, arenaUsed, arenaPeak;
+    int32_t maxBatchSeen, maxInflightSeen;
+    double  pointRealizations;   /* sum of N * Nmc over completed requests */
+} rf_server_stats;
+
+int rf_server_create(rf_context* ctx, const rf_server_config* cfg, rf_server** server);
+/* Fails queued requests, waits for the batches on the device, frees all. */
+int rf_server_destroy(rf_server* server);
+/* Thread-safe.  RF_E_NOMEM if the request can never fit the budget. */
+int rf_server_submit(rf_server* server, const rf_request* req, int64_t* id);
+/* Blocks until request id completes (timeoutSec < 0: no limit); status 1 in
+ * res on time-out.  RF_E_ARG for an unknown or released id. */
+int rf_server_wait(rf_server* server, int64_t id, double timeoutSec, rf_request_result* res);
+/* Copies a completed request's field to a host buffer (N x Nmc FP32). */
+int rf_server_read(rf_server* server, int64_t id, float* host_out);
+/* Frees a completed request's output; its id becomes unknown. */
+int rf_server_release(rf_server* server, int64_t id);
+int rf_server_get_stats(rf_server* server, rf_server_stats* stats);
 
 #ifdef __cplusplus
 }

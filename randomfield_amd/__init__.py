@@ -155,6 +155,17 @@ class FieldPlan:
         check(lib().rf_plan_execute(self._h, _ptr(x), N, _ptr(out), _dtype_code(out.dtype),
                                     _ptr(stream)), self.ctx.handle)
 
+    def set_variant(self, variant: int) -> None:
+        """MFMA launch shape 0..3 (waves per block x K-steps per LDS stage)."""
+        check(lib().rf_plan_set_variant(self._h, int(variant)), self.ctx.handle)
+
+    def autotune(self, x, out, stream=None) -> int:
+        """Times every MFMA launch shape on these points, keeps the fastest."""
+        chosen = C.c_int32(0)
+        check(lib().rf_plan_autotune(self._h, _ptr(x), int(x.shape[0]), _ptr(out), _dtype_code(out.dtype),
+                                     _ptr(stream), C.byref(chosen)), self.ctx.handle)
+        return int(chosen.value)
+
     def execute_grid(self, origin, step, nstep, out, stream=None) -> None:
         """Structured grid (x fastest): out device (Nmc, prod(nstep))."""
         o = np.zeros(3); o[: len(origin)] = origin

@@ -25,9 +25,11 @@ DTYPE_F64, DTYPE_F32 = 0, 1
 EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
-    "rf_plan_execute", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
+    "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
     "rf_stats_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
+    "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_read",
+    "rf_server_release", "rf_server_get_stats",
 )
 
 
@@ -48,6 +50,38 @@ class SpectrumInfo(C.Structure):
         ("nPhaseUniform", C.c_int64), ("kNStep", C.c_int32 * 3), ("nSignBlocks", C.c_int32),
         ("nVarIso", C.c_int32), ("pad", C.c_int32), ("kMax", C.c_double * 3),
         ("kDelta", C.c_double * 3), ("ampMult", C.c_double), ("isoStep", C.c_double),
+    ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbmBudget", C.c_int64), ("maxInflight", C.c_int32), ("maxBatch", C.c_int32),
+        ("form", C.c_int32), ("maxNmc", C.c_int32), ("xRangeNorm", C.c_double * 3),
+        ("batchTerms", C.c_double), ("reserved", C.c_int32 * 8),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32), ("Nmc", C.c_int32), ("margiFirst", C.c_int32), ("xOnDevice", C.c_int32),
+        ("N", C.c_int64), ("x", C.c_void_p), ("corrL", C.c_double * 3),
+        ("fieldAvg", C.c_double), ("fieldVar", C.c_double), ("seed", C.c_uint64),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("Nmc", C.c_int32), ("N", C.c_int64), ("d_out", C.c_void_p),
+        ("tSubmit", C.c_double), ("tLaunch", C.c_double), ("tDone", C.c_double), ("batch", C.c_int64),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("batches", C.c_int64),
+        ("arenaCapacity", C.c_int64), ("arenaUsed", C.c_int64), ("arenaPeak", C.c_int64),
+        ("maxBatchSeen", C.c_int32), ("maxInflightSeen", C.c_int32), ("pointRealizations", C.c_double),
     ]
 
 
@@ -91,6 +125,8 @@ def lib() -> C.CDLL:
     L.rf_plan_destroy.argtypes = [vp]
     L.rf_plan_info.argtypes = [vp, C.POINTER(SpectrumInfo)]
     L.rf_plan_execute.argtypes = [vp, vp, i64, vp, i32, vp]
+    L.rf_plan_set_variant.argtypes = [vp, i32]
+    L.rf_plan_autotune.argtypes = [vp, vp, i64, vp, i32, vp, vp]
     L.rf_plan_execute_grid.argtypes = [vp, vp, vp, vp, vp, i32, vp]
     L.rf_generate.argtypes = [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32]
     L.rf_marginal_device.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp]
@@ -102,6 +138,13 @@ def lib() -> C.CDLL:
     L.rf_uniform_host.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
     L.rf_uniform_host.restype = C.c_double
     L.rf_philox4x32_10_host.argtypes = [vp, vp, vp]
+    L.rf_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.rf_server_destroy.argtypes = [vp]
+    L.rf_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]
+    L.rf_server_wait.argtypes = [vp, i64, C.c_double, C.POINTER(RequestResult)]
+    L.rf_server_read.argtypes = [vp, i64, vp]
+    L.rf_server_release.argtypes = [vp, i64]
+    L.rf_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     _lib = L
     return L
 

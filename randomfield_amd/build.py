@@ -58,7 +58,7 @@ def build_hip(jobs: int = 8) -> Path:
         list(ex.map(_run, cmds))
     out = LIB / "librf_hip.so"
     tmp = LIB / "librf_hip.so.tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)
     return out
 

@@ -1,8 +1,9 @@
 // rf_blend.cpp -- host side of rf_blend_device: analyses the subdomain boxes
 // (a tensor-product arrangement, as prepare_Localization's setGrid of
-// subdivision origins produces, localization_RF.f90:151-155), builds the
-// per-axis cover tables and the separable partition-of-unity factors, and
-// launches the gather-form blend kernel.
+// subdivision origins produces, localization_RF.f90:151-155), splits each
+// axis into segments of constant cover, lists the covering subdomains of
+// every segment product (cell) in ascending order, tabulates the separable
+// partition-of-unity factors, and launches the gather-form blend kernel.
 //
 // Partition of unity (generateUnityPartition_Matrix, localization_RF.f90:705-857):
 // the weight of a subdomain point is a product over tapered faces of a
@@ -115,6 +116,63 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
             }
         }
     }
+    // segments: runs of group indices with the same covering intervals
+    std::vector<int32_t> segOf[3];
+    std::vector<std::pair<int32_t, int32_t>> segCover[3];
+    int nSeg[3] = {1, 1, 1};
+    for (int a = 0; a < 3; a++) {
+        if (a >= nDim) {
+            segOf[a].assign(1, 0);
+            segCover[a].push_back({0, -1});
+            continue;
+        }
+        segOf[a].resize((size_t)groupN[a]);
+        for (int g = 0; g < groupN[a]; g++) {
+            const std::pair<int32_t, int32_t> key{cover[a][(size_t)g * 2], cover[a][(size_t)g * 2 + 1]};
+            if (segCover[a].empty() || segCover[a].back() != key) segCover[a].push_back(key);
+            segOf[a][(size_t)g] = (int32_t)segCover[a].size() - 1;
+        }
+        nSeg[a] = (int)segCover[a].size();
+    }
+    // cells: the covering subdomains of each segment product, ascending
+    const int64_t nCell = (int64_t)nSeg[0] * nSeg[1] * nSeg[2];
+    std::vector<int32_t> cellOff;
+    std::vector<rf_blend_entry> entries;
+    cellOff.reserve((size_t)nCell + 1);
+    for (int64_t c = 0; c < nCell; c++) {
+        const int c0 = (int)(c % nSeg[0]), c1 = (int)((c / nSeg[0]) % nSeg[1]), c2 = (int)(c / ((int64_t)nSeg[0] * nSeg[1]));
+        cellOff.push_back((int32_t)entries.size());
+        const int32_t iv[3][2] = {{segCover[0][c0].first, segCover[0][c0].second},
+                                  {segCover[1][c1].first, segCover[1][c1].second},
+                                  {segCover[2][c2].first, segCover[2][c2].second}};
+        std::vector<int32_t> subs;
+        for (int k2 = 0; k2 < 2; k2++)
+            for (int k1 = 0; k1 < 2; k1++)
+                for (int k0 = 0; k0 < 2; k0++) {
+                    if (iv[0][k0] < 0 || iv[1][k1] < 0 || iv[2][k2] < 0) continue;
+                    const int32_t sb = subOf[(size_t)(iv[0][k0] + (int64_t)nInt[0] * (iv[1][k1] + (int64_t)nInt[1] * iv[2][k2]))];
+                    if (sb >= 0) subs.push_back(sb);
+                }
+        std::sort(subs.begin(), subs.end());
+        if ((int)subs.size() > kBlendMaxEntries)
+            return rf_set_error(ctx, RF_E_UNSUPPORTED, "more than 2^d subdomains cover a group point");
+        for (int32_t sb : subs) {
+            rf_blend_entry e{};
+            int64_t n[3] = {1, 1, 1};
+            for (int ax = 0; ax < nDim; ax++) {
+                n[ax] = sizes[sb * nDim + ax];
+                e.o[ax] = origins[sb * nDim + ax];
+            }
+            e.field = d_fields[sb];
+            e.subTot = n[0] * n[1] * n[2];
+            e.stride1 = n[0];
+            e.stride2 = n[0] * n[1];
+            e.base = -((int64_t)e.o[0] + (int64_t)e.o[1] * e.stride1 + (int64_t)e.o[2] * e.stride2);
+            e.s = sb;
+            entries.push_back(e);
+        }
+    }
+    cellOff.push_back((int32_t)entries.size());
     // taper factor tables
     std::vector<double> lowF((size_t)nSub * 3 * maxSize, 1.0), upF((size_t)nSub * 3 * maxSize, 1.0);
     for (int s = 0; s < nSub; s++)
@@ -129,15 +187,6 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
         if (mode == RF_REF_COMPAT)
             axis_factors(groupN[a], std::min(nOvl[a], groupN[a]), true, true, mode, pu1[a][0].data(), pu1[a][1].data());
     }
-    std::vector<int64_t> subTot((size_t)nSub);
-    for (int s = 0; s < nSub; s++) {
-        int64_t t = 1;
-        for (int a = 0; a < nDim; a++) t *= sizes[s * nDim + a];
-        subTot[(size_t)s] = t;
-    }
-    std::vector<int32_t> intO[3], intZ[3];
-    for (int a = 0; a < nDim; a++)
-        for (auto& iv : ints[a]) { intO[a].push_back(iv.first); intZ[a].push_back(iv.second); }
 
     // one device block for every table
     std::vector<char> blob;
@@ -147,19 +196,16 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
         if (bytes) std::memcpy(blob.data() + off, src, bytes);
         return off;
     };
-    size_t oCover[3] = {0, 0, 0}, oIO[3] = {0, 0, 0}, oIZ[3] = {0, 0, 0}, oP1[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    size_t oSeg[3] = {0, 0, 0}, oP1[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int a = 0; a < 3; a++) oSeg[a] = put(segOf[a].data(), segOf[a].size() * 4);
     for (int a = 0; a < nDim; a++) {
-        oCover[a] = put(cover[a].data(), cover[a].size() * 4);
-        oIO[a] = put(intO[a].data(), intO[a].size() * 4);
-        oIZ[a] = put(intZ[a].data(), intZ[a].size() * 4);
         oP1[a][0] = put(pu1[a][0].data(), pu1[a][0].size() * 8);
         oP1[a][1] = put(pu1[a][1].data(), pu1[a][1].size() * 8);
     }
-    const size_t oSub = put(subOf.data(), subOf.size() * 4);
-    const size_t oTot = put(subTot.data(), subTot.size() * 8);
+    const size_t oCell = put(cellOff.data(), cellOff.size() * 4);
+    const size_t oEnt = put(entries.data(), entries.size() * sizeof(rf_blend_entry));
     const size_t oLow = put(lowF.data(), lowF.size() * 8);
     const size_t oUp = put(upF.data(), upF.size() * 8);
-    const size_t oF = put(d_fields, (size_t)nSub * sizeof(void*));
 
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
@@ -171,22 +217,19 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
         b.nDim = nDim; b.Nmc = Nmc; b.mode = mode;
         for (int a = 0; a < 3; a++) {
             b.groupN[a] = a < nDim ? groupN[a] : 1;
-            b.nInt[a] = nInt[a];
+            b.nSeg[a] = nSeg[a];
+            b.seg[a] = reinterpret_cast<const int32_t*>(d_blob + oSeg[a]);
             if (a < nDim) {
-                b.cover[a] = reinterpret_cast<const int32_t*>(d_blob + oCover[a]);
-                b.intOrigin[a] = reinterpret_cast<const int32_t*>(d_blob + oIO[a]);
-                b.intSize[a] = reinterpret_cast<const int32_t*>(d_blob + oIZ[a]);
                 b.pu1Low[a] = reinterpret_cast<const double*>(d_blob + oP1[a][0]);
                 b.pu1Up[a] = reinterpret_cast<const double*>(d_blob + oP1[a][1]);
             }
         }
         b.groupTot = gTot;
-        b.subOf = reinterpret_cast<const int32_t*>(d_blob + oSub);
-        b.subTot = reinterpret_cast<const int64_t*>(d_blob + oTot);
+        b.cellOff = reinterpret_cast<const int32_t*>(d_blob + oCell);
+        b.entries = reinterpret_cast<const rf_blend_entry*>(d_blob + oEnt);
         b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
         b.upF = reinterpret_cast<const double*>(d_blob + oUp);
         b.maxSize = maxSize;
-        b.fields = reinterpret_cast<const void* const*>(d_blob + oF);
         e = rf_launch_blend_dev(b, d_group, dtype, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // tables freed below

@@ -1,27 +1,41 @@
 // rf_blend.h -- device-side description of a tensor-product arrangement of
 // overlapping subdomains, consumed by the gather-form blend kernel.
+//
+// Along each axis the group index range splits into segments over which the
+// set of covering subdomain intervals is constant; a cell is a product of
+// per-axis segments, and every group point of a cell reads the same <= 2^d
+// subdomains.  The cell's entries are listed in ascending subdomain index,
+// the order in which the reference adds fields into the group.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+struct rf_blend_entry {
+    const void* field;            // subdomain field, [Nmc][subTot]
+    int64_t subTot;               // points of the subdomain (realization stride)
+    int64_t base;                 // -(o0 + o1 * n0 + o2 * n0 * n1)
+    int64_t stride1, stride2;     // n0, n0 * n1
+    int32_t s;                    // subdomain index (taper tables)
+    int32_t o[3];                 // box origin in group indices
+};
+
 struct rf_blend_dev {
     int nDim, Nmc, mode;
     int groupN[3];
-    int nInt[3];                  // distinct subdomain intervals per axis
+    int nSeg[3];
     int64_t groupTot;
-    const int32_t* cover[3];      // [g][2]: intervals covering group index g (-1: none)
-    const int32_t* intOrigin[3];  // [interval] first group index
-    const int32_t* intSize[3];    // [interval] points
-    const int32_t* subOf;         // [i0 + nInt0*(i1 + nInt1*i2)] -> subdomain (-1: none)
-    const int64_t* subTot;        // [s] points of subdomain s
+    const int32_t* seg[3];        // [g] segment of group index g along axis a
+    const int32_t* cellOff;       // [cell .. cell + 1]: entry range of a cell
+    const rf_blend_entry* entries;
     const double* lowF;           // [s][3][maxSize] lower-face taper factor by position
     const double* upF;            // [s][3][maxSize] upper-face taper factor by position
     int maxSize;
     const double* pu1Low[3];      // [g] reverse taper of the group grid (REF_COMPAT)
     const double* pu1Up[3];
-    const void* const* fields;    // device array of device pointers
 };
+
+constexpr int kBlendMaxEntries = 8;
 
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s);
 int rf_moments_parts(int64_t N);

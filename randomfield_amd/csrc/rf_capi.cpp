@@ -32,23 +32,6 @@ void dfree(void* p) {
 
 int plan_fail(rf_context* ctx, int code, const std::string& msg) { return rf_set_error(ctx, code, msg); }
 
-// Marginal-map constants of multiVariateTransformation (randomFieldND.f90:154-169).
-rf_epilogue make_epilogue(int margi, double avg, double var) {
-    rf_epilogue e{};
-    e.margi = margi;
-    e.scale = 1.0;
-    e.shift = 0.0;
-    if (margi == RF_FOM_GAUSSIAN) {
-        e.scale = std::sqrt(var);
-        e.shift = avg;
-    } else if (margi == RF_FOM_LOGNORMAL) {
-        const double nv = std::log(1.0 + var / (avg * avg));
-        e.scale = std::sqrt(nv);
-        e.shift = std::log(avg) - nv / 2.0;
-    }
-    return e;
-}
-
 bool is_iso_direct(const rf_params& p) { return p.method == RF_ISOTROPIC && !p.isoShared; }
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -70,20 +53,10 @@ int build_form_operands(rf_field_plan* P, hipStream_t s) {
         }
     } else if (P->p.form == RF_FORM_MFMA_F16 || P->p.form == RF_FORM_MFMA_BF16) {
         RF_HIP_TRY(ctx, rf_launch_mfma_prep(P->p.form, P->p.nDim, kN, P->kNpad, Nmc, P->Rpad,
-                                            P->d_kRev, P->d_phase, P->d_amp, ampMult * P->bScale,
+                                            P->d_kRev, P->d_phase, P->d_amp, kN, ampMult * P->bScale,
                                             P->d_B, s));
     }
     return RF_OK;
-}
-
-// Power-of-two scale that brings the largest |amp * ampMult| near 2^14, so the
-// FP16 hi/lo split keeps small amplitudes out of the subnormal range.
-double mfma_bscale(const rf_field_plan* P, double maxAmp) {
-    const double m = maxAmp * P->spec.info.ampMult;
-    if (!(m > 0.0)) return 1.0;
-    int e = 0;
-    std::frexp(m, &e);   // m = f * 2^e, f in [0.5, 1)
-    return std::ldexp(1.0, 14 - e);
 }
 
 }  // namespace
@@ -232,7 +205,7 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
     if (calculate)
         for (int r = 0; r < Nmc; r++) P->calc[r] = calculate[r] ? 1 : 0;
     P->nCalc = (int)std::count(P->calc.begin(), P->calc.end(), (uint8_t)1);
-    P->epi = make_epilogue(p->margiFirst, p->fieldAvg, p->fieldVar);
+    P->epi = rf_make_epilogue(p->margiFirst, p->fieldAvg, p->fieldVar);
 
     // phase uniforms: Nmc x nPhaseUniform, realization-major
     const int64_t nPU = info.nPhaseUniform * Nmc;
@@ -314,7 +287,7 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
         RF_HIP_TRY(ctx, rf_launch_krev(kN * d, P->d_k, P->d_kRev, s));
     }
     if (mfma) {
-        P->bScale = mfma_bscale(P.get(), maxAmp);
+        P->bScale = rf_mfma_bscale(P->spec.info.ampMult, maxAmp);
         RF_HIP_TRY(ctx, hipMalloc(&P->d_B, rf_mfma_bbytes(p->form, P->kNpad, P->Rpad)));
     }
     const bool perSlab = (p->mode == RF_REF_COMPAT && p->method != RF_ISOTROPIC);
@@ -374,7 +347,7 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
                 else e = rf_launch_sum_fp32(a, P->d_kRev, P->d_phT, P->d_aT, P->Rpad, s);
                 break;
             default:
-                e = rf_launch_sum_mfma(p.form, a, P->kNpad, P->Rpad, P->d_B, 1.0 / P->bScale, s);
+                e = rf_launch_sum_mfma(p.form, P->mfmaVariant, a, P->kNpad, P->Rpad, P->d_B, 1.0 / P->bScale, s);
                 break;
         }
         if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("sum kernel launch: ") + hipGetErrorString(e));
@@ -415,6 +388,60 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
     return RF_OK;
 }
 
+int rf_plan_set_variant(rf_field_plan* P, int32_t variant) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
+    if (variant < 0 || variant >= rf_mfma_num_variants())
+        return rf_set_error(P->ctx, RF_E_ARG, "MFMA variant out of range");
+    P->mfmaVariant = variant;
+    return RF_OK;
+}
+
+int rf_plan_autotune(rf_field_plan* P, const double* d_x, int64_t N, void* d_out, int32_t outDtype,
+                     void* stream, int32_t* chosen) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
+    rf_context* ctx = P->ctx;
+    if (P->p.form != RF_FORM_MFMA_F16 && P->p.form != RF_FORM_MFMA_BF16) {
+        if (chosen) *chosen = 0;
+        return RF_OK;                           // one launch shape only
+    }
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RF_HIP_TRY(ctx, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return rf_set_error(ctx, RF_E_HIP, "hipEventCreate failed");
+    }
+    int best = P->mfmaVariant, rc = RF_OK;
+    float bestMs = 0.0f;
+    for (int v = 0; v < rf_mfma_num_variants() && rc == RF_OK; v++) {
+        P->mfmaVariant = v;
+        rc = rf_plan_execute(P, d_x, N, d_out, outDtype, s);            // warm-up
+        if (rc) break;
+        hipError_t e = hipEventRecord(e0, s);
+        for (int rep = 0; rep < 2 && rc == RF_OK && e == hipSuccess; rep++)
+            rc = rf_plan_execute(P, d_x, N, d_out, outDtype, s);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = rf_set_error(ctx, RF_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        if (v == 0 || ms < bestMs) {
+            bestMs = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->mfmaVariant = best;
+    if (chosen) *chosen = best;
+    return rc;
+}
+
 int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* step,
                          const int32_t* nstep, void* d_out, int32_t outDtype, void* stream) {
     if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
@@ -429,6 +456,15 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     for (int a = 0; a < p.nDim; a++) {
         if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
         if (!(step[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid steps must be > 0");
+    }
+    {
+        // every GEMM dimension of the three stages is an int32
+        const int32_t* kn = P->spec.info.kNStep;
+        int64_t rows1 = (int64_t)P->spec.info.nSignBlocks * p.Nmc, plane = 1;
+        for (int a = 1; a < p.nDim; a++) rows1 *= kn[a];
+        for (int a = 0; a + 1 < p.nDim; a++) plane *= nstep[a];
+        if (rows1 > INT32_MAX || plane > INT32_MAX)
+            return rf_set_error(ctx, RF_E_ARG, "grid or lattice too large for the grid form");
     }
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
@@ -491,7 +527,7 @@ int rf_marginal_device(rf_context* ctx, int32_t margiFirst, double fieldAvg, dou
     if (count < 0 || (count > 0 && !d_field)) return rf_set_error(ctx, RF_E_ARG, "bad field buffer");
     if (margiFirst == RF_FOM_NONE || count == 0) return RF_OK;
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const rf_epilogue e = make_epilogue(margiFirst, fieldAvg, fieldVar);
+    const rf_epilogue e = rf_make_epilogue(margiFirst, fieldAvg, fieldVar);
     RF_HIP_TRY(ctx, rf_launch_marginal(margiFirst, e.scale, e.shift, d_field, count, dtype,
                                        stream ? (hipStream_t)stream : ctx->stream));
     return RF_OK;

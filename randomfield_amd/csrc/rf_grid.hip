@@ -45,13 +45,19 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
     return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
 }
 
+// RealOnly: only Re(D) is formed (2 FMAs per complex product instead of 4);
+// the last stage, whose output is the real field, uses it.
+template <bool RealOnly>
 __global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
     __shared__ float2 sA[kTK][kTM + 1];
     __shared__ float2 sB[kTK][kTN + 1];
-    const int batch = blockIdx.z;
-    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+    const int n0 = blockIdx.x * kTN;
     const int tid = threadIdx.x;
     const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
+    // grid y and z are capped at 65535 by the launcher; larger row-tile and
+    // batch counts are covered by these block-uniform loops
+    for (int batch = blockIdx.z; batch < g.batches; batch += gridDim.z)
+    for (int m0 = blockIdx.y * kTM; m0 < g.M; m0 += gridDim.y * kTM) {
     const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB;
     const float2* B = g.B + (int64_t)batch * g.bStrideB;
     float2 acc[4][4];
@@ -83,7 +89,14 @@ __global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = cmul_add(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; j++) {
+                    if (RealOnly) {
+                        acc[i][j].x = fmaf(a[i].x, b[j].x, acc[i][j].x);
+                        acc[i][j].x = fmaf(-a[i].y, b[j].y, acc[i][j].x);
+                    } else {
+                        acc[i][j] = cmul_add(a[i], b[j], acc[i][j]);
+                    }
+                }
         }
         __syncthreads();
     }
@@ -104,6 +117,7 @@ __global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
                 else static_cast<float*>(g.D)[o] = u;
             }
         }
+    }
     }
 }
 
@@ -147,8 +161,11 @@ __global__ void k_coeffs(int64_t n, const double* __restrict__ amp, const double
 }
 
 hipError_t gemm(const GemmDesc& g, hipStream_t s) {
-    dim3 grid((unsigned)((g.N + kTN - 1) / kTN), (unsigned)((g.M + kTM - 1) / kTM), (unsigned)g.batches);
-    hipLaunchKernelGGL(k_cgemm, grid, dim3(256), 0, s, g);
+    const int64_t mt = ((int64_t)g.M + kTM - 1) / kTM;
+    dim3 grid((unsigned)((g.N + kTN - 1) / kTN), (unsigned)(mt < 65535 ? mt : 65535),
+              (unsigned)(g.batches < 65535 ? g.batches : 65535));
+    if (g.realOut) hipLaunchKernelGGL(k_cgemm<true>, grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(k_cgemm<false>, grid, dim3(256), 0, s, g);
     return hipGetLastError();
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,7 @@ struct rf_field_plan {
     rf_epilogue epi{};
     void* d_gridScratch = nullptr;   // structured-grid form (grown on demand)
     size_t gridScratchBytes = 0;
+    int mfmaVariant = 0;         // MFMA launch shape (rf_plan_set_variant / rf_plan_autotune)
 };
 
 // ---- host spectrum (rf_spectrum.cpp)
@@ -115,13 +117,70 @@ int rf_fp32_rchunk();
 
 // MFMA form (rf_mfma.hip)
 int rf_mfma_kpad(int form);
+int rf_mfma_num_variants();
 hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, int Nmc, int Npad,
                                const double* d_kRev, const double* d_phase, const double* d_amp,
-                               double scale, void* d_B, hipStream_t s);
-hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, int64_t kNpad, int Npad,
+                               int64_t ampStrideR, double scale, void* d_B, hipStream_t s);
+hipError_t rf_launch_sum_mfma(int form, int variant, const rf_sum_args& a, int64_t kNpad, int Npad,
                               const void* d_B, double outScale, hipStream_t s);
 int rf_mfma_npad(int Nmc);
+// Power-of-two scale that brings the largest |amp * ampMult| near 2^14, so the
+// FP16 hi/lo split keeps small amplitudes out of the subnormal range.
+inline double rf_mfma_bscale(double ampMult, double maxAmp) {
+    const double m = maxAmp * ampMult;
+    if (!(m > 0.0)) return 1.0;
+    int e = 0;
+    std::frexp(m, &e);   // m = f * 2^e, f in [0.5, 1)
+    return std::ldexp(1.0, 14 - e);
+}
 size_t rf_mfma_bbytes(int form, int64_t kNpad, int Npad);
+
+// grouped launches: a batch of independent fields (serving requests) in one
+// launch.  Field q owns tiles tile0 .. tile0 + nTiles(q) - 1 (ascending in q);
+// block b finds its field by binary search over tile0.
+struct rf_group_req {
+    rf_sum_args a;        // points, output, epilogue of the field (i0 = 0, i1 = N)
+    const void* ops0;     // MFMA: coefficient panels; direct isotropic: kT
+    const void* ops1;     // MFMA: wavevector stream;  direct isotropic: pT
+    const void* ops2;     //                           direct isotropic: aT
+    int64_t tile0;        // first tile of the field
+    int64_t nSteps;       // MFMA K-steps (multiple of 2)
+    int32_t mtTotal;      // MFMA M-tiles of the panels
+    int32_t Rpad;         // direct isotropic: padded realizations (multiple of 16)
+    float outScale;       // MFMA: 1 / panel scale
+    int32_t pad;
+};
+constexpr int kGroupMT = 4;       // M-tiles per tile, grouped MFMA kernel
+constexpr int kGroupW = 8;        // waves per block, grouped MFMA kernel (256 points)
+constexpr int kGroupIsoPts = 256; // points per tile, grouped direct isotropic kernel
+constexpr int kGroupIsoR = 16;    // realizations per tile, grouped direct isotropic kernel
+// tiles of one field
+inline int64_t rf_group_tiles_mfma(int64_t N, int mtTotal) {
+    return (N + 32 * kGroupW - 1) / (32 * kGroupW) * ((mtTotal + kGroupMT - 1) / kGroupMT);
+}
+inline int64_t rf_group_tiles_iso(int64_t N, int Rpad) {
+    return (N + kGroupIsoPts - 1) / kGroupIsoPts * (Rpad / kGroupIsoR);
+}
+hipError_t rf_launch_group_mfma(int form, const rf_group_req* d_reqs, int nReq, int64_t nTiles,
+                                hipStream_t s);
+hipError_t rf_launch_group_iso(const rf_group_req* d_reqs, int nReq, int64_t nTiles, hipStream_t s);
+
+// epilogue constants of multiVariateTransformation (randomFieldND.f90:154-169)
+inline rf_epilogue rf_make_epilogue(int margi, double avg, double var) {
+    rf_epilogue e{};
+    e.margi = margi;
+    e.scale = 1.0;
+    e.shift = 0.0;
+    if (margi == RF_FOM_GAUSSIAN) {
+        e.scale = std::sqrt(var);
+        e.shift = avg;
+    } else if (margi == RF_FOM_LOGNORMAL) {
+        const double nv = std::log(1.0 + var / (avg * avg));
+        e.scale = std::sqrt(nv);
+        e.shift = std::log(avg) - nv / 2.0;
+    }
+    return e;
+}
 
 // structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);

@@ -33,11 +33,15 @@ typedef float f16x __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 
-constexpr int kWaves = 4;               // waves per workgroup
-constexpr int kThreads = 64 * kWaves;
-constexpr int kPointsPerBlock = 32 * kWaves;
 constexpr int kMaxMT = 8;               // up to 256 realizations per block
 constexpr int kStepK = 8;               // wavevectors per K-step (16 K entries)
+constexpr int kKPad = 2 * kStepK;       // K padding: whole stages of up to 2 steps
+
+// launch variants: waves per workgroup (points per block = 32 * waves) and
+// K-steps per LDS stage (one barrier per stage)
+struct Variant { int waves, steps; };
+constexpr Variant kVariants[] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 template <typename V> struct Elem;
 template <> struct Elem<h8> { typedef _Float16 T; };
@@ -114,29 +118,29 @@ __device__ inline void basis(const double* __restrict__ k, int h, const double* 
 }
 
 // Panels in HBM (16-byte units): [t][mtile][hilo][lane]; LDS double buffer of
-// MT x 2 x 64 units.  Wavevectors: separate stream [t][8][3] doubles read
-// with scalar loads, so the LDS serves only the coefficient fragments and the
-// only vector-memory traffic in the loop is the panel DMA.  The basis of step
-// t+1 is formed while the MFMAs of step t run.
-template <typename V, int D, int MT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
-                const u32x4* __restrict__ kq, float outScale) {
-    constexpr int kPanelUnits = MT * 2 * 64;
-    // two distinct objects: the waitcnt pass can tell the DMA target of the
-    // next step from the buffer being read
-    __shared__ u32x4 ldsA[kPanelUnits];
-    __shared__ u32x4 ldsB[kPanelUnits];
-
+// S K-steps x MT x 2 x 64 units.  Wavevectors: separate stream [t][8][3]
+// doubles read with scalar loads, so the LDS serves only the coefficient
+// fragments and the only vector-memory traffic in the loop is the panel DMA.
+// The basis of step t+1 is formed while the MFMAs of step t run.
+// One block's work: points pt0 .. pt0 + 32W - 1 (clipped to a.i1) and the
+// M-tiles mt0 .. mt0 + MT - 1.  ldsA and ldsB are two distinct __shared__
+// objects of S * MT * 128 units or more (distinct, so the waitcnt pass can
+// tell the DMA target of the next stage from the buffer being read).
+template <typename V, int D, int MT, int W, int S>
+__device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSteps, int mtTotal,
+                                               const u32x4* __restrict__ panels,
+                                               const u32x4* __restrict__ kq, float outScale,
+                                               int64_t pt0, int mt0, u32x4* ldsA, u32x4* ldsB) {
+    constexpr int kStepUnits = MT * 2 * 64;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const int h = lane >> 5;
     const int n = lane & 31;
-    const int mt0 = blockIdx.y * kMaxMT;            // first M-tile of this block
-    const int64_t stepUnits = (int64_t)mtTotal * 128;
-    const int64_t pt = a.i0 + (int64_t)blockIdx.x * kPointsPerBlock + wave * 32 + n;
+    const int64_t hbmStepUnits = (int64_t)mtTotal * 128;
+    const int64_t pt = pt0 + wave * 32 + n;
     const int64_t ptc = pt < a.i1 ? pt : a.i1 - 1;
+    const int64_t nStages = nSteps / S;             // nSteps is a multiple of S
 
     double xn[D];
 #pragma unroll
@@ -148,64 +152,73 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
 
-    // direct global->LDS copy of step t's panels into buffer buf: 2*MT
-    // chunks of 64 units spread over the waves
-    // (static trip count, so waits on older loads need not drain the DMA)
-    constexpr int kDmaPerWave = (2 * MT + kWaves - 1) / kWaves;
-    auto dma_step = [&](int64_t t, u32x4* dst) {
-        const u32x4* src = panels + t * stepUnits + (int64_t)mt0 * 128;
+    // direct global->LDS copy of stage u (S steps x 2*MT chunks of 64 units)
+    // spread over the waves (static trip count, so waits on older loads need
+    // not drain the DMA)
+    constexpr int kChunks = 2 * MT * S;
+    constexpr int kDmaPerWave = (kChunks + W - 1) / W;
+    auto dma_stage = [&](int64_t u, u32x4* dst) {
 #pragma unroll
         for (int i = 0; i < kDmaPerWave; i++) {
-            const int c = wave + i * kWaves;
-            if (2 * MT >= kWaves * (i + 1) || c < 2 * MT)
-                __builtin_amdgcn_global_load_lds((const void*)(src + c * 64 + lane),
-                                                 (lds_void*)&dst[c * 64], 16, 0, 0);
+            const int c = wave + i * W;
+            if (kChunks >= W * (i + 1) || c < kChunks) {
+                const int sI = c / (2 * MT), cI = c % (2 * MT);
+                const u32x4* src = panels + (u * S + sI) * hbmStepUnits + (int64_t)mt0 * 128;
+                __builtin_amdgcn_global_load_lds((const void*)(src + cI * 64 + lane),
+                                                 (lds_void*)&dst[sI * kStepUnits + cI * 64], 16, 0, 0);
+            }
         }
     };
 
     V bh, bl;
     const double* kqd = reinterpret_cast<const double*>(kq);
-    // one K-step: prefetch step t+1's panels into nxt, form the basis of
-    // step t+1 while the MFMAs of step t read cur
-    auto step = [&](int64_t t, const u32x4* cur, u32x4* nxt) {
-        // unconditional (the last step re-reads itself into the unused
+    // one stage: prefetch stage u+1 into nxt; per step, form the basis of the
+    // next step while the MFMAs of this step read cur
+    auto stage = [&](int64_t u, const u32x4* cur, u32x4* nxt) {
+        // unconditional (the last stage re-reads itself into the unused
         // buffer), so every path leaves the same number of loads in flight
-        dma_step(t + 1 < nSteps ? t + 1 : t, nxt);
-        V nbh, nbl;
-        basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
+        dma_stage(u + 1 < nStages ? u + 1 : u, nxt);
 #pragma unroll
-        for (int m = 0; m < MT; m += 2) {
-            const V ah0 = __builtin_bit_cast(V, cur[(m * 2 + 0) * 64 + lane]);
-            const V al0 = __builtin_bit_cast(V, cur[(m * 2 + 1) * 64 + lane]);
-            if (MT > 1) {
-                const V ah1 = __builtin_bit_cast(V, cur[(m * 2 + 2) * 64 + lane]);
-                const V al1 = __builtin_bit_cast(V, cur[(m * 2 + 3) * 64 + lane]);
-                acc[m] = mfma(ah0, bh, acc[m]);
-                acc[m + 1] = mfma(ah1, bh, acc[m + 1]);
-                acc[m] = mfma(ah0, bl, acc[m]);
-                acc[m + 1] = mfma(ah1, bl, acc[m + 1]);
-                acc[m] = mfma(al0, bh, acc[m]);
-                acc[m + 1] = mfma(al1, bh, acc[m + 1]);
-            } else {
-                acc[m] = mfma(ah0, bh, acc[m]);
-                acc[m] = mfma(ah0, bl, acc[m]);
-                acc[m] = mfma(al0, bh, acc[m]);
+        for (int sI = 0; sI < S; sI++) {
+            const int64_t t = u * S + sI;
+            const u32x4* c = cur + sI * kStepUnits;
+            V nbh, nbl;
+            basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
+#pragma unroll
+            for (int m = 0; m < MT; m += 2) {
+                const V ah0 = __builtin_bit_cast(V, c[(m * 2 + 0) * 64 + lane]);
+                const V al0 = __builtin_bit_cast(V, c[(m * 2 + 1) * 64 + lane]);
+                if (MT > 1) {
+                    const V ah1 = __builtin_bit_cast(V, c[(m * 2 + 2) * 64 + lane]);
+                    const V al1 = __builtin_bit_cast(V, c[(m * 2 + 3) * 64 + lane]);
+                    acc[m] = mfma(ah0, bh, acc[m]);
+                    acc[m + 1] = mfma(ah1, bh, acc[m + 1]);
+                    acc[m] = mfma(ah0, bl, acc[m]);
+                    acc[m + 1] = mfma(ah1, bl, acc[m + 1]);
+                    acc[m] = mfma(al0, bh, acc[m]);
+                    acc[m + 1] = mfma(al1, bh, acc[m + 1]);
+                } else {
+                    acc[m] = mfma(ah0, bh, acc[m]);
+                    acc[m] = mfma(ah0, bl, acc[m]);
+                    acc[m] = mfma(al0, bh, acc[m]);
+                }
             }
+            bh = nbh;
+            bl = nbl;
         }
-        bh = nbh;
-        bl = nbl;
         // this wave's global->LDS copies are complete (vmcnt(0); expcnt and
         // lgkmcnt untouched) before the barrier publishes them to the others
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        __syncthreads();   // step t+1 landed in LDS; step t's buffer is free
+        __syncthreads();   // stage u+1 landed in LDS; stage u's buffer is free
     };
 
-    dma_step(0, ldsA);
+    dma_stage(0, ldsA);
     basis<V, D>(kqd, h, xn, &bh, &bl);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
-    for (int64_t t = 0; t < nSteps; t += 2) {
-        step(t, ldsA, ldsB);
-        if (t + 1 < nSteps) step(t + 1, ldsB, ldsA);
+    for (int64_t u = 0; u < nStages; u += 2) {
+        stage(u, ldsA, ldsB);
+        if (u + 1 < nStages) stage(u + 1, ldsB, ldsA);
     }
     if (pt >= a.i1) return;
     // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
@@ -225,6 +238,54 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
     }
 }
 
+
+template <typename V, int D, int MT, int W, int S>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
+                const u32x4* __restrict__ kq, float outScale) {
+    __shared__ u32x4 ldsA[S * MT * 128];
+    __shared__ u32x4 ldsB[S * MT * 128];
+    sum_mfma_block<V, D, MT, W, S>(a, nSteps, mtTotal, panels, kq, outScale,
+                                   a.i0 + (int64_t)blockIdx.x * (32 * W), blockIdx.y * kMaxMT, ldsA, ldsB);
+}
+
+// Grouped form for a batch of independent fields (requests of a serving
+// batch): block b runs one tile = (field, point block, block of up to
+// kGroupMT M-tiles).  Each field's points, panels and output are its own;
+// the per-point operation order is that of k_sum_mfma, so a field's result
+// does not depend on what it is batched with.
+__device__ inline int group_find(const rf_group_req* __restrict__ reqs, int nReq, int64_t b) {
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename V, int W, int S>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
+    __shared__ u32x4 ldsA[S * kGroupMT * 128];
+    __shared__ u32x4 ldsB[S * kGroupMT * 128];
+    const int64_t b = blockIdx.x;
+    const rf_group_req& q = reqs[group_find(reqs, nReq, b)];
+    const int nSub = (q.mtTotal + kGroupMT - 1) / kGroupMT;
+    const int64_t local = b - q.tile0;
+    const int64_t pt0 = (local / nSub) * (32 * W);
+    const int mt0 = (int)(local % nSub) * kGroupMT;
+    const int mt = q.mtTotal - mt0 < kGroupMT ? q.mtTotal - mt0 : kGroupMT;
+    const rf_sum_args a = q.a;
+    const u32x4* P = static_cast<const u32x4*>(q.ops0);
+    const u32x4* K = static_cast<const u32x4*>(q.ops1);
+    switch (mt) {
+        case 1: sum_mfma_block<V, 3, 1, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        case 2: sum_mfma_block<V, 3, 2, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        default: sum_mfma_block<V, 3, kGroupMT, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+    }
+}
+
 // Coefficient panels from FP64 phases and amplitudes, one thread per (step,
 // mtile, lane, element) writing both halves; the first 24 threads of each
 // step also write its 8 wavevectors (revolutions, 3 components, zero padded)
@@ -232,8 +293,8 @@ void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restr
 template <typename T>
 __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int mtTotal,
                             const double* __restrict__ kRev, const double* __restrict__ phase,
-                            const double* __restrict__ amp, double scale, T* __restrict__ panels,
-                            double* __restrict__ kqd) {
+                            const double* __restrict__ amp, int64_t ampStrideR, double scale,
+                            T* __restrict__ panels, double* __restrict__ kqd) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t perStep = (int64_t)mtTotal * 64 * 8;
     if (e >= nSteps * perStep) return;
@@ -253,7 +314,7 @@ __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int m
     const int64_t j = t * kStepK + 4 * h + q;
     double v = 0.0;
     if (r < Nmc && j < kN) {
-        const double A = amp[(int64_t)r * kN + j] * scale;
+        const double A = amp[(int64_t)r * ampStrideR + j] * scale;
         const double ph = phase[(int64_t)r * kN + j];
         v = (el & 1) ? -A * sin(ph) : A * cos(ph);
     }
@@ -264,39 +325,51 @@ __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int m
     pan[64 * 8 + (int64_t)lane * 8 + el] = lo;
 }
 
-template <typename V, int D>
+template <typename V, int D, int W, int S>
 hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, const void* B,
-                     float outScale, dim3 grid, hipStream_t s) {
+                     float outScale, hipStream_t s) {
     const u32x4* P = static_cast<const u32x4*>(B);
     const u32x4* K = P + nSteps * mtTotal * 128;
+    const int64_t n = a.i1 - a.i0;
+    const dim3 grid((unsigned)((n + 32 * W - 1) / (32 * W)), (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
+    const dim3 block(64 * W);
     switch (mt) {
-        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8>), grid, dim3(kThreads), 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
     }
     return hipGetLastError();
 }
 
+template <typename V, int D>
+hipError_t launch_variant(int variant, int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal,
+                          const void* B, float outScale, hipStream_t s) {
+    switch (variant) {
+        case 1: return launch_mt<V, D, 4, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 2: return launch_mt<V, D, 8, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 3: return launch_mt<V, D, 8, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
+        default: return launch_mt<V, D, 4, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+    }
+}
+
 template <typename V>
-hipError_t launch_form(const rf_sum_args& a, int64_t kNpad, int Npad, const void* B,
+hipError_t launch_form(int variant, const rf_sum_args& a, int64_t kNpad, int Npad, const void* B,
                        double outScale, hipStream_t s) {
     const int mtTotal = Npad / 32;
     const int mt = mtTotal >= kMaxMT ? kMaxMT : mtTotal;
-    const int64_t n = a.i1 - a.i0;
-    dim3 grid((unsigned)((n + kPointsPerBlock - 1) / kPointsPerBlock),
-              (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
     const int64_t nSteps = kNpad / kStepK;
     switch (a.nDim) {
-        case 1: return launch_mt<V, 1>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
-        case 2: return launch_mt<V, 2>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
-        default: return launch_mt<V, 3>(mt, a, nSteps, mtTotal, B, (float)outScale, grid, s);
+        case 1: return launch_variant<V, 1>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
+        case 2: return launch_variant<V, 2>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
+        default: return launch_variant<V, 3>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
     }
 }
 
 }  // namespace
 
-int rf_mfma_kpad(int) { return kStepK; }
+int rf_mfma_kpad(int) { return kKPad; }
+int rf_mfma_num_variants() { return kNumVariants; }
 
 // Realizations are padded to whole 32-row tiles; above 256 to whole blocks of
 // 8 tiles, and below that to a power-of-two tile count (1, 2, 4, 8).
@@ -316,7 +389,7 @@ size_t rf_mfma_bbytes(int, int64_t kNpad, int Npad) {
 
 hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, int Nmc, int Npad,
                                const double* d_kRev, const double* d_phase, const double* d_amp,
-                               double scale, void* d_B, hipStream_t s) {
+                               int64_t ampStrideR, double scale, void* d_B, hipStream_t s) {
     const int64_t nSteps = kNpad / kStepK;
     const int mtTotal = Npad / 32;
     const int64_t total = nSteps * mtTotal * 64 * 8;
@@ -325,16 +398,28 @@ hipError_t rf_launch_mfma_prep(int form, int nDim, int64_t kN, int64_t kNpad, in
     double* kqd = reinterpret_cast<double*>(base + (size_t)nSteps * mtTotal * 128 * 16);
     if (form == RF_FORM_MFMA_BF16)
         hipLaunchKernelGGL(k_mfma_prep<__bf16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
-                           d_kRev, d_phase, d_amp, scale, reinterpret_cast<__bf16*>(base), kqd);
+                           d_kRev, d_phase, d_amp, ampStrideR, scale, reinterpret_cast<__bf16*>(base), kqd);
     else
         hipLaunchKernelGGL(k_mfma_prep<_Float16>, dim3(grid), dim3(256), 0, s, nDim, kN, nSteps, Nmc, mtTotal,
-                           d_kRev, d_phase, d_amp, scale, reinterpret_cast<_Float16*>(base), kqd);
+                           d_kRev, d_phase, d_amp, ampStrideR, scale, reinterpret_cast<_Float16*>(base), kqd);
     return hipGetLastError();
 }
 
-hipError_t rf_launch_sum_mfma(int form, const rf_sum_args& a, int64_t kNpad, int Npad,
+hipError_t rf_launch_group_mfma(int form, const rf_group_req* d_reqs, int nReq, int64_t nTiles,
+                                hipStream_t s) {
+    if (nTiles <= 0 || nReq <= 0) return hipSuccess;
+    if (nTiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (form == RF_FORM_MFMA_BF16)
+        hipLaunchKernelGGL((k_group_mfma<b8, kGroupW, 2>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
+    else
+        hipLaunchKernelGGL((k_group_mfma<h8, kGroupW, 2>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_sum_mfma(int form, int variant, const rf_sum_args& a, int64_t kNpad, int Npad,
                               const void* d_B, double outScale, hipStream_t s) {
     if (a.i1 <= a.i0) return hipSuccess;
-    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(a, kNpad, Npad, d_B, outScale, s);
-    return launch_form<h8>(a, kNpad, Npad, d_B, outScale, s);
+    if (kNpad % kKPad != 0 || variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
+    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(variant, a, kNpad, Npad, d_B, outScale, s);
+    return launch_form<h8>(variant, a, kNpad, Npad, d_B, outScale, s);
 }

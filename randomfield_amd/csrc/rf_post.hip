@@ -14,12 +14,46 @@ namespace {
 constexpr int kBlock = 256;
 
 template <typename T>
-__global__ void k_marginal(int margi, double scale, double shift, T* __restrict__ d, int64_t n) {
+__global__ __launch_bounds__(kBlock) void k_marginal(int margi, double scale, double shift,
+                                                     T* __restrict__ d, int64_t n) {
 #pragma clang fp contract(off)
+    constexpr int U = 4;                  // independent loads in flight per thread
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += stride) {
-        const double y = (double)d[e] * scale + shift;                 // :171
-        d[e] = (T)((margi == RF_FOM_LOGNORMAL) ? exp(y) : y);         // :175
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += U * stride) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = (e + u * stride < n) ? d[e + u * stride] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (e + u * stride < n) {
+                const double y = (double)v[u] * scale + shift;                  // :171
+                d[e + u * stride] = (T)((margi == RF_FOM_LOGNORMAL) ? exp(y) : y);  // :175
+            }
+    }
+}
+
+// The same map over 16-byte vectors (4 floats or 2 doubles per load): nVec
+// vectors from d, which must be 16-byte aligned; the tail goes to k_marginal.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_marginal_vec(int margi, double scale, double shift,
+                                                         T* __restrict__ d, int64_t nVec) {
+#pragma clang fp contract(off)
+    constexpr int L = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(L)));
+    vec* v = reinterpret_cast<vec*>(d);
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nVec; e += 2 * stride) {
+        vec a = v[e];
+        vec b = (e + stride < nVec) ? v[e + stride] : a;
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            const double ya = (double)a[l] * scale + shift;
+            const double yb = (double)b[l] * scale + shift;
+            a[l] = (T)((margi == RF_FOM_LOGNORMAL) ? exp(ya) : ya);
+            b[l] = (T)((margi == RF_FOM_LOGNORMAL) ? exp(yb) : yb);
+        }
+        v[e] = a;
+        if (e + stride < nVec) v[e + stride] = b;
     }
 }
 
@@ -80,67 +114,83 @@ __global__ void k_affine(T* __restrict__ d, int64_t N, const double* __restrict_
 }
 
 // ----------------------------------------------------------------- blend
+// One thread per group point and chunk of kBlendR realizations (grid y), so
+// the blocks in flight sweep the points of a few realizations at a time (few
+// pages per wave) and the loads of a chunk are independent.  Sums run in FP64
+// in ascending subdomain order (the reference's loop over fields,
+// calls_RF.f90:237), each term v * PU then + (localization_RF.f90:228,253).
+constexpr int kBlendR = 4;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict__ group) {
 #pragma clang fp contract(off)
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= b.groupTot) return;
-    int gi[3] = {0, 0, 0};
+    int gi[3];
     {
         int64_t rem = g;
-        for (int a = 0; a < b.nDim; a++) { gi[a] = (int)(rem % b.groupN[a]); rem /= b.groupN[a]; }
+        gi[0] = (int)(rem % b.groupN[0]);
+        rem /= b.groupN[0];
+        gi[1] = (int)(rem % b.groupN[1]);
+        gi[2] = (int)(rem / b.groupN[1]);
     }
-    int cv[3][2], nc[3] = {1, 1, 1};
-    cv[1][0] = cv[2][0] = 0;
-    for (int a = 0; a < b.nDim; a++) {
-        nc[a] = 0;
-        for (int c = 0; c < 2; c++) {
-            const int v = b.cover[a][2 * gi[a] + c];
-            if (v >= 0) cv[a][nc[a]++] = v;
+    int cell = b.seg[0][gi[0]];
+    if (b.nDim > 1) cell += b.nSeg[0] * b.seg[1][gi[1]];
+    if (b.nDim > 2) cell += b.nSeg[0] * b.nSeg[1] * b.seg[2][gi[2]];
+    const int e0 = b.cellOff[cell];
+    const int nh = b.cellOff[cell + 1] - e0;
+    const T* F[kBlendMaxEntries];
+    int64_t rs[kBlendMaxEntries], loc[kBlendMaxEntries];
+    double w[kBlendMaxEntries];
+#pragma unroll
+    for (int h = 0; h < kBlendMaxEntries; h++) {
+        F[h] = nullptr;
+        rs[h] = 0;
+        loc[h] = 0;
+        w[h] = 0.0;
+        if (h < nh) {
+            const rf_blend_entry& E = b.entries[e0 + h];
+            F[h] = static_cast<const T*>(E.field);
+            rs[h] = E.subTot;
+            loc[h] = E.base + gi[0] + (int64_t)gi[1] * E.stride1 + (int64_t)gi[2] * E.stride2;
+            // PU product in the reference's order: lower faces of axes 1..d,
+            // then upper faces (localization_RF.f90:755-851)
+            const double* lf = b.lowF + (int64_t)E.s * 3 * b.maxSize;
+            const double* uf = b.upF + (int64_t)E.s * 3 * b.maxSize;
+            double pu = 1.0;
+            for (int a = 0; a < b.nDim; a++) pu = pu * lf[a * b.maxSize + (gi[a] - E.o[a])];
+            for (int a = 0; a < b.nDim; a++) pu = pu * uf[a * b.maxSize + (gi[a] - E.o[a])];
+            w[h] = pu;
         }
     }
-    // <= 8 covering subdomains, kept in ascending subdomain index so the sum
-    // runs in the reference's order (loop over fields, calls_RF.f90:237)
-    int sub[8];
-    int64_t loc[8];
-    double w[8];
-    int nh = 0;
-    for (int a2 = 0; a2 < nc[2]; a2++)
-        for (int a1 = 0; a1 < nc[1]; a1++)
-            for (int a0 = 0; a0 < nc[0]; a0++) {
-                const int iv[3] = {cv[0][a0], cv[1][a1], cv[2][a2]};
-                const int s = b.subOf[iv[0] + b.nInt[0] * (iv[1] + b.nInt[1] * iv[2])];
-                if (s < 0) continue;
-                int64_t l = 0, stride = 1;
-                int pos[3] = {0, 0, 0};
-                for (int a = 0; a < b.nDim; a++) {
-                    pos[a] = gi[a] - b.intOrigin[a][iv[a]];
-                    l += (int64_t)pos[a] * stride;
-                    stride *= b.intSize[a][iv[a]];
-                }
-                // PU product in the reference's order: lower faces of axes
-                // 1..d, then upper faces (localization_RF.f90:755-851)
-                double pu = 1.0;
-                for (int a = 0; a < b.nDim; a++) pu = pu * b.lowF[((int64_t)s * 3 + a) * b.maxSize + pos[a]];
-                for (int a = 0; a < b.nDim; a++) pu = pu * b.upF[((int64_t)s * 3 + a) * b.maxSize + pos[a]];
-                int q = nh++;
-                while (q > 0 && sub[q - 1] > s) { sub[q] = sub[q - 1]; loc[q] = loc[q - 1]; w[q] = w[q - 1]; q--; }
-                sub[q] = s; loc[q] = l; w[q] = pu;
-            }
     double pu1 = 1.0;
     if (b.mode == RF_REF_COMPAT) {
         for (int a = 0; a < b.nDim; a++) pu1 = pu1 * b.pu1Low[a][gi[a]];
         for (int a = 0; a < b.nDim; a++) pu1 = pu1 * b.pu1Up[a][gi[a]];
     }
-    for (int r = 0; r < b.Nmc; r++) {
-        double acc = 0.0;
-        for (int h = 0; h < nh; h++) {
-            const T* F = static_cast<const T*>(b.fields[sub[h]]);
-            const double v = (double)F[(int64_t)r * b.subTot[sub[h]] + loc[h]];
-            acc = v * w[h] + acc;                        // :228 then :253
+    {
+        const int r0 = blockIdx.y * kBlendR;
+        double acc[kBlendR];
+#pragma unroll
+        for (int j = 0; j < kBlendR; j++) acc[j] = 0.0;
+#pragma unroll
+        for (int h = 0; h < kBlendMaxEntries; h++) {
+            if (h < nh) {
+#pragma unroll
+                for (int j = 0; j < kBlendR; j++)
+                    if (r0 + j < b.Nmc) {
+                        const double v = (double)F[h][(int64_t)(r0 + j) * rs[h] + loc[h]];
+                        acc[j] = v * w[h] + acc[j];
+                    }
+            }
         }
-        if (b.mode == RF_REF_COMPAT && r == 0) acc = acc / pu1;   // calls_RF.f90:355
-        group[(int64_t)r * b.groupTot + g] = (T)acc;
+#pragma unroll
+        for (int j = 0; j < kBlendR; j++)
+            if (r0 + j < b.Nmc) {
+                double u = acc[j];
+                if (b.mode == RF_REF_COMPAT && r0 + j == 0) u = u / pu1;   // calls_RF.f90:355
+                group[(int64_t)(r0 + j) * b.groupTot + g] = (T)u;
+            }
     }
 }
 
@@ -149,14 +199,34 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
 hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, int64_t count,
                               int dtype, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    int64_t blocks = (count + kBlock - 1) / kBlock;
-    if (blocks > 65536) blocks = 65536;
-    if (dtype == RF_DTYPE_F64)
-        hipLaunchKernelGGL(k_marginal<double>, dim3((unsigned)blocks), dim3(kBlock), 0, s, margi,
-                           scale, shift, static_cast<double*>(d), count);
-    else
-        hipLaunchKernelGGL(k_marginal<float>, dim3((unsigned)blocks), dim3(kBlock), 0, s, margi,
-                           scale, shift, static_cast<float*>(d), count);
+    const size_t esz = dtype == RF_DTYPE_F64 ? 8 : 4;
+    int64_t head = 0;                        // elements done by the vector kernel
+    if (((uintptr_t)d & 15) == 0) {
+        const int64_t L = (int64_t)(16 / esz);
+        const int64_t nVec = count / L;
+        if (nVec > 0) {
+            int64_t blocks = (nVec + 2 * kBlock - 1) / (2 * kBlock);
+            if (blocks > 65536) blocks = 65536;
+            if (dtype == RF_DTYPE_F64)
+                hipLaunchKernelGGL(k_marginal_vec<double>, dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                                   margi, scale, shift, static_cast<double*>(d), nVec);
+            else
+                hipLaunchKernelGGL(k_marginal_vec<float>, dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                                   margi, scale, shift, static_cast<float*>(d), nVec);
+            head = nVec * L;
+        }
+    }
+    const int64_t rest = count - head;
+    if (rest > 0) {
+        int64_t blocks = (rest + 4 * kBlock - 1) / (4 * kBlock);
+        if (blocks > 65536) blocks = 65536;
+        if (dtype == RF_DTYPE_F64)
+            hipLaunchKernelGGL(k_marginal<double>, dim3((unsigned)blocks), dim3(kBlock), 0, s, margi,
+                               scale, shift, static_cast<double*>(d) + head, rest);
+        else
+            hipLaunchKernelGGL(k_marginal<float>, dim3((unsigned)blocks), dim3(kBlock), 0, s, margi,
+                               scale, shift, static_cast<float*>(d) + head, rest);
+    }
     return hipGetLastError();
 }
 
@@ -196,12 +266,11 @@ hipError_t rf_launch_affine(void* d, int64_t N, int Nmc, int dtype, const double
 }
 
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s) {
-    const unsigned grid = (unsigned)((b.groupTot + kBlock - 1) / kBlock);
+    const dim3 grid((unsigned)((b.groupTot + kBlock - 1) / kBlock), (unsigned)((b.Nmc + kBlendR - 1) / kBlendR));
+    if (grid.y > 65535) return hipErrorInvalidValue;
     if (dtype == RF_DTYPE_F64)
-        hipLaunchKernelGGL(k_blend<double>, dim3(grid), dim3(kBlock), 0, s, b,
-                           static_cast<double*>(d_group));
+        hipLaunchKernelGGL(k_blend<double>, grid, dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
     else
-        hipLaunchKernelGGL(k_blend<float>, dim3(grid), dim3(kBlock), 0, s, b,
-                           static_cast<float*>(d_group));
+        hipLaunchKernelGGL(k_blend<float>, grid, dim3(kBlock), 0, s, b, static_cast<float*>(d_group));
     return hipGetLastError();
 }

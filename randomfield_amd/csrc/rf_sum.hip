@@ -105,24 +105,24 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double
 // argument cannot be shared across realizations, so each term forms its own
 // FP64 k.x.  kT[j][r][D] in revolutions, aT/pT[j][r] as in k_sum_fp32.
 template <int D, int R>
-__global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const double* __restrict__ kT,
-                                                         const float* __restrict__ pT,
-                                                         const float* __restrict__ aT, int Rpad) {
-    const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int r0 = blockIdx.y * R;
-    if (i >= a.i1) return;
+__device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, const double* __restrict__ kT,
+                                                   const float* __restrict__ pT,
+                                                   const float* __restrict__ aT, int Rpad, int64_t i,
+                                                   int r0) {
     double xn[D];
 #pragma unroll
     for (int d = 0; d < D; d++) xn[d] = a.x[i * D + d] / a.corrL[d];
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
+    const int nr = a.Nmc - r0;               // realizations of this chunk (wave-uniform)
     for (int64_t j = 0; j < a.kN; j++) {
         const double* kj = kT + (j * Rpad + r0) * D;
         const float* pj = pT + j * Rpad + r0;
         const float* aj = aT + j * Rpad + r0;
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            if (r >= nr) break;                  // padded realizations stay zero
             double t = kj[r * D] * xn[0];
 #pragma unroll
             for (int d = 1; d < D; d++) t = fma(kj[r * D + d], xn[d], t);
@@ -135,6 +135,36 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const do
         if (r0 + r < a.Nmc)
             store_out(a.out, (int64_t)(r0 + r) * a.N + i, epilogue32(acc[r], a.epi), a.outDtype);
     }
+}
+
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const double* __restrict__ kT,
+                                                         const float* __restrict__ pT,
+                                                         const float* __restrict__ aT, int Rpad) {
+    const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.i1) return;
+    sum_fp32_iso_point<D, R>(a, kT, pT, aT, Rpad, i, blockIdx.y * R);
+}
+
+// Grouped form: block b runs one tile = (field, block of kGroupIsoPts points,
+// chunk of kGroupIsoR realizations), found by binary search over the fields'
+// first tiles; same per-point operation order as k_sum_fp32_iso.
+__global__ __launch_bounds__(kBlock) void k_group_iso(const rf_group_req* __restrict__ reqs, int nReq) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const rf_group_req& q = reqs[lo];
+    const int nChunk = q.Rpad / kGroupIsoR;
+    const int64_t local = b - q.tile0;
+    const int64_t i = (local / nChunk) * kGroupIsoPts + threadIdx.x;
+    if (i >= q.a.N) return;
+    sum_fp32_iso_point<3, kGroupIsoR>(q.a, static_cast<const double*>(q.ops0), static_cast<const float*>(q.ops1),
+                                      static_cast<const float*>(q.ops2), q.Rpad, i,
+                                      (int)(local % nChunk) * kGroupIsoR);
 }
 
 template <int D>
@@ -193,3 +223,11 @@ hipError_t rf_launch_sum_fp32_iso_t(const rf_sum_args& a, const double* d_kT, co
 }
 
 int rf_fp32_rchunk() { return kRChunk; }
+
+hipError_t rf_launch_group_iso(const rf_group_req* d_reqs, int nReq, int64_t nTiles, hipStream_t s) {
+    static_assert(kGroupIsoPts == kBlock, "one thread per point");
+    if (nTiles <= 0 || nReq <= 0) return hipSuccess;
+    if (nTiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_group_iso, dim3((unsigned)nTiles), dim3(kBlock), 0, s, d_reqs, nReq);
+    return hipGetLastError();
+}

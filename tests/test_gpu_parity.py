@@ -112,6 +112,40 @@ def test_cfg3_small_forms(ctx, cfg3_small, form):
     assert report(f"cfg3-small {NAMES[form]}", got.astype(np.float64), ref) <= TOL[form]
 
 
+@pytest.mark.parametrize("Nmc", [64, 3, 300])
+def test_mfma_variants_identical(ctx, cfg3_small, Nmc):
+    """Every MFMA launch shape gives bit-identical fields (same per-point
+    operation order); the autotuner keeps one of them and leaves its result."""
+    import torch
+    x, p0, kU, phU, ref = cfg3_small
+    p = R.make_params(nDim=3, method=R.RANDOMIZATION, mode=R.FIXED, kSampling=R.KSAMP_DENSITY,
+                      nKDensity=4096, Nmc=Nmc, corrL=(2.0, 2.0, 0.5), xRange=(1, 1, 1),
+                      form=R.FORM_MFMA_F16)
+    if Nmc != 64:
+        kU, phU = R.reference_draws(p, R.reference_seed(11))
+    plan = R.FieldPlan(ctx, p, kU, phU)
+    dx = torch.from_numpy(x).cuda()
+    outs = []
+    for v in range(4):
+        plan.set_variant(v)
+        o = torch.empty((Nmc, x.shape[0]), dtype=torch.float32, device="cuda")
+        plan.execute(dx, o)
+        ctx.sync()
+        outs.append(o.cpu().numpy())
+    for v in range(1, 4):
+        assert np.array_equal(outs[v], outs[0]), v
+    o = torch.empty((Nmc, x.shape[0]), dtype=torch.float32, device="cuda")
+    chosen = plan.autotune(dx, o)
+    ctx.sync()
+    assert 0 <= chosen < 4
+    assert np.array_equal(o.cpu().numpy(), outs[0])
+    with pytest.raises(R.RFError):
+        plan.set_variant(4)
+    plan.close()
+    if Nmc == 64:
+        assert report("mfma variants", outs[0].astype(np.float64), ref) <= TOL[R.FORM_MFMA_F16]
+
+
 def test_randomization_box_ref_compat(ctx):
     x = W.cfg1_points(1500, 9)[:, :2] * 0.8
     p = R.make_params(nDim=2, method=R.RANDOMIZATION, mode=R.REF_COMPAT, Nmc=3, corrL=(1.0, 2.0),
@@ -182,6 +216,28 @@ def test_lognormal_fused(ctx, iso_case):
     with pytest.raises(R.RFError) as e:
         R.multivariate_transformation(ctx, R.FOM_LOGNORMAL, -1.0, 0.25, h)
     assert e.value.code == R.RF_E_LOGNORMAL
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("offset,count", [(0, 100003), (1, 100003), (3, 7), (0, 4)])
+def test_marginal_device_vector_and_tail(ctx, dtype, offset, count):
+    """Aligned vector body + scalar tail, and a misaligned start (offset in
+    elements), against the oracle's map of the same values."""
+    import torch
+    tt = torch.float32 if dtype == np.float32 else torch.float64
+    g = np.random.default_rng(count + offset).standard_normal(offset + count).astype(dtype)
+    d = torch.from_numpy(g.copy()).cuda()
+    view = d[offset:]
+    for margi, avg, var in ((R.FOM_LOGNORMAL, 1.0, 0.25), (R.FOM_GAUSSIAN, 2.0, 9.0)):
+        view.copy_(torch.from_numpy(g[offset:]).to(tt))
+        R.check(R.lib().rf_marginal_device(ctx.handle, margi, avg, var, R._ptr(view), count,
+                                           R.DTYPE_F32 if dtype == np.float32 else R.DTYPE_F64, None))
+        ctx.sync()
+        ref = O.marginal(margi, avg, var, g[offset:].astype(np.float64)[None, :])[0].astype(dtype)
+        got = d.cpu().numpy()
+        assert np.array_equal(got[:offset], g[:offset])          # untouched before the view
+        rt = 1e-6 if dtype == np.float32 else 1e-14
+        assert np.allclose(got[offset:], ref, rtol=rt, atol=0), (margi, offset, count)
 
 
 # ------------------------------------------------------------ device helpers

@@ -178,8 +178,9 @@ def run_cfg2(args):
 def run_cfg4(args):
     """cfg 4: 512x512x256 grid (spacing 0.25), 4x4x2 uneven subdomains with
     7-point overlaps, Shinozuka per subdomain on its own lattice (seed 100 +
-    field index), Nmc 16, FP32; grid-form generation then the
-    variance-preserving blend.  A step is all 32 generations + the blend."""
+    field index), Nmc 16, FP32.  A step is the generation of all 32
+    subdomains (three grouped launches) and the variance-preserving blend,
+    replayed as one captured graph."""
     import numpy as np
     import torch
 
@@ -189,7 +190,8 @@ def run_cfg4(args):
     groupN, parts, nOvl, step, Nmc = (512, 512, 256), (4, 4, 2), 7, (0.25, 0.25, 0.25), 16
     origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
     ctx = R.Context(0)
-    sptr = torch.cuda.current_stream().cuda_stream
+    stream = torch.cuda.Stream()
+    sptr = stream.cuda_stream
     plans, fields, geo = [], [], []
     for s in range(len(sizes)):
         n = [int(v) for v in sizes[s]]
@@ -201,17 +203,30 @@ def run_cfg4(args):
         fields.append(torch.empty((Nmc, int(np.prod(n))), dtype=torch.float32, device="cuda"))
         geo.append((o, n, ext))
     g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float32, device="cuda")
+    batch = R.GridBatch(ctx, plans, [x[0] for x in geo], [step] * len(geo), [x[1] for x in geo], fields,
+                        R.DTYPE_F32)
+    bplan = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields, R.DTYPE_F32)
+    graph = R.Graph(ctx, sptr)
+    with graph:
+        batch.execute(sptr)
+        bplan.execute(g, sptr)
 
-    def generate():
-        for s, pl in enumerate(plans):
-            pl.execute_grid(geo[s][0], step, geo[s][1], fields[s], stream=sptr)
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, np.array([a.elapsed_time(b) for a, b in ev])
 
-    def blend():
-        R.blend(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields, g, R.DTYPE_F32,
-                stream=sptr)
-
-    wall_g, ms_g = _timed(generate, args.steps, args.warmup)
-    wall_b, ms_b = _timed(blend, args.steps, args.warmup)
+    wall_s, ms_s = timed(lambda: graph.launch(sptr), args.steps, args.warmup)
+    _, ms_g = timed(lambda: batch.execute(sptr), args.steps, 1)
+    _, ms_b = timed(lambda: bplan.execute(g, sptr), args.steps, 1)
     nglob = float(np.prod(groupN))
     nsub = float(sum(np.prod(s) for s in sizes))
     work = nglob * Nmc
@@ -229,21 +244,22 @@ def run_cfg4(args):
     sk = O.skvec(po, k)
     xs = W.grid_points(n, step, o)
     sel = np.random.default_rng(0).choice(xs.shape[0], 24, replace=False)
-    generate()
-    torch.cuda.synchronize()
     t = time.perf_counter()
     ref = O.shinozuka(po, xs[sel], k, sk, phU, threads=_threads())
     cpu_s = time.perf_counter() - t
     got = fields[0][:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
+    graph.close()
+    bplan.close()
+    batch.close()
     for pl in plans:
         pl.close()
     ctx.close()
-    step_ms = wall_g + wall_b
-    return _line("points x realizations / s (cfg4 localization: generation + blend)", work / (step_ms * 1e-3),
-                 args.steps, args.warmup, step_ms, "f32",
+    return _line("points x realizations / s (cfg4 localization: generation + blend)", work / (wall_s * 1e-3),
+                 args.steps, args.warmup, wall_s, "f32",
                  "cfg4: 512x512x256 grid, 4x4x2 uneven subdomains, 7-point overlap, Nmc 16, "
-                 f"N_k {info.kNTotal} (subdomain 0), grid form + variance-preserving blend",
-                 generation_ms=wall_g, blend_ms=wall_b,
+                 f"N_k {info.kNTotal} (subdomain 0), grid form in grouped launches + variance-preserving blend, "
+                 "one captured graph per step",
+                 step_kernel_ms=float(ms_s.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
                  blend_roofline={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
                                  "frac": gbs / 8000.0, "kernel_ms": float(ms_b.mean()),
                                  "bytes_per_launch": bytes_blend},

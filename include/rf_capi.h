@@ -209,11 +209,64 @@ int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN,
                     const void* const* d_fields, void* d_group,
                     int32_t dtype, void* stream);
 
+/* The blend split into set-up and launch (same arguments and result as
+ * rf_blend_device): create analyses the boxes and uploads the tables once
+ * (synchronous); execute enqueues one kernel and allocates nothing, so it
+ * can be captured into a graph; destroy frees the tables (no launch of the
+ * plan may be pending). */
+typedef struct rf_blend_plan rf_blend_plan;
+int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t* groupN,
+                         int32_t Nmc, int32_t nSub, const int32_t* origins,
+                         const int32_t* sizes, const int32_t* nOvl,
+                         const int32_t* faceMasks, int32_t mode,
+                         const void* const* d_fields, int32_t dtype,
+                         rf_blend_plan** plan);
+int rf_blend_plan_execute(rf_blend_plan* plan, void* d_group, void* stream);
+int rf_blend_plan_destroy(rf_blend_plan* plan);
+
+/* Batched structured-grid generation: the subdomains of a localization
+ * group (one MPI rank each in the reference, localization_RF.f90:127-192,
+ * calls_RF.f90:250-296) as one batch.  Field i is rf_plan_execute_grid of
+ * plans[i] on origins/steps/nsteps[3i..3i+2] into d_outs[i]; create builds
+ * every field's phase tables and coefficients and the GEMM descriptors
+ * (allocates, synchronous); execute enqueues three grouped launches (all
+ * fields' first, middle and last contractions) and allocates nothing, so it
+ * can be captured into a graph.  Results equal rf_plan_execute_grid's. */
+typedef struct rf_grid_batch rf_grid_batch;
+int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans,
+                         const double* origins, const double* steps,
+                         const int32_t* nsteps, void* const* d_outs,
+                         int32_t outDtype, rf_grid_batch** batch);
+int rf_grid_batch_execute(rf_grid_batch* batch, void* stream);
+int rf_grid_batch_destroy(rf_grid_batch* batch);
+
+/* Graph capture of the launches a thread enqueues on an explicit stream
+ * between begin and end (only allocation-free entry points: the execute
+ * functions of a batch or blend plan, rf_plan_execute of a FIXED-mode plan);
+ * launch replays them on a stream.  No counterpart in the reference. */
+typedef struct rf_graph rf_graph;
+int rf_graph_capture_begin(rf_context* ctx, void* stream);
+int rf_graph_capture_end(rf_context* ctx, void* stream, rf_graph** graph);
+int rf_graph_launch(rf_graph* graph, void* stream);
+int rf_graph_destroy(rf_graph* graph);
+
 /* Per-realization FP64 mean/std (biased) and global mean/std; host outputs
  * avg[Nmc], std[Nmc], g[2] (synchronous). */
 int rf_stats_device(rf_context* ctx, const void* d_field, int64_t N,
                     int32_t Nmc, int32_t dtype, double* avg, double* std,
                     double* g);
+/* rebuild_Sk_FFT + rebuild_Sk + rebuild_corrL (statistics_RF.f90:180-443) of
+ * one realization of a structured-grid field (x fastest; d_field holds
+ * realizations of prod(nstep) points back to back).  For each axis a, R
+ * receives lags 0 .. nstep[a]/2-1 of the line-averaged circular
+ * autocorrelation of (u - avg)/std (what the reference's FFT, |.|^2, line
+ * average and inverse FFT compute), axes back to back, and corrL[a] (may be
+ * NULL) the reference's estimate 2 (sum R - (R[0] + R[last])/2) xStep[a].
+ * FP64 sums; host outputs; synchronous. */
+int rf_corr_device(rf_context* ctx, const void* d_field, int32_t dtype,
+                   int32_t nDim, const int32_t* nstep, const double* xStep,
+                   int32_t realization, double avg, double std, double* R,
+                   double* corrL);
 /* Subtract each realization's mean and divide by its RMS (FP64 sums). */
 int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N,
                         int32_t Nmc, int32_t dtype, void* stream);

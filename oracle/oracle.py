@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
         L.rfo_blend.argtypes = [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp]
         L.rfo_normalize.argtypes = [vp, C.c_int64, C.c_int]
         L.rfo_stats.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp]
+        L.rfo_corr.argtypes = [vp, C.c_int, vp, C.c_double, C.c_double, vp]
+        L.rfo_corrL.argtypes = [vp, C.c_int, vp, vp, vp]
         L.rfo_iso_nvar.argtypes = [C.c_int]
         _L = L
     return _L
@@ -182,3 +184,22 @@ def stats(field: np.ndarray):
 
 def iso_nvar(nDim: int) -> int:
     return lib().rfo_iso_nvar(nDim)
+
+
+def corr(u: np.ndarray, nstep, avg: float, std: float) -> list:
+    """Directional circular autocorrelation of one realization on a grid
+    (x fastest): a list of nDim arrays of lags 0 .. n_a/2 - 1."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    n = np.ascontiguousarray(nstep, dtype=np.int32)
+    M = [int(v) // 2 for v in n]
+    R = np.zeros(sum(M))
+    lib().rfo_corr(_p(u), len(n), _p(n), avg, std, _p(R))
+    return list(np.split(R, np.cumsum(M)[:-1]))
+
+
+def corrL(R: list, nstep, xStep) -> np.ndarray:
+    n = np.ascontiguousarray(nstep, dtype=np.int32)
+    flat = np.ascontiguousarray(np.concatenate(R), dtype=np.float64)
+    out = np.zeros(len(n))
+    lib().rfo_corrL(_p(flat), len(n), _p(n), _p(np.asarray(xStep, np.float64)), _p(out))
+    return out

@@ -519,3 +519,47 @@ void rfo_stats(const double* field, int64_t N, int Nmc, double* avg,
     *gAvg = ts / ((double)N * Nmc);
     *gStd = sqrt(ts2 / ((double)N * Nmc) - (*gAvg) * (*gAvg));
 }
+
+/* rebuild_Sk_FFT + rebuild_Sk + rebuild_corrL (statistics_RF.f90:180-443) on
+ * one realization u of a structured grid (x fastest), restated without the
+ * FFT: per direction a, the line-wise FFT, division by n_a, |.|^2, average
+ * over lines and unnormalised inverse FFT equal the circular
+ * autocorrelation averaged over lines,
+ *     R_a[m] = (1/N) sum_x v(x) v(x + m e_a mod n_a),  v = (u - avg)/std,
+ * of which lags m = 0 .. n_a/2 - 1 are kept (:215-233).  R holds them for
+ * a = 0 .. nDim-1 back to back.  corrL_a = 2 (sum R_a - (R_a[0] +
+ * R_a[M-1])/2) xStep_a (:438-445).  Sums in FP64, loop order x fastest. */
+void rfo_corr(const double* u, int nDim, const int32_t* n, double avg, double std, double* R)
+{
+    int64_t N = 1;
+    for (int a = 0; a < nDim; a++) N *= n[a];
+    int64_t stride[3] = {1, 1, 1};
+    for (int a = 1; a < nDim; a++) stride[a] = stride[a - 1] * n[a - 1];
+    double* out = R;
+    for (int a = 0; a < nDim; a++) {
+        const int M = n[a] / 2;
+        for (int m = 0; m < M; m++) {
+            double s = 0.0;
+            for (int64_t i = 0; i < N; i++) {
+                const int64_t ia = (i / stride[a]) % n[a];
+                const int64_t ja = (ia + m) % n[a];
+                const int64_t j = i + (ja - ia) * stride[a];
+                s += ((u[i] - avg) / std) * ((u[j] - avg) / std);
+            }
+            out[m] = s / (double)N;
+        }
+        out += M;
+    }
+}
+
+void rfo_corrL(const double* R, int nDim, const int32_t* n, const double* xStep, double* corrL)
+{
+    const double* r = R;
+    for (int a = 0; a < nDim; a++) {
+        const int M = n[a] / 2;
+        double s = 0.0;
+        for (int m = 0; m < M; m++) s += r[m];
+        corrL[a] = M > 0 ? 2.0 * (s - (r[0] + r[M - 1]) / 2.0) * xStep[a] : 0.0;
+        r += M;
+    }
+}

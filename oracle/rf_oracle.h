@@ -115,6 +115,8 @@ void rfo_blend(int nDim, const int32_t* groupN, int Nmc, int nSubdomains,
 void rfo_normalize(double* field, int64_t N, int Nmc);
 /* statistics_RF.f90:62-87: per-realization mean and (biased) std, then the
  * global mean and std over all realizations. */
+void rfo_corr(const double* u, int nDim, const int32_t* n, double avg, double std, double* R);
+void rfo_corrL(const double* R, int nDim, const int32_t* n, const double* xStep, double* corrL);
 void rfo_stats(const double* field, int64_t N, int Nmc, double* avg,
                double* std, double* gAvg, double* gStd);
 

@@ -242,6 +242,105 @@ def blend(ctx: Context, groupN, Nmc, origins, sizes, nOvl, faceMasks, mode, d_fi
                                 dtype_code, _ptr(stream)), ctx.handle)
 
 
+class BlendPlan:
+    """rf_blend_plan: blend tables uploaded once; execute() is one launch."""
+
+    def __init__(self, ctx: Context, groupN, Nmc, origins, sizes, nOvl, faceMasks, mode, d_fields, dtype_code):
+        self.ctx = ctx
+        self._fields = list(d_fields)
+        g = np.ascontiguousarray(groupN, dtype=np.int32)
+        o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1)
+        s = np.ascontiguousarray(sizes, dtype=np.int32).reshape(-1)
+        n = np.ascontiguousarray(nOvl, dtype=np.int32)
+        m = np.ascontiguousarray(faceMasks, dtype=np.int32)
+        ptrs = (C.c_void_p * len(self._fields))(*[_ptr(f).value for f in self._fields])
+        self._h = C.c_void_p()
+        check(lib().rf_blend_plan_create(ctx.handle, len(groupN), _ptr(g), Nmc, len(self._fields), _ptr(o), _ptr(s),
+                                         _ptr(n), _ptr(m), mode, C.cast(ptrs, C.c_void_p), dtype_code,
+                                         C.byref(self._h)), ctx.handle)
+
+    def execute(self, d_group, stream=None) -> None:
+        check(lib().rf_blend_plan_execute(self._h, _ptr(d_group), _ptr(stream)), self.ctx.handle)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rf_blend_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GridBatch:
+    """rf_grid_batch: several structured-grid fields in three grouped launches."""
+
+    def __init__(self, ctx: Context, plans, origins, steps, nsteps, outs, dtype_code):
+        self.ctx = ctx
+        self._plans, self._outs = list(plans), list(outs)
+        n = len(self._plans)
+
+        def pad3(rows, fill, dt):
+            a = np.full((n, 3), fill, dtype=dt)
+            for i, r in enumerate(rows):
+                a[i, : len(r)] = r
+            return a
+        o, st, ns = pad3(origins, 0.0, np.float64), pad3(steps, 1.0, np.float64), pad3(nsteps, 1, np.int32)
+        pp = (C.c_void_p * n)(*[pl._h.value for pl in self._plans])
+        op = (C.c_void_p * n)(*[_ptr(t).value for t in self._outs])
+        self._h = C.c_void_p()
+        check(lib().rf_grid_batch_create(ctx.handle, n, C.cast(pp, C.c_void_p), _ptr(o), _ptr(st), _ptr(ns),
+                                         C.cast(op, C.c_void_p), dtype_code, C.byref(self._h)), ctx.handle)
+
+    def execute(self, stream=None) -> None:
+        check(lib().rf_grid_batch_execute(self._h, _ptr(stream)), self.ctx.handle)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rf_grid_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """Captures the library launches enqueued on `stream` inside the with
+    block; launch() replays them."""
+
+    def __init__(self, ctx: Context, stream):
+        self.ctx, self.stream = ctx, stream
+        self._h = C.c_void_p()
+
+    def __enter__(self):
+        check(lib().rf_graph_capture_begin(self.ctx.handle, _ptr(self.stream)), self.ctx.handle)
+        return self
+
+    def __exit__(self, et, ev, tb):
+        rc = lib().rf_graph_capture_end(self.ctx.handle, _ptr(self.stream), C.byref(self._h))
+        if et is None:
+            check(rc, self.ctx.handle)
+
+    def launch(self, stream=None) -> None:
+        check(lib().rf_graph_launch(self._h, _ptr(stream if stream is not None else self.stream)), self.ctx.handle)
+
+    def close(self) -> None:
+        if self._h:
+            lib().rf_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def stats(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int):
     avg = np.zeros(Nmc)
     sd = np.zeros(Nmc)
@@ -249,6 +348,18 @@ def stats(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int):
     check(lib().rf_stats_device(ctx.handle, _ptr(d_field), N, Nmc, dtype_code, _ptr(avg), _ptr(sd),
                                 _ptr(g)), ctx.handle)
     return avg, sd, g
+
+
+def corr(ctx: Context, d_field, dtype_code: int, nstep, xStep, avg: float, std: float, realization: int = 0):
+    """rf_corr_device: per-axis lag correlations (list of arrays) and corrL."""
+    n = np.ascontiguousarray(nstep, dtype=np.int32)
+    M = [int(v) // 2 for v in n]
+    R = np.zeros(sum(M))
+    cl = np.zeros(len(n))
+    check(lib().rf_corr_device(ctx.handle, _ptr(d_field), dtype_code, len(n), _ptr(n),
+                               _ptr(np.asarray(xStep, np.float64)), realization, avg, std, _ptr(R), _ptr(cl)),
+          ctx.handle)
+    return list(np.split(R, np.cumsum(M)[:-1])), cl
 
 
 def normalize(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int, stream=None) -> None:

@@ -26,10 +26,13 @@ EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
     "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
-    "rf_stats_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
+    "rf_stats_device", "rf_corr_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
     "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_read",
     "rf_server_release", "rf_server_get_stats",
+    "rf_blend_plan_create", "rf_blend_plan_execute", "rf_blend_plan_destroy",
+    "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy",
+    "rf_graph_capture_begin", "rf_graph_capture_end", "rf_graph_launch", "rf_graph_destroy",
 )
 
 
@@ -134,10 +137,21 @@ def lib() -> C.CDLL:
     L.rf_blend_device.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]
     L.rf_stats_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     L.rf_normalize_device.argtypes = [vp, vp, i64, i32, i32, vp]
+    L.rf_corr_device.argtypes = [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp]
     L.rf_device_uniforms.argtypes = [vp, C.c_uint64, C.c_uint32, i64, vp, vp]
     L.rf_uniform_host.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
     L.rf_uniform_host.restype = C.c_double
     L.rf_philox4x32_10_host.argtypes = [vp, vp, vp]
+    L.rf_blend_plan_create.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, i32, C.POINTER(vp)]
+    L.rf_blend_plan_execute.argtypes = [vp, vp, vp]
+    L.rf_blend_plan_destroy.argtypes = [vp]
+    L.rf_grid_batch_create.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, C.POINTER(vp)]
+    L.rf_grid_batch_execute.argtypes = [vp, vp]
+    L.rf_grid_batch_destroy.argtypes = [vp]
+    L.rf_graph_capture_begin.argtypes = [vp, vp]
+    L.rf_graph_capture_end.argtypes = [vp, vp, C.POINTER(vp)]
+    L.rf_graph_launch.argtypes = [vp, vp]
+    L.rf_graph_destroy.argtypes = [vp]
     L.rf_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
     L.rf_server_destroy.argtypes = [vp]
     L.rf_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]

@@ -21,6 +21,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <new>
 #include <utility>
 
 #include "rf_blend.h"
@@ -51,14 +53,22 @@ void axis_factors(int N, int n, bool lower, bool upper, int mode, double* low, d
 
 }  // namespace
 
-extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
-                               int32_t nSub, const int32_t* origins, const int32_t* sizes,
-                               const int32_t* nOvl, const int32_t* faceMasks, int32_t mode,
-                               const void* const* d_fields, void* d_group, int32_t dtype,
-                               void* stream) {
+struct rf_blend_plan {
+    rf_context* ctx = nullptr;
+    char* d_blob = nullptr;
+    rf_blend_dev b{};
+    int dtype = RF_DTYPE_F64;
+};
+
+extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
+                                    int32_t nSub, const int32_t* origins, const int32_t* sizes,
+                                    const int32_t* nOvl, const int32_t* faceMasks, int32_t mode,
+                                    const void* const* d_fields, int32_t dtype, rf_blend_plan** out) {
+    if (out) *out = nullptr;
+    if (!out) return rf_set_error(ctx, RF_E_ARG, "plan out-pointer is NULL");
     if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
     if (nDim < 1 || nDim > 3 || Nmc < 1 || nSub < 1) return rf_set_error(ctx, RF_E_ARG, "bad blend sizes");
-    if (!groupN || !origins || !sizes || !nOvl || !faceMasks || !d_fields || !d_group)
+    if (!groupN || !origins || !sizes || !nOvl || !faceMasks || !d_fields)
         return rf_set_error(ctx, RF_E_ARG, "NULL blend argument");
     if (mode != RF_REF_COMPAT && mode != RF_FIXED) return rf_set_error(ctx, RF_E_ARG, "unknown mode");
     if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
@@ -208,32 +218,74 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
     const size_t oUp = put(upF.data(), upF.size() * 8);
 
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    char* d_blob = nullptr;
-    RF_HIP_TRY(ctx, hipMalloc((void**)&d_blob, blob.size()));
-    hipError_t e = hipMemcpyAsync(d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) {
-        rf_blend_dev b{};
-        b.nDim = nDim; b.Nmc = Nmc; b.mode = mode;
-        for (int a = 0; a < 3; a++) {
-            b.groupN[a] = a < nDim ? groupN[a] : 1;
-            b.nSeg[a] = nSeg[a];
-            b.seg[a] = reinterpret_cast<const int32_t*>(d_blob + oSeg[a]);
-            if (a < nDim) {
-                b.pu1Low[a] = reinterpret_cast<const double*>(d_blob + oP1[a][0]);
-                b.pu1Up[a] = reinterpret_cast<const double*>(d_blob + oP1[a][1]);
-            }
-        }
-        b.groupTot = gTot;
-        b.cellOff = reinterpret_cast<const int32_t*>(d_blob + oCell);
-        b.entries = reinterpret_cast<const rf_blend_entry*>(d_blob + oEnt);
-        b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
-        b.upF = reinterpret_cast<const double*>(d_blob + oUp);
-        b.maxSize = maxSize;
-        e = rf_launch_blend_dev(b, d_group, dtype, s);
+    std::unique_ptr<rf_blend_plan> P(new (std::nothrow) rf_blend_plan());
+    if (!P) return rf_set_error(ctx, RF_E_NOMEM, "blend plan allocation failed");
+    P->ctx = ctx;
+    P->dtype = dtype;
+    RF_HIP_TRY(ctx, hipMalloc((void**)&P->d_blob, blob.size()));
+    const hipError_t e = hipMemcpy(P->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(P->d_blob);
+        return rf_set_error(ctx, RF_E_HIP, std::string("blend tables: ") + hipGetErrorString(e));
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(s);   // tables freed below
-    (void)hipFree(d_blob);
+    rf_blend_dev& b = P->b;
+    const char* d_blob = P->d_blob;
+    b.nDim = nDim; b.Nmc = Nmc; b.mode = mode;
+    for (int a = 0; a < 3; a++) {
+        b.groupN[a] = a < nDim ? groupN[a] : 1;
+        b.nSeg[a] = nSeg[a];
+        b.seg[a] = reinterpret_cast<const int32_t*>(d_blob + oSeg[a]);
+        if (a < nDim) {
+            b.pu1Low[a] = reinterpret_cast<const double*>(d_blob + oP1[a][0]);
+            b.pu1Up[a] = reinterpret_cast<const double*>(d_blob + oP1[a][1]);
+        }
+    }
+    b.groupTot = gTot;
+    b.cellOff = reinterpret_cast<const int32_t*>(d_blob + oCell);
+    b.entries = reinterpret_cast<const rf_blend_entry*>(d_blob + oEnt);
+    b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
+    b.upF = reinterpret_cast<const double*>(d_blob + oUp);
+    b.maxSize = maxSize;
+    *out = P.release();
+    return RF_OK;
+}
+
+extern "C" int rf_blend_plan_execute(rf_blend_plan* P, void* d_group, void* stream) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "blend plan is NULL");
+    if (!d_group) return rf_set_error(P->ctx, RF_E_ARG, "NULL group buffer");
+    rf_context* ctx = P->ctx;
+    int dev = -1;
+    RF_HIP_TRY(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device) RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
+    return RF_OK;
+}
+
+extern "C" int rf_blend_plan_destroy(rf_blend_plan* P) {
+    if (!P) return RF_OK;
+    (void)hipSetDevice(P->ctx->device);
+    (void)hipFree(P->d_blob);          // the caller has no launch of this plan pending
+    delete P;
+    return RF_OK;
+}
+
+extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
+                               int32_t nSub, const int32_t* origins, const int32_t* sizes,
+                               const int32_t* nOvl, const int32_t* faceMasks, int32_t mode,
+                               const void* const* d_fields, void* d_group, int32_t dtype,
+                               void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!d_group) return rf_set_error(ctx, RF_E_ARG, "NULL blend argument");
+    rf_blend_plan* P = nullptr;
+    int rc = rf_blend_plan_create(ctx, nDim, groupN, Nmc, nSub, origins, sizes, nOvl, faceMasks, mode, d_fields,
+                                  dtype, &P);
+    if (rc) return rc;
+    rc = rf_blend_plan_execute(P, d_group, stream);
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const hipError_t e = hipStreamSynchronize(s);   // tables freed below
+    rf_blend_plan_destroy(P);
+    if (rc) return rc;
     if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("blend: ") + hipGetErrorString(e));
     return RF_OK;
 }

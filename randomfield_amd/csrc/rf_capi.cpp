@@ -480,9 +480,11 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     }
     double corrL[3] = {1.0, 1.0, 1.0};
     for (int a = 0; a < p.nDim; a++) corrL[a] = p.corrL[a];
-    RF_HIP_TRY(ctx, rf_launch_grid_sum(p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
-                                       nstep, corrL, P->d_amp, P->d_phase, in.ampMult, P->epi, d_out,
-                                       outDtype, static_cast<float2*>(P->d_gridScratch), s));
+    rf_grid_stages st;
+    RF_HIP_TRY(ctx, rf_grid_prepare(p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step, nstep,
+                                    corrL, P->d_amp, P->d_phase, in.ampMult, P->epi, d_out, outDtype,
+                                    static_cast<float2*>(P->d_gridScratch), s, &st));
+    RF_HIP_TRY(ctx, rf_grid_run(st, s));
     return RF_OK;
 }
 
@@ -612,6 +614,53 @@ int rf_stats_device(rf_context* ctx, const void* d_field, int64_t N, int32_t Nmc
     if (g) {
         g[0] = t1 / ((double)N * Nmc);
         g[1] = std::sqrt(t2 / ((double)N * Nmc) - g[0] * g[0]);
+    }
+    return RF_OK;
+}
+
+int rf_corr_device(rf_context* ctx, const void* d_field, int32_t dtype, int32_t nDim, const int32_t* nstep,
+                   const double* xStep, int32_t realization, double avg, double std, double* R, double* corrL) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!d_field || !nstep || !R || nDim < 1 || nDim > 3) return rf_set_error(ctx, RF_E_ARG, "bad field or grid");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    if (!(std > 0.0) || realization < 0) return rf_set_error(ctx, RF_E_ARG, "std must be > 0, realization >= 0");
+    int64_t N = 1, stride[3] = {1, 1, 1};
+    int Mtot = 0, Mmax = 0;
+    for (int a = 0; a < nDim; a++) {
+        if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
+        if (a > 0) stride[a] = stride[a - 1] * nstep[a - 1];
+        N *= nstep[a];
+        Mtot += nstep[a] / 2;
+        Mmax = std::max(Mmax, nstep[a] / 2);
+    }
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = ctx->stream;
+    const int nPart = rf_corr_parts(N);
+    const size_t bytes = sizeof(double) * ((size_t)nPart * Mmax + (size_t)Mtot);
+    int rc = ensure_scratch(ctx, bytes);
+    if (rc) return rc;
+    double* part = static_cast<double*>(ctx->d_scratch);
+    double* dR = part + (size_t)nPart * Mmax;
+    const size_t esz = dtype == RF_DTYPE_F64 ? 8 : 4;
+    const void* u = static_cast<const char*>(d_field) + (size_t)realization * N * esz;
+    int off = 0;
+    for (int a = 0; a < nDim; a++) {
+        const int M = nstep[a] / 2;
+        RF_HIP_TRY(ctx, rf_launch_corr(u, dtype, N, stride[a], nstep[a], M, avg, std, part, nPart, dR + off, s));
+        off += M;
+    }
+    RF_HIP_TRY(ctx, hipMemcpyAsync(R, dR, sizeof(double) * Mtot, hipMemcpyDeviceToHost, s));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+    // rebuild_corrL (statistics_RF.f90:438-445): twice the trapezoid of R
+    if (corrL) {
+        const double* r = R;
+        for (int a = 0; a < nDim; a++) {
+            const int M = nstep[a] / 2;
+            double sum = 0.0;
+            for (int m = 0; m < M; m++) sum += r[m];
+            corrL[a] = M > 0 ? 2.0 * (sum - (r[0] + r[M - 1]) / 2.0) * (xStep ? xStep[a] : 1.0) : 0.0;
+            r += M;
+        }
     }
     return RF_OK;
 }

@@ -21,16 +21,6 @@ namespace {
 
 constexpr int kTM = 64, kTN = 64, kTK = 16;   // block tile; 256 threads, 4x4 each
 
-struct GemmDesc {
-    const float2* A; int64_t aSm, aSk, aStrideB; int aDiv, aMod;
-    const float2* B; int64_t bSk, bSn, bStrideB;
-    void* D; int64_t dSm, dSn, dStrideB;
-    int M, N, K, batches;
-    int realOut;          // 0: complex D, 1: Re(D) with the epilogue below
-    int outDtype;
-    rf_epilogue epi;
-};
-
 __device__ inline float2 cmul_add(float2 a, float2 b, float2 c) {
     c.x = fmaf(a.x, b.x, c.x);
     c.x = fmaf(-a.y, b.y, c.x);
@@ -45,19 +35,14 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
     return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
 }
 
-// RealOnly: only Re(D) is formed (2 FMAs per complex product instead of 4);
-// the last stage, whose output is the real field, uses it.
+// One 64 x 64 output tile (rows m0.., columns n0.., batch) of
+// D = A B; RealOnly forms only Re(D) (2 FMAs per complex product instead of
+// 4), as the last stage, whose output is the real field, needs.
 template <bool RealOnly>
-__global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
-    __shared__ float2 sA[kTK][kTM + 1];
-    __shared__ float2 sB[kTK][kTN + 1];
-    const int n0 = blockIdx.x * kTN;
+__device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0, int batch,
+                                          float2 (*sA)[kTM + 1], float2 (*sB)[kTN + 1]) {
     const int tid = threadIdx.x;
     const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
-    // grid y and z are capped at 65535 by the launcher; larger row-tile and
-    // batch counts are covered by these block-uniform loops
-    for (int batch = blockIdx.z; batch < g.batches; batch += gridDim.z)
-    for (int m0 = blockIdx.y * kTM; m0 < g.M; m0 += gridDim.y * kTM) {
     const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB;
     const float2* B = g.B + (int64_t)batch * g.bStrideB;
     float2 acc[4][4];
@@ -109,7 +94,7 @@ __global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
             const int n = n0 + tn + j;
             if (n >= g.N) continue;
             const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
-            if (!g.realOut) {
+            if (!RealOnly) {
                 static_cast<float2*>(g.D)[o] = acc[i][j];
             } else {
                 const float u = epi32(acc[i][j].x, g.epi);
@@ -118,7 +103,40 @@ __global__ __launch_bounds__(256) void k_cgemm(GemmDesc g) {
             }
         }
     }
+}
+
+template <bool RealOnly>
+__global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g) {
+    __shared__ float2 sA[kTK][kTM + 1];
+    __shared__ float2 sB[kTK][kTN + 1];
+    // grid y and z are capped at 65535 by the launcher; larger row-tile and
+    // batch counts are covered by these block-uniform loops
+    for (int batch = blockIdx.z; batch < g.batches; batch += gridDim.z)
+        for (int m0 = blockIdx.y * kTM; m0 < g.M; m0 += gridDim.y * kTM)
+            gemm_tile<RealOnly>(g, m0, blockIdx.x * kTN, batch, sA, sB);
+}
+
+// Grouped form: the GEMMs of several fields in one launch; GEMM q owns tiles
+// tile0 .. tile0 + tiles(q) - 1, ordered (batch, row tile, column tile).
+template <bool RealOnly>
+__global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG) {
+    __shared__ float2 sA[kTK][kTM + 1];
+    __shared__ float2 sB[kTK][kTN + 1];
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nG - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gs[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
     }
+    const rf_gemm_desc& g = gs[lo];
+    const int64_t local = b - g.tile0;
+    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + kTM - 1) / kTM;
+    const int nI = (int)(local % ntN);
+    const int64_t r = local / ntN;
+    const int mI = (int)(r % ntM);
+    const int batch = (int)(r / ntM);
+    gemm_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, sA, sB);
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -160,7 +178,15 @@ __global__ void k_coeffs(int64_t n, const double* __restrict__ amp, const double
     C[e] = make_float2((float)(a * c), (float)(a * s));
 }
 
-hipError_t gemm(const GemmDesc& g, hipStream_t s) {
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int64_t rf_gemm_tiles(const rf_gemm_desc& g) {
+    return (int64_t)((g.N + kTN - 1) / kTN) * ((g.M + kTM - 1) / kTM) * g.batches;
+}
+
+hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
     const int64_t mt = ((int64_t)g.M + kTM - 1) / kTM;
     dim3 grid((unsigned)((g.N + kTN - 1) / kTN), (unsigned)(mt < 65535 ? mt : 65535),
               (unsigned)(g.batches < 65535 ? g.batches : 65535));
@@ -169,9 +195,14 @@ hipError_t gemm(const GemmDesc& g, hipStream_t s) {
     return hipGetLastError();
 }
 
-inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
-
-}  // namespace
+hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles,
+                                hipStream_t s) {
+    if (nG <= 0 || nTiles <= 0) return hipSuccess;
+    if (nTiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (realOut) hipLaunchKernelGGL(k_cgemm_group<true>, dim3((unsigned)nTiles), dim3(256), 0, s, d_gs, nG);
+    else hipLaunchKernelGGL(k_cgemm_group<false>, dim3((unsigned)nTiles), dim3(256), 0, s, d_gs, nG);
+    return hipGetLastError();
+}
 
 // Scratch for one grid execution (float2 elements).
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n) {
@@ -186,11 +217,15 @@ size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, c
     return (size_t)(tables + coeff + t1 + t2);
 }
 
-hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta,
-                              int nb, const double* origin, const double* step, const int32_t* n,
-                              const double* corrL, const double* d_amp, const double* d_phase,
-                              double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
-                              float2* scratch, hipStream_t s) {
+// Builds the phase tables and coefficients in scratch (launches on s) and
+// describes the GEMM stages: stage 1 rows (r, b, j_d..j_2) x j1 -> x1;
+// stage 2 (3-D) per (r, b, j3): x2 x j2 times j2 x x1; last stage per r:
+// x_d x (b, j_d) times (b, j_d) x plane, real part, epilogue, into d_out.
+hipError_t rf_grid_prepare(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
+                           const double* origin, const double* step, const int32_t* n,
+                           const double* corrL, const double* d_amp, const double* d_phase,
+                           double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                           float2* scratch, hipStream_t s, rf_grid_stages* st) {
     const int64_t R = (int64_t)nb * Nmc;
     float2* p = scratch;
     float2* E[3];
@@ -208,25 +243,22 @@ hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const do
     }
     int64_t nkAll = 1;
     for (int a = 0; a < nDim; a++) nkAll *= kNStep[a];
-    float2* C = p; p += R * nkAll;
+    float2* Cf = p; p += R * nkAll;
     hipLaunchKernelGGL(k_coeffs, dim3(blocks_for(R * nkAll)), dim3(256), 0, s, R * nkAll, d_amp, d_phase,
-                       ampMult, C);
-    // stage 1: rows (r, b, j_d..j_2) x j1 -> x1
+                       ampMult, Cf);
+    *st = rf_grid_stages{};
     const int64_t rows1 = R * nkAll / kNStep[0];
     float2* T1 = p; p += rows1 * n[0];
-    GemmDesc g{};
-    g.A = C; g.aSm = kNStep[0]; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
+    rf_gemm_desc& g = st->g[st->n++];
+    g.A = Cf; g.aSm = kNStep[0]; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
     g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;
     g.D = T1; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
     g.M = (int)rows1; g.N = n[0]; g.K = kNStep[0]; g.batches = 1; g.realOut = 0;
-    hipError_t e = gemm(g, s);
-    if (e != hipSuccess) return e;
     const float2* last = T1;
     int64_t plane = n[0];                      // points already resolved
     if (nDim == 3) {
-        // stage 2: per (r, b, j3): x2 x j2 (E2^{s2}) times j2 x x1
         float2* T2 = p; p += R * kNStep[2] * n[1] * n[0];
-        GemmDesc h{};
+        rf_gemm_desc& h = st->g[st->n++];
         h.A = E[1]; h.aSm = 1; h.aSk = n[1]; h.aStrideB = (int64_t)kNStep[1] * n[1];
         h.B = T1; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)kNStep[1] * n[0];
         h.D = T2; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
@@ -235,18 +267,23 @@ hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const do
         // bit 1 of b (3-D, nb = 4), i.e. (batch / (2 nk3)) % 2
         h.aDiv = kNStep[2] * 2;
         h.aMod = 2;
-        e = gemm(h, s);
-        if (e != hipSuccess) return e;
         last = T2;
         plane = (int64_t)n[0] * n[1];
     }
-    // last stage: per r, x_d x (b, j_d) times (b, j_d) x plane, real part
     const int ad = nDim - 1;
-    GemmDesc f{};
+    rf_gemm_desc& f = st->g[st->n++];
     f.A = Ext[ad]; f.aSm = 1; f.aSk = n[ad]; f.aStrideB = 0; f.aDiv = 1; f.aMod = 1;
     f.B = last; f.bSk = plane; f.bSn = 1; f.bStrideB = (int64_t)nb * kNStep[ad] * plane;
     f.D = d_out; f.dSm = plane; f.dSn = 1; f.dStrideB = plane * n[ad];
     f.M = n[ad]; f.N = (int)plane; f.K = nb * kNStep[ad]; f.batches = Nmc; f.realOut = 1;
     f.outDtype = outDtype; f.epi = epi;
-    return gemm(f, s);
+    return hipGetLastError();
+}
+
+hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s) {
+    for (int i = 0; i < st.n; i++) {
+        const hipError_t e = rf_launch_gemm(st.g[i], s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }

@@ -182,13 +182,33 @@ inline rf_epilogue rf_make_epilogue(int margi, double avg, double var) {
     return e;
 }
 
-// structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice
+// structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice:
+// batched complex FP32 GEMMs D[b] = A[(b / aDiv) % aMod] B[b]
+struct rf_gemm_desc {
+    const float2* A; int64_t aSm, aSk, aStrideB; int aDiv, aMod;
+    const float2* B; int64_t bSk, bSn, bStrideB;
+    void* D; int64_t dSm, dSn, dStrideB;
+    int M, N, K, batches;
+    int realOut;          // 0: complex D, 1: Re(D) with the epilogue, outDtype
+    int outDtype;
+    rf_epilogue epi;
+    int64_t tile0;        // grouped launches: first tile of this GEMM
+};
+struct rf_grid_stages {
+    rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)
+    int n;
+};
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
-hipError_t rf_launch_grid_sum(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta,
-                              int nb, const double* origin, const double* step, const int32_t* n,
-                              const double* corrL, const double* d_amp, const double* d_phase,
-                              double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
-                              float2* scratch, hipStream_t s);
+hipError_t rf_grid_prepare(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
+                           const double* origin, const double* step, const int32_t* n,
+                           const double* corrL, const double* d_amp, const double* d_phase,
+                           double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                           float2* scratch, hipStream_t s, rf_grid_stages* st);
+hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);
+int64_t rf_gemm_tiles(const rf_gemm_desc& g);
+hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s);
+hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles,
+                                hipStream_t s);
 
 // post-processing (rf_post.hip)
 hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, int64_t count,

@@ -113,6 +113,46 @@ __global__ void k_affine(T* __restrict__ d, int64_t N, const double* __restrict_
         col[i] = (T)(((double)col[i] - m) * s);
 }
 
+// ------------------------------------------------- directional correlation
+// rebuild_Sk_FFT (statistics_RF.f90:262-409) restated as the circular
+// autocorrelation it computes: for lag m (grid y) the sum over all points of
+// v(x) v(x + m e_a mod n_a), v = (u - avg)/std, in FP64; partials per block
+// (grid x), reduced in fixed order by k_corr_final.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_corr_partial(const T* __restrict__ u, int64_t N, int64_t strideA,
+                                                         int nA, double avg, double std,
+                                                         double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    const int m = blockIdx.y;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+        const int64_t ia = (i / strideA) % nA;
+        int64_t ja = ia + m;
+        if (ja >= nA) ja -= nA;
+        const int64_t j = i + (ja - ia) * strideA;
+        const double vi = ((double)u[i] - avg) / std;
+        const double vj = ((double)u[j] - avg) / std;
+        acc = fma(vi, vj, acc);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)m * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void k_corr_final(const double* __restrict__ part, int nPart, int M, double invN,
+                             double* __restrict__ R) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    double s = 0.0;
+    for (int p = 0; p < nPart; p++) s += part[(int64_t)m * nPart + p];
+    R[m] = s * invN;
+}
+
 // ----------------------------------------------------------------- blend
 // One thread per group point and chunk of kBlendR realizations (grid y), so
 // the blocks in flight sweep the points of a few realizations at a time (few
@@ -272,5 +312,27 @@ hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, 
         hipLaunchKernelGGL(k_blend<double>, grid, dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
     else
         hipLaunchKernelGGL(k_blend<float>, grid, dim3(kBlock), 0, s, b, static_cast<float*>(d_group));
+    return hipGetLastError();
+}
+
+int rf_corr_parts(int64_t N) {
+    int64_t p = (N + kBlock * 16 - 1) / (kBlock * 16);
+    if (p < 1) p = 1;
+    if (p > 512) p = 512;
+    return (int)p;
+}
+
+hipError_t rf_launch_corr(const void* u, int dtype, int64_t N, int64_t strideA, int nA, int M, double avg,
+                          double std, double* d_part, int nPart, double* d_R, hipStream_t s) {
+    if (M <= 0) return hipSuccess;
+    const dim3 grid((unsigned)nPart, (unsigned)M);
+    if (dtype == RF_DTYPE_F64)
+        hipLaunchKernelGGL(k_corr_partial<double>, grid, dim3(kBlock), 0, s, static_cast<const double*>(u), N,
+                           strideA, nA, avg, std, d_part);
+    else
+        hipLaunchKernelGGL(k_corr_partial<float>, grid, dim3(kBlock), 0, s, static_cast<const float*>(u), N,
+                           strideA, nA, avg, std, d_part);
+    hipLaunchKernelGGL(k_corr_final, dim3((unsigned)((M + 63) / 64)), dim3(64), 0, s, d_part, nPart, M,
+                       1.0 / (double)N, d_R);
     return hipGetLastError();
 }

@@ -112,3 +112,92 @@ def test_localization_pipeline_small(ctx):
     e = rel_rms(g.cpu().numpy().astype(np.float64), gref)
     print(f"[parity] localization pipeline: rel_rms={e:.3e}", flush=True)
     assert e <= 1e-5
+
+
+def _grid_fields(ctx, specs, dtype):
+    import torch
+    plans, outs, geo = [], [], []
+    for (nstep, step, origin, corrL, Nmc, seed) in specs:
+        d = len(nstep)
+        ext = [(nstep[a] - 1) * step[a] for a in range(d)]
+        p = R.make_params(nDim=d, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=Nmc, corrL=corrL,
+                          xRange=ext, seed=seed, margiFirst=R.FOM_GAUSSIAN, fieldAvg=0.5, fieldVar=2.0)
+        plans.append(R.FieldPlan(ctx, p))
+        outs.append(torch.empty((Nmc, int(np.prod(nstep))), dtype=dtype, device="cuda"))
+        geo.append((origin, step, nstep))
+    return plans, outs, geo
+
+
+SPECS = [((20, 18, 16), (0.25, 0.25, 0.25), (1.5, -2.0, 0.25), (1.0, 1.2, 0.8), 3, 11),
+         ((33, 7), (0.3, 0.5), (0.0, 1.0), (1.0, 1.0), 5, 12),
+         ((9, 40, 12), (0.5, 0.2, 0.3), (0.0, 0.0, 0.0), (0.7, 1.3, 1.0), 1, 13),
+         ((64, 64, 8), (0.25, 0.25, 0.25), (4.0, 0.0, 1.0), (1.0, 1.0, 1.0), 16, 14)]
+
+
+def test_grid_batch_equals_single(ctx):
+    """All fields in three grouped launches give the same bits as one
+    rf_plan_execute_grid per field."""
+    import torch
+    plans, outs, geo = _grid_fields(ctx, SPECS, torch.float32)
+    singles = []
+    for pl, o, (org, st, ns) in zip(plans, outs, geo):
+        pl.execute_grid(org, st, ns, o)
+        ctx.sync()
+        singles.append(o.cpu().numpy().copy())
+        o.zero_()
+    b = R.GridBatch(ctx, plans, [g[0] for g in geo], [g[1] for g in geo], [g[2] for g in geo], outs, R.DTYPE_F32)
+    b.execute()
+    ctx.sync()
+    for i, o in enumerate(outs):
+        assert np.array_equal(o.cpu().numpy(), singles[i]), i
+    b.close()
+    for pl in plans:
+        pl.close()
+
+
+def test_graph_replay_of_batch_and_blend(ctx):
+    """Generation of every subdomain (grouped launches) and the blend,
+    captured once and replayed; each replay, with eager work and
+    synchronisation in between, matches the eager result."""
+    import torch
+    groupN, parts, nOvl, step, Nmc = (40, 36, 20), (3, 2, 2), 5, (0.25, 0.25, 0.25), 2
+    origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
+    specs = []
+    for s in range(len(sizes)):
+        n = tuple(int(v) for v in sizes[s])
+        specs.append((n, step, tuple(origins[s][a] * step[a] for a in range(3)), (1.0, 1.0, 1.0), Nmc, 100 + s + 1))
+    plans, outs, geo = _grid_fields(ctx, specs, torch.float32)
+    g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float32, device="cuda")
+    batch = R.GridBatch(ctx, plans, [x[0] for x in geo], [x[1] for x in geo], [x[2] for x in geo], outs, R.DTYPE_F32)
+    bp = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, outs, R.DTYPE_F32)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    batch.execute(sp)
+    bp.execute(g, sp)
+    torch.cuda.synchronize()
+    eager = g.cpu().numpy().copy()
+    eager_f = [o.cpu().numpy().copy() for o in outs]
+    # the one-shot blend gives the same group field
+    g2 = torch.empty_like(g)
+    R.blend(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, outs, g2, R.DTYPE_F32)
+    ctx.sync()
+    assert np.array_equal(g2.cpu().numpy(), eager)
+    graph = R.Graph(ctx, sp)
+    with graph:
+        batch.execute(sp)
+        bp.execute(g, sp)
+    for rep in range(3):
+        g.fill_(float(rep + 7))
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        graph.launch(sp)
+        torch.cuda.synchronize()
+        assert np.array_equal(g.cpu().numpy(), eager), rep
+        for i, o in enumerate(outs):
+            assert np.array_equal(o.cpu().numpy(), eager_f[i]), (rep, i)
+    graph.close()
+    bp.close()
+    batch.close()
+    for pl in plans:
+        pl.close()

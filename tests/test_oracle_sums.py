@@ -192,3 +192,45 @@ def test_normalize_and_stats():
     assert np.allclose(avg, 0, atol=1e-12) and np.allclose(sd, 1, rtol=1e-12)
     a2, s2, _, _ = O.stats(f)
     assert np.allclose(a2, f.mean(1), rtol=1e-12) and np.allclose(s2, f.std(1), rtol=1e-10)
+
+
+# ---------------------------------------------------------- directional correlation
+def _ref_fft_steps(u, nstep, avg, std):
+    """statistics_RF.f90:262-409 step by step with numpy's FFT: line FFTs
+    along each axis, divide by n, times the conjugate, mean over the other
+    axes, unnormalised inverse FFT, real part, first n/2 lags."""
+    v = ((u - avg) / std).reshape(tuple(reversed(nstep)))       # [z][y][x]
+    out = []
+    nd = len(nstep)
+    for a in range(nd):
+        ax = nd - 1 - a
+        F = np.fft.fft(v, axis=ax) / nstep[a]
+        P = (F * np.conj(F))
+        other = tuple(i for i in range(nd) if i != ax)
+        Pm = P.mean(axis=other) if other else P
+        r = np.real(np.fft.ifft(Pm) * nstep[a])
+        out.append(r[: nstep[a] // 2])
+    return out
+
+
+@pytest.mark.parametrize("nstep", [(16,), (12, 9), (10, 7, 6), (5, 8, 11)])
+def test_corr_matches_fft_statement(nstep):
+    rng = np.random.default_rng(len(nstep) + nstep[0])
+    u = rng.normal(0.3, 1.7, int(np.prod(nstep)))
+    avg, std = float(u.mean()), float(u.std())
+    got = O.corr(u, nstep, avg, std)
+    ref = _ref_fft_steps(u, nstep, avg, std)
+    for a in range(len(nstep)):
+        assert np.allclose(got[a], ref[a], rtol=1e-11, atol=1e-12), a
+    # lag 0 of a normalised field is its variance, 1
+    assert all(abs(r[0] - 1.0) < 1e-12 for r in got)
+
+
+def test_corrL_trapezoid_of_gaussian_correlation():
+    """2 * trapezoid of R(tau) = exp(-pi tau^2 / L^2) over [0, (M-1) dx]
+    approaches the integral L; a plain sum would be off by R(0) dx."""
+    L, dx, M = 1.3, 0.01, 600
+    r = np.exp(-np.pi * (np.arange(M) * dx) ** 2 / L ** 2)
+    got = O.corrL([r], [2 * M], [dx])[0]
+    assert abs(got - L) < 1e-6
+    assert abs(2 * r.sum() * dx - L) > 5e-3

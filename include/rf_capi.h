@@ -267,6 +267,18 @@ int rf_corr_device(rf_context* ctx, const void* d_field, int32_t dtype,
                    int32_t nDim, const int32_t* nstep, const double* xStep,
                    int32_t realization, double avg, double std, double* R,
                    double* corrL);
+/* interpolateToMesh (sample_RF.f90:872-1106): multilinear interpolation of
+ * a structured-grid field (nstep points per axis from xMin at xStep, x
+ * fastest, Nmc realizations back to back) to nNodes mesh nodes d_nodes
+ * (nDim x nNodes) for every realization (the reference interpolates the
+ * first only); d_out: nNodes x Nmc.  The reference's cell rule and corner
+ * order (border nodes use the last cell); nodes outside the grid get NaN
+ * and set *outside (may be NULL).  Synchronous. */
+int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep,
+                          const double* xMin, const double* xStep, int32_t Nmc,
+                          const void* d_grid, int32_t gridDtype,
+                          const double* d_nodes, int64_t nNodes, void* d_out,
+                          int32_t outDtype, void* stream, int32_t* outside);
 /* Subtract each realization's mean and divide by its RMS (FP64 sums). */
 int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N,
                         int32_t Nmc, int32_t dtype, void* stream);

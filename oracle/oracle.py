@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         L.rfo_normalize.argtypes = [vp, C.c_int64, C.c_int]
         L.rfo_stats.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp, vp]
         L.rfo_corr.argtypes = [vp, C.c_int, vp, C.c_double, C.c_double, vp]
+        L.rfo_interp.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, C.c_int64, vp]
+        L.rfo_interp.restype = C.c_int64
         L.rfo_corrL.argtypes = [vp, C.c_int, vp, vp, vp]
         L.rfo_iso_nvar.argtypes = [C.c_int]
         _L = L
@@ -203,3 +205,16 @@ def corrL(R: list, nstep, xStep) -> np.ndarray:
     out = np.zeros(len(n))
     lib().rfo_corrL(_p(flat), len(n), _p(n), _p(np.asarray(xStep, np.float64)), _p(out))
     return out
+
+
+def interp(grid: np.ndarray, nstep, xMin, xStep, x: np.ndarray):
+    """Multilinear interpolation of grid (Nmc, prod(nstep)) to nodes x (n, nDim):
+    returns (out (Nmc, n), number of nodes outside the grid)."""
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    Nmc = g.shape[0] if g.ndim == 2 else 1
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    n = np.ascontiguousarray(nstep, dtype=np.int32)
+    out = np.zeros((Nmc, xs.shape[0]))
+    bad = lib().rfo_interp(len(n), _p(n), _p(np.asarray(xMin, np.float64)), _p(np.asarray(xStep, np.float64)),
+                           Nmc, _p(g), _p(xs), xs.shape[0], _p(out))
+    return out, int(bad)

@@ -563,3 +563,57 @@ void rfo_corrL(const double* R, int nDim, const int32_t* n, const double* xStep,
         r += M;
     }
 }
+
+/* interpolateToMesh (sample_RF.f90:872-1106): multilinear interpolation of
+ * a structured-grid field (x fastest, nstep points from xMin at xStep) to
+ * nodes x[i*nDim + d], every realization.  1-based cell position
+ * c = (x - xMin)/xStep + 1, cell floor(c) moved down one at the upper border
+ * (:1024); the 2^d corners in the reference's order (:996-1013), weight
+ * prod(1 - |c - corner|), accumulated in that order (:1072-1086).  Returns
+ * the number of nodes outside the grid (their output is NaN). */
+int64_t rfo_interp(int nDim, const int32_t* nstep, const double* xMin, const double* xStep, int Nmc,
+                   const double* grid, const double* x, int64_t nNodes, double* out)
+{
+    static const int nb2[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+    static const int nb3[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    int64_t Ng = 1, stride[3] = {1, 1, 1};
+    for (int a = 0; a < nDim; a++) Ng *= nstep[a];
+    for (int a = 1; a < nDim; a++) stride[a] = stride[a - 1] * nstep[a - 1];
+    const int nc = 1 << nDim;
+    int64_t bad = 0;
+    for (int64_t i = 0; i < nNodes; i++) {
+        double c[3];
+        int64_t ci[3];
+        int inside = 1;
+        for (int a = 0; a < nDim; a++) {
+            c[a] = (x[i * nDim + a] - xMin[a]) / xStep[a] + 1.0;
+            ci[a] = (int64_t)floor(c[a]);
+            if (ci[a] == nstep[a]) ci[a] -= 1;
+            if (ci[a] < 1 || ci[a] + 1 > nstep[a]) inside = 0;
+        }
+        if (nDim == 1 && nstep[0] == 1) inside = 0;
+        if (!inside) {
+            bad++;
+            for (int n = 0; n < Nmc; n++) out[(int64_t)n * nNodes + i] = NAN;
+            continue;
+        }
+        for (int n = 0; n < Nmc; n++) {
+            const double* g = grid + (int64_t)n * Ng;
+            double acc = 0.0;
+            for (int j = 0; j < nc; j++) {
+                double w = 1.0;
+                int64_t idx = 0;
+                for (int a = 0; a < nDim; a++) {
+                    const int off = nDim == 3 ? nb3[j][a] : (nDim == 2 ? nb2[j][a] : j);
+                    const double dist = fabs(c[a] - (double)(ci[a] + off));
+                    w = w * (1.0 - dist);
+                    idx += (ci[a] + off - 1) * stride[a];
+                }
+                acc = acc + g[idx] * w;
+            }
+            out[(int64_t)n * nNodes + i] = acc;
+        }
+    }
+    return bad;
+}

@@ -116,6 +116,8 @@ void rfo_normalize(double* field, int64_t N, int Nmc);
 /* statistics_RF.f90:62-87: per-realization mean and (biased) std, then the
  * global mean and std over all realizations. */
 void rfo_corr(const double* u, int nDim, const int32_t* n, double avg, double std, double* R);
+int64_t rfo_interp(int nDim, const int32_t* nstep, const double* xMin, const double* xStep, int Nmc,
+                   const double* grid, const double* x, int64_t nNodes, double* out);
 void rfo_corrL(const double* R, int nDim, const int32_t* n, const double* xStep, double* corrL);
 void rfo_stats(const double* field, int64_t N, int Nmc, double* avg,
                double* std, double* gAvg, double* gStd);

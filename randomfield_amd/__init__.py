@@ -362,6 +362,18 @@ def corr(ctx: Context, d_field, dtype_code: int, nstep, xStep, avg: float, std: 
     return list(np.split(R, np.cumsum(M)[:-1])), cl
 
 
+def interpolate(ctx: Context, nstep, xMin, xStep, Nmc: int, d_grid, d_nodes, d_out, stream=None) -> bool:
+    """rf_interpolate_device: grid field (Nmc, prod(nstep)) to nodes (n, nDim)
+    into d_out (Nmc, n); returns True if some node was outside the grid."""
+    n = np.ascontiguousarray(nstep, dtype=np.int32)
+    flag = C.c_int32(0)
+    check(lib().rf_interpolate_device(ctx.handle, len(n), _ptr(n), _ptr(np.asarray(xMin, np.float64)),
+                                      _ptr(np.asarray(xStep, np.float64)), Nmc, _ptr(d_grid),
+                                      _dtype_code(d_grid.dtype), _ptr(d_nodes), int(d_nodes.shape[0]), _ptr(d_out),
+                                      _dtype_code(d_out.dtype), _ptr(stream), C.byref(flag)), ctx.handle)
+    return bool(flag.value)
+
+
 def normalize(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int, stream=None) -> None:
     check(lib().rf_normalize_device(ctx.handle, _ptr(d_field), N, Nmc, dtype_code, _ptr(stream)),
           ctx.handle)

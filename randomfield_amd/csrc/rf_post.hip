@@ -153,6 +153,76 @@ __global__ void k_corr_final(const double* __restrict__ part, int nPart, int M, 
     R[m] = s * invN;
 }
 
+// ------------------------------------------------ interpolation to nodes
+// interpolateToMesh (sample_RF.f90:872-1106) for every realization: one
+// thread per node and chunk of kInterpR realizations (grid y).  1-based cell
+// position c = (x - xMin)/xStep + 1 in FP64, cell floor(c) moved down one on
+// the upper border (:1024), corners in the reference's order (:996-1013),
+// weight prod(1 - |c - corner|), sum in corner order (:1072-1086).  Nodes
+// outside the grid get NaN and raise *flag.
+struct rf_interp_dev {
+    int nDim, Nmc;
+    int nstep[3];
+    double xMin[3], xStep[3];
+    int64_t Ng, nNodes;
+};
+constexpr int kInterpR = 4;
+
+template <typename TG, typename TO>
+__global__ __launch_bounds__(kBlock) void k_interp(rf_interp_dev q, const TG* __restrict__ grid,
+                                                   const double* __restrict__ x, TO* __restrict__ out,
+                                                   int* __restrict__ flag) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= q.nNodes) return;
+    const int r0 = blockIdx.y * kInterpR;
+    double c[3] = {0.0, 0.0, 0.0};
+    int64_t ci[3] = {1, 1, 1};
+    bool inside = true;
+    for (int a = 0; a < q.nDim; a++) {
+        c[a] = (x[i * q.nDim + a] - q.xMin[a]) / q.xStep[a] + 1.0;
+        ci[a] = (int64_t)floor(c[a]);
+        if (ci[a] == q.nstep[a]) ci[a] -= 1;
+        if (ci[a] < 1 || ci[a] + 1 > q.nstep[a]) inside = false;
+    }
+    if (!inside) {
+        flag[0] = 1;
+        for (int r = r0; r < r0 + kInterpR && r < q.Nmc; r++) out[(int64_t)r * q.nNodes + i] = (TO)NAN;
+        return;
+    }
+    const int64_t s1 = q.nstep[0], s2 = (int64_t)q.nstep[0] * q.nstep[1];
+    // corner offsets in the reference's order; 2-D uses the first four
+    // entries of the 2-D table, 1-D (0), (1)
+    const int o3[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    const int o2[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+    const int nc = 1 << q.nDim;
+    double w[8];
+    int64_t idx[8];
+    for (int j = 0; j < 8; j++) {
+        w[j] = 0.0;
+        idx[j] = 0;
+        if (j >= nc) continue;
+        double wj = 1.0;
+        int64_t id = 0;
+        for (int a = 0; a < q.nDim; a++) {
+            const int off = q.nDim == 3 ? o3[j][a] : (q.nDim == 2 ? o2[j][a] : j);
+            wj = wj * (1.0 - fabs(c[a] - (double)(ci[a] + off)));
+            id += (ci[a] + off - 1) * (a == 0 ? 1 : (a == 1 ? s1 : s2));
+        }
+        w[j] = wj;
+        idx[j] = id;
+    }
+#pragma unroll
+    for (int rr = 0; rr < kInterpR; rr++) {
+        const int r = r0 + rr;
+        if (r >= q.Nmc) break;
+        const TG* g = grid + (int64_t)r * q.Ng;
+        double acc = 0.0;
+        for (int j = 0; j < nc; j++) acc = acc + (double)g[idx[j]] * w[j];
+        out[(int64_t)r * q.nNodes + i] = (TO)acc;
+    }
+}
+
 // ----------------------------------------------------------------- blend
 // One thread per group point and chunk of kBlendR realizations (grid y), so
 // the blocks in flight sweep the points of a few realizations at a time (few
@@ -334,5 +404,34 @@ hipError_t rf_launch_corr(const void* u, int dtype, int64_t N, int64_t strideA, 
                            strideA, nA, avg, std, d_part);
     hipLaunchKernelGGL(k_corr_final, dim3((unsigned)((M + 63) / 64)), dim3(64), 0, s, d_part, nPart, M,
                        1.0 / (double)N, d_R);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const double* xMin, const double* xStep, int Nmc,
+                            const void* grid, int gridDtype, const double* x, int64_t nNodes, void* out,
+                            int outDtype, int* d_flag, hipStream_t s) {
+    if (nNodes <= 0) return hipSuccess;
+    rf_interp_dev q{};
+    q.nDim = nDim;
+    q.Nmc = Nmc;
+    q.Ng = 1;
+    for (int a = 0; a < 3; a++) {
+        q.nstep[a] = a < nDim ? nstep[a] : 1;
+        q.xMin[a] = a < nDim ? xMin[a] : 0.0;
+        q.xStep[a] = a < nDim ? xStep[a] : 1.0;
+        q.Ng *= q.nstep[a];
+    }
+    q.nNodes = nNodes;
+    const dim3 grid2((unsigned)((nNodes + kBlock - 1) / kBlock), (unsigned)((Nmc + kInterpR - 1) / kInterpR));
+    if (grid2.y > 65535) return hipErrorInvalidValue;
+    const bool g64 = gridDtype == RF_DTYPE_F64, o64 = outDtype == RF_DTYPE_F64;
+    if (g64 && o64)
+        hipLaunchKernelGGL((k_interp<double, double>), grid2, dim3(kBlock), 0, s, q, static_cast<const double*>(grid), x, static_cast<double*>(out), d_flag);
+    else if (g64)
+        hipLaunchKernelGGL((k_interp<double, float>), grid2, dim3(kBlock), 0, s, q, static_cast<const double*>(grid), x, static_cast<float*>(out), d_flag);
+    else if (o64)
+        hipLaunchKernelGGL((k_interp<float, double>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<double*>(out), d_flag);
+    else
+        hipLaunchKernelGGL((k_interp<float, float>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<float*>(out), d_flag);
     return hipGetLastError();
 }

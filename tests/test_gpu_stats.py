@@ -64,3 +64,37 @@ def test_generated_field_moments_and_correlation(ctx):
             tau = m * step[a]
             target = (n - m) / n * np.exp(-np.pi * tau ** 2 / L ** 2)
             assert abs(Ra[m] - target) < 0.06, (a, m, Ra[m], target)
+
+
+@pytest.mark.parametrize("nDim", [2, 3])
+@pytest.mark.parametrize("gdt,odt", [(np.float64, np.float64), (np.float32, np.float32)])
+def test_interpolate_matches_oracle(ctx, nDim, gdt, odt):
+    """interpolateToMesh on the device, every realization, against the
+    oracle; FP64 bit for bit (same operation order)."""
+    import torch
+    rng = np.random.default_rng(nDim)
+    nstep = (33, 20, 12)[:nDim]
+    xMin = (-1.0, 2.0, 0.5)[:nDim]
+    xStep = (0.25, 0.5, 0.75)[:nDim]
+    Nmc = 6
+    grid = rng.standard_normal((Nmc, int(np.prod(nstep)))).astype(gdt)
+    hi = np.array([xMin[a] + xStep[a] * (nstep[a] - 1) for a in range(nDim)])
+    x = rng.uniform(np.array(xMin), hi, (20000, nDim))
+    x[0] = hi
+    ref, bad = O.interp(grid.astype(np.float64), nstep, xMin, xStep, x)
+    assert bad == 0
+    tt = {np.float64: torch.float64, np.float32: torch.float32}
+    out = torch.empty((Nmc, x.shape[0]), dtype=tt[odt], device="cuda")
+    outside = R.interpolate(ctx, nstep, xMin, xStep, Nmc, torch.from_numpy(grid).cuda(), torch.from_numpy(x).cuda(),
+                            out)
+    assert not outside
+    got = out.cpu().numpy()
+    if odt == np.float64:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.array_equal(got, ref.astype(np.float32))
+    xo = np.concatenate([x[:10], (np.array(xMin) - 1.0)[None, :]])
+    out2 = torch.empty((Nmc, xo.shape[0]), dtype=tt[odt], device="cuda")
+    assert R.interpolate(ctx, nstep, xMin, xStep, Nmc, torch.from_numpy(grid).cuda(), torch.from_numpy(xo).cuda(),
+                         out2)
+    assert np.isnan(out2.cpu().numpy()[:, -1]).all()

@@ -234,3 +234,42 @@ def test_corrL_trapezoid_of_gaussian_correlation():
     got = O.corrL([r], [2 * M], [dx])[0]
     assert abs(got - L) < 1e-6
     assert abs(2 * r.sum() * dx - L) > 5e-3
+
+
+# ---------------------------------------------------------- interpolation to mesh nodes
+@pytest.mark.parametrize("nDim", [2, 3])
+def test_interp_reproduces_multilinear_fields(nDim):
+    """Multilinear interpolation is exact on fields a + sum b_d x_d + c prod x_d
+    (sample_RF.f90:1018-1086), at grid nodes, on the upper border (the
+    floor(c) == n rule) and flags nodes outside the grid."""
+    rng = np.random.default_rng(nDim)
+    nstep = (7, 5, 4)[:nDim]
+    xMin = (-1.0, 2.0, 0.5)[:nDim]
+    xStep = (0.5, 0.25, 1.0)[:nDim]
+    axes = [xMin[a] + xStep[a] * np.arange(nstep[a]) for a in range(nDim)]
+    mesh = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([m.transpose().reshape(-1) for m in mesh], axis=1)   # x fastest
+
+    def f(p, k):
+        return 0.3 * k + p @ np.arange(1, nDim + 1) + 0.7 * k * np.prod(p, axis=1)
+    grid = np.stack([f(pts, k) for k in range(3)])
+    hi = np.array([xMin[a] + xStep[a] * (nstep[a] - 1) for a in range(nDim)])
+    x = rng.uniform(np.array(xMin), hi, (500, nDim))
+    x[0] = hi                      # upper corner
+    x[1] = np.array(xMin)          # lower corner
+    x[2] = pts[7]                  # a grid node
+    out, bad = O.interp(grid, nstep, xMin, xStep, x)
+    assert bad == 0
+    ref = np.stack([f(x, k) for k in range(3)])
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(out[:, 2], grid[:, 7])
+    # below the grid, and more than one cell above it, are outside; the
+    # border rule keeps [n, n+1) in the last cell (the reference's weights
+    # 1 - |c - corner| there are not an extrapolation; only the count is
+    # checked)
+    xo = np.array([hi + 1.5 * np.array(xStep), np.array(xMin) - 0.1])
+    out2, bad2 = O.interp(grid, nstep, xMin, xStep, xo)
+    assert bad2 == 2 and np.isnan(out2).all()
+    xe = hi + 0.5 * np.array(xStep)
+    out3, bad3 = O.interp(grid, nstep, xMin, xStep, xe[None, :])
+    assert bad3 == 0 and np.isfinite(out3).all()

@@ -428,7 +428,7 @@ def main() -> int:
     # launch shapes: each timed briefly, then the library's own autotuner
     # picks the one used below
     variants_ms = []
-    for v in range(4):
+    for v in range(R.MFMA_VARIANTS):
         plan.set_variant(v)
         _, vms = _timed(lambda: plan.execute(dx, out, stream=sptr), 3, 1)
         variants_ms.append(float(vms.mean()))

@@ -22,7 +22,7 @@ import numpy as np
 from ._capi import (  # noqa: F401
     CM_COS, CM_GAUSSIAN, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
     FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
-    RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
+    MFMA_VARIANTS, RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
     RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, Params, RFError, SpectrumInfo, check, flib, lib,
     make_params,
 )
@@ -30,7 +30,8 @@ from ._capi import (  # noqa: F401
 __all__ = [
     "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate",
     "create_random_field", "multivariate_transformation", "reference_seed", "reference_draws",
-    "blend", "stats", "normalize", "device_uniforms", "RFError",
+    "blend", "BlendPlan", "GridBatch", "Graph", "stats", "corr", "interpolate", "normalize",
+    "device_uniforms", "RFError", "MFMA_VARIANTS",
 ]
 
 
@@ -156,7 +157,8 @@ class FieldPlan:
                                     _ptr(stream)), self.ctx.handle)
 
     def set_variant(self, variant: int) -> None:
-        """MFMA launch shape 0..3 (waves per block x K-steps per LDS stage)."""
+        """MFMA launch shape 0 .. MFMA_VARIANTS-1 (waves per block, K-steps per
+        LDS stage, MFMA issue order)."""
         check(lib().rf_plan_set_variant(self._h, int(variant)), self.ctx.handle)
 
     def autotune(self, x, out, stream=None) -> int:

@@ -35,13 +35,16 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 
 constexpr int kMaxMT = 8;               // up to 256 realizations per block
 constexpr int kStepK = 8;               // wavevectors per K-step (16 K entries)
-constexpr int kKPad = 2 * kStepK;       // K padding: whole stages of up to 2 steps
+constexpr int kKPad = 4 * kStepK;       // K padding: whole stages of up to 4 steps
 
-// launch variants: waves per workgroup (points per block = 32 * waves) and
-// K-steps per LDS stage (one barrier per stage)
-struct Variant { int waves, steps; };
-constexpr Variant kVariants[] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}};
+// launch variants: waves per workgroup (points per block = 32 * waves),
+// K-steps per LDS stage (one barrier per stage) and MFMA issue order (0:
+// tiles in pairs, the three products of a pair interleaved; 1: one product
+// over all tiles, then the next)
+struct Variant { int waves, steps, order; };
+constexpr Variant kVariants[] = {{4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {8, 2, 0}, {8, 4, 0}, {8, 2, 1}, {8, 4, 1}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+static_assert(kNumVariants == RF_MFMA_VARIANTS, "variant count of the C ABI");
 
 template <typename V> struct Elem;
 template <> struct Elem<h8> { typedef _Float16 T; };
@@ -126,7 +129,7 @@ __device__ inline void basis(const double* __restrict__ k, int h, const double* 
 // M-tiles mt0 .. mt0 + MT - 1.  ldsA and ldsB are two distinct __shared__
 // objects of S * MT * 128 units or more (distinct, so the waitcnt pass can
 // tell the DMA target of the next stage from the buffer being read).
-template <typename V, int D, int MT, int W, int S>
+template <typename V, int D, int MT, int W, int S, int ORD>
 __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSteps, int mtTotal,
                                                const u32x4* __restrict__ panels,
                                                const u32x4* __restrict__ kq, float outScale,
@@ -184,6 +187,20 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
             const u32x4* c = cur + sI * kStepUnits;
             V nbh, nbl;
             basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
+            if (ORD == 1) {
+                V ah[MT], al[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    ah[m] = __builtin_bit_cast(V, c[(m * 2 + 0) * 64 + lane]);
+                    al[m] = __builtin_bit_cast(V, c[(m * 2 + 1) * 64 + lane]);
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m] = mfma(ah[m], bh, acc[m]);
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m] = mfma(ah[m], bl, acc[m]);
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m] = mfma(al[m], bh, acc[m]);
+            } else
 #pragma unroll
             for (int m = 0; m < MT; m += 2) {
                 const V ah0 = __builtin_bit_cast(V, c[(m * 2 + 0) * 64 + lane]);
@@ -239,13 +256,13 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
 }
 
 
-template <typename V, int D, int MT, int W, int S>
+template <typename V, int D, int MT, int W, int S, int ORD>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
                 const u32x4* __restrict__ kq, float outScale) {
     __shared__ u32x4 ldsA[S * MT * 128];
     __shared__ u32x4 ldsB[S * MT * 128];
-    sum_mfma_block<V, D, MT, W, S>(a, nSteps, mtTotal, panels, kq, outScale,
+    sum_mfma_block<V, D, MT, W, S, ORD>(a, nSteps, mtTotal, panels, kq, outScale,
                                    a.i0 + (int64_t)blockIdx.x * (32 * W), blockIdx.y * kMaxMT, ldsA, ldsB);
 }
 
@@ -280,9 +297,9 @@ void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
     const u32x4* P = static_cast<const u32x4*>(q.ops0);
     const u32x4* K = static_cast<const u32x4*>(q.ops1);
     switch (mt) {
-        case 1: sum_mfma_block<V, 3, 1, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
-        case 2: sum_mfma_block<V, 3, 2, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
-        default: sum_mfma_block<V, 3, kGroupMT, W, S>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        case 1: sum_mfma_block<V, 3, 1, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        case 2: sum_mfma_block<V, 3, 2, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        default: sum_mfma_block<V, 3, kGroupMT, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
     }
 }
 
@@ -325,7 +342,7 @@ __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int m
     pan[64 * 8 + (int64_t)lane * 8 + el] = lo;
 }
 
-template <typename V, int D, int W, int S>
+template <typename V, int D, int W, int S, int ORD>
 hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, const void* B,
                      float outScale, hipStream_t s) {
     const u32x4* P = static_cast<const u32x4*>(B);
@@ -334,10 +351,10 @@ hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, 
     const dim3 grid((unsigned)((n + 32 * W - 1) / (32 * W)), (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
     const dim3 block(64 * W);
     switch (mt) {
-        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8, W, S>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
     }
     return hipGetLastError();
 }
@@ -345,11 +362,15 @@ hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, 
 template <typename V, int D>
 hipError_t launch_variant(int variant, int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal,
                           const void* B, float outScale, hipStream_t s) {
+    static_assert(kNumVariants == 7, "variant table and dispatch agree");
     switch (variant) {
-        case 1: return launch_mt<V, D, 4, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
-        case 2: return launch_mt<V, D, 8, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
-        case 3: return launch_mt<V, D, 8, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
-        default: return launch_mt<V, D, 4, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 1: return launch_mt<V, D, 4, 2, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 2: return launch_mt<V, D, 8, 1, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 3: return launch_mt<V, D, 8, 2, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 4: return launch_mt<V, D, 8, 4, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 5: return launch_mt<V, D, 8, 2, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 6: return launch_mt<V, D, 8, 4, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+        default: return launch_mt<V, D, 4, 1, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
     }
 }
 

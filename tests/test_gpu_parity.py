@@ -126,21 +126,21 @@ def test_mfma_variants_identical(ctx, cfg3_small, Nmc):
     plan = R.FieldPlan(ctx, p, kU, phU)
     dx = torch.from_numpy(x).cuda()
     outs = []
-    for v in range(4):
+    for v in range(R.MFMA_VARIANTS):
         plan.set_variant(v)
         o = torch.empty((Nmc, x.shape[0]), dtype=torch.float32, device="cuda")
         plan.execute(dx, o)
         ctx.sync()
         outs.append(o.cpu().numpy())
-    for v in range(1, 4):
+    for v in range(1, R.MFMA_VARIANTS):
         assert np.array_equal(outs[v], outs[0]), v
     o = torch.empty((Nmc, x.shape[0]), dtype=torch.float32, device="cuda")
     chosen = plan.autotune(dx, o)
     ctx.sync()
-    assert 0 <= chosen < 4
+    assert 0 <= chosen < R.MFMA_VARIANTS
     assert np.array_equal(o.cpu().numpy(), outs[0])
     with pytest.raises(R.RFError):
-        plan.set_variant(4)
+        plan.set_variant(R.MFMA_VARIANTS)
     plan.close()
     if Nmc == 64:
         assert report("mfma variants", outs[0].astype(np.float64), ref) <= TOL[R.FORM_MFMA_F16]

@@ -217,10 +217,19 @@ void rfo_skvec(const rfo_plan* p, const double* kPts, double* Sk)
     }
     for (int64_t j = 0; j < p->kNTotal; j++) {
         double s = 1.0;
+        int zeros = 0;
         for (int i = 0; i < d; i++) {
             double k = kPts[j * d + i];
             s = s * 1.0 * exp(-((k * k) * 1.0) / (4.0 * RFO_PI));
+            zeros += (k == 0.0);
         }
+        /* FIXED, Shinozuka lattice: the lattice starts at k = 0 with full
+         * weight, so the sign blocks count each k_a = 0 plane (a >= 2) twice
+         * and the one-sided k_1 sum takes the k_1 = 0 plane whole (a left
+         * Riemann sum): weight 1/2 per zero component is the trapezoid rule
+         * (discrete variance ~1 instead of ~1.28 at 10-20 correlation
+         * lengths).  REF_COMPAT keeps the reference's weights. */
+        if (p->mode == RFO_FIXED && p->method == RFO_SHINOZUKA) s = ldexp(s, -zeros);
         Sk[j] = s;
     }
 }

@@ -216,10 +216,16 @@ int rf_spectrum_build_tables(const rf_params* p, const rf_spectrum_info& info,
     } else {
         for (int64_t j = 0; j < kN; j++) {
             double s = 1.0;
+            int zeros = 0;
             for (int c = 0; c < d; c++) {
                 const double k = spec->k[(size_t)(j * d + c)];
                 s = s * 1.0 * std::exp(-((k * k) * 1.0) / (4.0 * RF_PI_REF));
+                zeros += (k == 0.0);
             }
+            // FIXED, Shinozuka lattice: trapezoid weight 1/2 per zero
+            // component (the reference's k = 0 planes are counted twice by
+            // the sign blocks, or whole at the end of the one-sided k_1 sum)
+            if (p->mode == RF_FIXED && p->method == RF_SHINOZUKA) s = std::ldexp(s, -zeros);
             spec->Sk[j] = s;
         }
     }

@@ -70,15 +70,25 @@ def test_ref_compat_block1_is_zero():
 
 
 def test_skvec_normalisation():
-    # S(k) = exp(-|k|^2/(4 pi)) integrates to (2 pi)^d: the lattice sum over the
-    # full space ~ (2 pi)^d / prod(kDelta) (corner rule, slightly above)
-    p = O.make_plan(3, O.SHINOZUKA, O.FIXED, [40.0] * 3)
-    k = O.kpoints(p)
-    s = O.skvec(p, k)
-    assert np.allclose(s, np.exp(-(k ** 2).sum(1) / (4 * O.PI_REF)), rtol=1e-14)
-    # half-space blocks: sum S * prod(kDelta) * 2 ~ (2pi)^3 (+ double-counted planes)
-    total = s.sum() * np.prod(list(p.kDelta)) * 2
-    assert total / (2 * math.pi) ** 3 == pytest.approx(1.0, rel=0.08)
+    # S(k) = exp(-|k|^2/(4 pi)) integrates to (2 pi)^d.  REF_COMPAT: the
+    # reference's table (spectra_RF.f90:196-205), whose half-space lattice sum
+    # counts the k_a = 0 planes twice (sign blocks) or whole (k_1 >= 0):
+    # above (2 pi)^3.  FIXED: trapezoid weights 2^-(zero components) bring
+    # the sum to (2 pi)^3 up to the kMax truncation (erfc(kMax/sqrt(4 pi))
+    # per axis, ~1 %).
+    for xr in (40.0, 12.0):
+        pr = O.make_plan(3, O.SHINOZUKA, O.REF_COMPAT, [xr] * 3)
+        pf = O.make_plan(3, O.SHINOZUKA, O.FIXED, [xr] * 3)
+        k = O.kpoints(pf)
+        sr, sf = O.skvec(pr, k), O.skvec(pf, k)
+        g = np.exp(-(k ** 2).sum(1) / (4 * O.PI_REF))
+        assert np.allclose(sr, g, rtol=1e-14)
+        zeros = (k == 0).sum(1)
+        assert np.array_equal(sf, sr * 0.5 ** zeros)
+        vol = np.prod(list(pf.kDelta)) * 2
+        trunc = math.erf(pf.kMax[0] / math.sqrt(4 * math.pi)) ** 3
+        assert sf.sum() * vol / (2 * math.pi) ** 3 == pytest.approx(trunc, rel=5e-3)
+        assert sr.sum() * vol / (2 * math.pi) ** 3 > 1.0 + 0.5 * pf.kDelta[0] / math.sqrt(4 * math.pi)
 
 
 def test_randomization_box_and_density():

@@ -57,6 +57,8 @@ struct rf_blend_plan {
     rf_context* ctx = nullptr;
     char* d_blob = nullptr;
     rf_blend_dev b{};
+    bool separable = false;
+    rf_blend_sep sep{};
     int dtype = RF_DTYPE_F64;
 };
 
@@ -198,6 +200,59 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
             axis_factors(groupN[a], std::min(nOvl[a], groupN[a]), true, true, mode, pu1[a][0].data(), pu1[a][1].data());
     }
 
+    // separable arrangement: the tapered faces of every subdomain along axis
+    // a are a function of its interval along a
+    bool separable = true;
+    std::vector<int> ivMask[3];
+    for (int a = 0; a < nDim; a++) ivMask[a].assign((size_t)nInt[a], -1);
+    for (int s = 0; s < nSub && separable; s++)
+        for (int a = 0; a < nDim; a++) {
+            const int m = (faceMasks[s] >> (2 * a)) & 3;
+            int& slot = ivMask[a][(size_t)subInt[(size_t)s * 3 + a]];
+            if (slot < 0) slot = m;
+            else if (slot != m) separable = false;
+        }
+    std::vector<rf_blend_axis> axT[3];
+    std::vector<rf_blend_sub> subT;
+    if (separable) {
+        for (int a = 0; a < 3; a++) {
+            if (a >= nDim) {
+                rf_blend_axis u{};
+                u.iv[0] = 0; u.iv[1] = -1; u.sz[0] = u.sz[1] = 1; u.f[0] = u.f[1] = 1.0; u.pu1 = 1.0;
+                axT[a].push_back(u);
+                continue;
+            }
+            axT[a].resize((size_t)groupN[a]);
+            for (int g = 0; g < groupN[a]; g++) {
+                rf_blend_axis& e = axT[a][(size_t)g];
+                e = rf_blend_axis{};
+                e.pu1 = pu1[a][0][(size_t)g] * pu1[a][1][(size_t)g];
+                for (int c = 0; c < 2; c++) {
+                    const int iv = cover[a][(size_t)g * 2 + c];
+                    e.iv[c] = iv;
+                    e.sz[c] = 1;
+                    e.f[c] = 0.0;
+                    if (iv < 0) continue;
+                    const int o = ints[a][(size_t)iv].first, z = ints[a][(size_t)iv].second;
+                    const int m = ivMask[a][(size_t)iv] < 0 ? 0 : ivMask[a][(size_t)iv];
+                    std::vector<double> lo((size_t)z), up((size_t)z);
+                    axis_factors(z, nOvl[a], m & 1, (m >> 1) & 1, mode, lo.data(), up.data());
+                    e.pos[c] = g - o;
+                    e.sz[c] = z;
+                    e.f[c] = lo[(size_t)(g - o)] * up[(size_t)(g - o)];
+                }
+            }
+        }
+        subT.assign((size_t)cells, rf_blend_sub{nullptr, 0});
+        for (int64_t c = 0; c < cells; c++) {
+            const int sb = subOf[(size_t)c];
+            if (sb < 0) continue;
+            int64_t t = 1;
+            for (int a = 0; a < nDim; a++) t *= sizes[sb * nDim + a];
+            subT[(size_t)c] = rf_blend_sub{d_fields[sb], t};
+        }
+    }
+
     // one device block for every table
     std::vector<char> blob;
     auto put = [&](const void* src, size_t bytes) {
@@ -216,6 +271,11 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
     const size_t oEnt = put(entries.data(), entries.size() * sizeof(rf_blend_entry));
     const size_t oLow = put(lowF.data(), lowF.size() * 8);
     const size_t oUp = put(upF.data(), upF.size() * 8);
+    size_t oAx[3] = {0, 0, 0}, oSubT = 0;
+    if (separable) {
+        for (int a = 0; a < 3; a++) oAx[a] = put(axT[a].data(), axT[a].size() * sizeof(rf_blend_axis));
+        oSubT = put(subT.data(), subT.size() * sizeof(rf_blend_sub));
+    }
 
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<rf_blend_plan> P(new (std::nothrow) rf_blend_plan());
@@ -246,6 +306,18 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
     b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
     b.upF = reinterpret_cast<const double*>(d_blob + oUp);
     b.maxSize = maxSize;
+    if (separable) {
+        P->separable = true;
+        rf_blend_sep& q = P->sep;
+        q.nDim = nDim; q.Nmc = Nmc; q.mode = mode;
+        for (int a = 0; a < 3; a++) {
+            q.groupN[a] = b.groupN[a];
+            q.nInt[a] = nInt[a];
+            q.ax[a] = reinterpret_cast<const rf_blend_axis*>(d_blob + oAx[a]);
+        }
+        q.groupTot = gTot;
+        q.sub = reinterpret_cast<const rf_blend_sub*>(d_blob + oSubT);
+    }
     *out = P.release();
     return RF_OK;
 }
@@ -258,7 +330,8 @@ extern "C" int rf_blend_plan_execute(rf_blend_plan* P, void* d_group, void* stre
     RF_HIP_TRY(ctx, hipGetDevice(&dev));
     if (dev != ctx->device) RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
+    if (P->separable) RF_HIP_TRY(ctx, rf_launch_blend_sep(P->sep, d_group, P->dtype, s));
+    else RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
     return RF_OK;
 }
 

@@ -37,6 +37,34 @@ struct rf_blend_dev {
 
 constexpr int kBlendMaxEntries = 8;
 
+// Separable arrangement (every subdomain's tapered faces along axis a depend
+// only on its interval along a, as in a tensor-product layout): the PU weight
+// is a product of per-axis factors, so one table per axis, indexed by the
+// group index, resolves both covering intervals, the positions inside them,
+// their sizes and factors; a point needs one table load per axis and one
+// subdomain lookup per contribution.
+struct rf_blend_axis {
+    int32_t iv[2];                // covering intervals (-1: none)
+    int32_t pos[2];               // position of the group index inside each
+    int32_t sz[2];                // interval lengths
+    int32_t pad[2];
+    double f[2];                  // taper factor (lower * upper face) at pos
+    double pu1;                   // REF_COMPAT reverse taper of the group grid
+};
+struct rf_blend_sub {
+    const void* field;            // NULL: no subdomain on this interval triple
+    int64_t subTot;
+};
+struct rf_blend_sep {
+    int nDim, Nmc, mode;
+    int groupN[3];
+    int nInt[3];
+    int64_t groupTot;
+    const rf_blend_axis* ax[3];   // [groupN[a]] (one unit entry for a >= nDim)
+    const rf_blend_sub* sub;      // [i0 + nInt0 * (i1 + nInt1 * i2)]
+};
+hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s);
+
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s);
 int rf_moments_parts(int64_t N);
 int rf_corr_parts(int64_t N);

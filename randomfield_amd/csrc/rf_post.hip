@@ -304,6 +304,72 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
     }
 }
 
+// Separable form: one thread per group point, all realizations; the <= 2^d
+// contributions come from the per-axis tables, the realization loop issues
+// 8 independent loads per contribution.  FP64 sums, each term v * PU
+// then + (localization_RF.f90:228,253); the PU product is taken per axis
+// (a reordering of the reference's lower-then-upper face product, equal up
+// to rounding) and contributions are added in interval order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
+#pragma clang fp contract(off)
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= b.groupTot) return;
+    int gi0, gi1, gi2;
+    if (b.groupTot <= 0x7fffffffLL) {
+        const uint32_t u = (uint32_t)g, n0 = (uint32_t)b.groupN[0], n1 = (uint32_t)b.groupN[1];
+        const uint32_t t = u / n0;
+        gi0 = (int)(u - t * n0);
+        gi2 = (int)(t / n1);
+        gi1 = (int)(t - (uint32_t)gi2 * n1);
+    } else {
+        const int64_t t = g / b.groupN[0];
+        gi0 = (int)(g - t * b.groupN[0]);
+        gi2 = (int)(t / b.groupN[1]);
+        gi1 = (int)(t - (int64_t)gi2 * b.groupN[1]);
+    }
+    const rf_blend_axis A0 = b.ax[0][gi0];
+    const rf_blend_axis A1 = b.ax[1][gi1];
+    const rf_blend_axis A2 = b.ax[2][gi2];
+    const double pu1 = (A0.pu1 * A1.pu1) * A2.pu1;
+    constexpr int RB = 8;                 // realizations per pass
+    for (int r0 = 0; r0 < b.Nmc; r0 += RB) {
+        double acc[RB];
+#pragma unroll
+        for (int j = 0; j < RB; j++) acc[j] = 0.0;
+        // contributions in interval order; a combination is recomputed per
+        // pass (a few cached loads) to keep the register count low
+#pragma unroll 1
+        for (int k = 0; k < 8; k++) {
+            // selects, not indexing: the tables stay in registers
+            const bool c0 = k & 1, c1 = (k >> 1) & 1, c2 = k >> 2;
+            const int i0 = c0 ? A0.iv[1] : A0.iv[0];
+            const int i1 = c1 ? A1.iv[1] : A1.iv[0];
+            const int i2 = c2 ? A2.iv[1] : A2.iv[0];
+            if (i0 < 0 || i1 < 0 || i2 < 0) continue;
+            const rf_blend_sub S = b.sub[i0 + b.nInt[0] * (i1 + b.nInt[1] * i2)];
+            if (!S.field) continue;
+            const int64_t s0 = c0 ? A0.sz[1] : A0.sz[0], s1 = c1 ? A1.sz[1] : A1.sz[0];
+            const int64_t p0 = c0 ? A0.pos[1] : A0.pos[0], p1 = c1 ? A1.pos[1] : A1.pos[0];
+            const int64_t p2 = c2 ? A2.pos[1] : A2.pos[0];
+            const T* F = static_cast<const T*>(S.field) + p0 + p1 * s0 + p2 * s0 * s1;
+            const double w = ((c0 ? A0.f[1] : A0.f[0]) * (c1 ? A1.f[1] : A1.f[0])) * (c2 ? A2.f[1] : A2.f[0]);
+            T v[RB];
+#pragma unroll
+            for (int j = 0; j < RB; j++) v[j] = (r0 + j < b.Nmc) ? F[(int64_t)(r0 + j) * S.subTot] : T(0);
+#pragma unroll
+            for (int j = 0; j < RB; j++) acc[j] = (double)v[j] * w + acc[j];
+        }
+#pragma unroll
+        for (int j = 0; j < RB; j++)
+            if (r0 + j < b.Nmc) {
+                double u = acc[j];
+                if (b.mode == RF_REF_COMPAT && r0 + j == 0) u = u / pu1;   // calls_RF.f90:355
+                group[(int64_t)(r0 + j) * b.groupTot + g] = (T)u;
+            }
+    }
+}
+
 }  // namespace
 
 hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, int64_t count,
@@ -433,5 +499,14 @@ hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const double* xMin, 
         hipLaunchKernelGGL((k_interp<float, double>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<double*>(out), d_flag);
     else
         hipLaunchKernelGGL((k_interp<float, float>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<float*>(out), d_flag);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s) {
+    const unsigned grid = (unsigned)((b.groupTot + kBlock - 1) / kBlock);
+    if (dtype == RF_DTYPE_F64)
+        hipLaunchKernelGGL(k_blend_sep<double>, dim3(grid), dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
+    else
+        hipLaunchKernelGGL(k_blend_sep<float>, dim3(grid), dim3(kBlock), 0, s, b, static_cast<float*>(d_group));
     return hipGetLastError();
 }

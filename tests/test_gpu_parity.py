@@ -250,7 +250,11 @@ def test_device_uniforms_match_host(ctx):
     assert np.array_equal(d.cpu().numpy(), host)
 
 
-def test_blend_matches_oracle(ctx):
+@pytest.mark.parametrize("separable", [True, False])
+def test_blend_matches_oracle(ctx, separable):
+    """Both blend kernels: the separable one (masks a function of the
+    intervals, as every tensor layout has) and the general cell-list one
+    (one subdomain's mask changed so the per-axis form does not apply)."""
     import torch
     groupN = (40, 33, 21)
     origins, sizes, masks = W.cfg4_layout(groupN, (3, 2, 2), 5)
@@ -258,7 +262,10 @@ def test_blend_matches_oracle(ctx):
     Nmc = 3
     fields = [rng.standard_normal((Nmc, int(np.prod(s)))) for s in sizes]
     for mode in (R.REF_COMPAT, R.FIXED):
-        fm = np.full(len(sizes), 63, np.int32) if mode == R.REF_COMPAT else masks
+        fm = np.full(len(sizes), 63, np.int32) if mode == R.REF_COMPAT else np.array(masks, np.int32)
+        if not separable:
+            fm = fm.copy()
+            fm[0] ^= 0b10          # upper x face of subdomain 0 only
         ref = O.blend(groupN, Nmc, origins, sizes, [5, 5, 5], fm, mode, fields)
         dfs = [torch.from_numpy(f).cuda() for f in fields]
         g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float64, device="cuda")

@@ -52,8 +52,9 @@ def test_generated_field_moments_and_correlation(ctx):
     plan.close()
     # statistics of the discrete sum on this lattice
     po = O.make_plan(3, O.SHINOZUKA, O.FIXED, ext)
-    k = O.kpoints(po)
-    S = O.skvec(po, k)
+    kt = O.kpoints(po)                                 # (N_k, 3)
+    S = O.skvec(po, kt)
+    k = kt.T
     w = po.ampMult ** 2 / 2.0 * S                     # variance of each term
     var_t = float(w.sum())
     box = np.ones(k.shape[1])

@@ -205,7 +205,7 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
             for (int m = 0; m < MT; m += 2) {
                 const V ah0 = __builtin_bit_cast(V, c[(m * 2 + 0) * 64 + lane]);
                 const V al0 = __builtin_bit_cast(V, c[(m * 2 + 1) * 64 + lane]);
-                if (MT > 1) {
+                if (m + 1 < MT) {
                     const V ah1 = __builtin_bit_cast(V, c[(m * 2 + 2) * 64 + lane]);
                     const V al1 = __builtin_bit_cast(V, c[(m * 2 + 3) * 64 + lane]);
                     acc[m] = mfma(ah0, bh, acc[m]);
@@ -299,6 +299,7 @@ void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
     switch (mt) {
         case 1: sum_mfma_block<V, 3, 1, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
         case 2: sum_mfma_block<V, 3, 2, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        case 3: sum_mfma_block<V, 3, 3, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
         default: sum_mfma_block<V, 3, kGroupMT, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
     }
 }

@@ -101,7 +101,7 @@ struct rf_server {
         q.bytesOut = align256((size_t)N * Nmc * sizeof(float));
         q.bytesX = q.r.xOnDevice ? 0 : align256((size_t)N * 3 * sizeof(double));
         if (q.r.method == RF_SHINOZUKA) {
-            q.Rpad = rf_mfma_npad(Nmc);
+            q.Rpad = (Nmc + 31) / 32 * 32;       // whole 32-row tiles (any count)
             q.bytesScr = 2 * align256((size_t)Nmc * kN * sizeof(double)) +   // uniforms, phases
                          align256(rf_mfma_bbytes(cfg.form, l.kNpad, q.Rpad));
         } else {

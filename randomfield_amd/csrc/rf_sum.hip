@@ -104,7 +104,9 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double
 // FP32 form of the isotropic sum with per-realization directions: the
 // argument cannot be shared across realizations, so each term forms its own
 // FP64 k.x.  kT[j][r][D] in revolutions, aT/pT[j][r] as in k_sum_fp32.
-template <int D, int R>
+// Guard: skip the padded realizations of the chunk (worth it when requests
+// have few realizations; the plan kernel keeps the unguarded schedule).
+template <int D, int R, bool Guard>
 __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, const double* __restrict__ kT,
                                                    const float* __restrict__ pT,
                                                    const float* __restrict__ aT, int Rpad, int64_t i,
@@ -116,13 +118,14 @@ __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, const d
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
     const int nr = a.Nmc - r0;               // realizations of this chunk (wave-uniform)
+    (void)nr;
     for (int64_t j = 0; j < a.kN; j++) {
         const double* kj = kT + (j * Rpad + r0) * D;
         const float* pj = pT + j * Rpad + r0;
         const float* aj = aT + j * Rpad + r0;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            if (r >= nr) break;                  // padded realizations stay zero
+            if (Guard && r >= nr) break;         // padded realizations stay zero
             double t = kj[r * D] * xn[0];
 #pragma unroll
             for (int d = 1; d < D; d++) t = fma(kj[r * D + d], xn[d], t);
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const do
                                                          const float* __restrict__ aT, int Rpad) {
     const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.i1) return;
-    sum_fp32_iso_point<D, R>(a, kT, pT, aT, Rpad, i, blockIdx.y * R);
+    sum_fp32_iso_point<D, R, false>(a, kT, pT, aT, Rpad, i, blockIdx.y * R);
 }
 
 // Grouped form: block b runs one tile = (field, block of kGroupIsoPts points,
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_group_iso(const rf_group_req* __rest
     const int64_t local = b - q.tile0;
     const int64_t i = (local / nChunk) * kGroupIsoPts + threadIdx.x;
     if (i >= q.a.N) return;
-    sum_fp32_iso_point<3, kGroupIsoR>(q.a, static_cast<const double*>(q.ops0), static_cast<const float*>(q.ops1),
+    sum_fp32_iso_point<3, kGroupIsoR, true>(q.a, static_cast<const double*>(q.ops0), static_cast<const float*>(q.ops1),
                                       static_cast<const float*>(q.ops2), q.Rpad, i,
                                       (int)(local % nChunk) * kGroupIsoR);
 }

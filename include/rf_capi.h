@@ -86,7 +86,7 @@ extern "C" {
 #define RF_DTYPE_F64      0
 #define RF_DTYPE_F32      1
 /* launch shapes of the MFMA forms (rf_plan_set_variant) */
-#define RF_MFMA_VARIANTS  7
+#define RF_MFMA_VARIANTS  9
 
 typedef struct rf_params {
     int32_t nDim;        /* 1..3 (isotropic: 2..3)                            */
@@ -160,7 +160,8 @@ int rf_plan_execute(rf_field_plan* plan, const double* d_x, int64_t N,
  * sum has one code path): variant 0 .. RF_MFMA_VARIANTS-1 = {waves per
  * block, K-steps per LDS stage, MFMA issue order} = {4,1,pairs}, {4,2,pairs},
  * {8,1,pairs}, {8,2,pairs}, {8,4,pairs}, {8,2,product-major},
- * {8,4,product-major}.  Results are identical across variants (same
+ * {8,4,product-major}, {8,2,pairs, wavevectors staged in LDS},
+ * {8,4,pairs, wavevectors in LDS}.  Results are identical across variants (same
  * per-point operation order); only the speed differs.  rf_plan_autotune
  * times every variant on the caller's points (each runs three times, the
  * last run's result stays in d_out), keeps the fastest in the plan and

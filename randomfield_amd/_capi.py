@@ -20,7 +20,7 @@ REF_COMPAT, FIXED = 0, 1
 FORM_FP64, FORM_FP32, FORM_MFMA_F16, FORM_MFMA_BF16 = 0, 1, 2, 3
 KSAMP_BOX, KSAMP_DENSITY = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
-MFMA_VARIANTS = 7
+MFMA_VARIANTS = 9
 
 # every entry point the header declares
 EXPORTS = (

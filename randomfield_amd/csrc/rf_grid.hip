@@ -11,23 +11,16 @@
 //
 // three small batched complex GEMMs instead of N * N_k cosines
 // (O(n N_k + n^2 N_k^(2/3) + n^3 N_k^(1/3)) per realization and block).
-// One tiled FP32 kernel does every stage: D[m][n] = sum_k A[m][k] B[k][n]
+// One tiled FP32 MFMA kernel does every stage: D[m][n] = sum_k A[m][k] B[k][n]
 // with per-stage strides; A may come in per-block variants (conjugated
 // tables of the negative-sign blocks).  The phase tables are formed in FP64
-// and rounded to FP32; accumulation is FP32 over <= 4*N_k^(1/3) terms.
+// and rounded to FP32; products and accumulation are FP32 over
+// <= 4*N_k^(1/3) terms.
 #include "rf_internal.h"
 
 namespace {
 
 constexpr int kTM = 64, kTN = 64, kTK = 16;   // block tile; 256 threads, 4x4 each
-
-__device__ inline float2 cmul_add(float2 a, float2 b, float2 c) {
-    c.x = fmaf(a.x, b.x, c.x);
-    c.x = fmaf(-a.y, b.y, c.x);
-    c.y = fmaf(a.x, b.y, c.y);
-    c.y = fmaf(a.y, b.x, c.y);
-    return c;
-}
 
 __device__ inline float epi32(float u, const rf_epilogue& e) {
     if (e.margi == RF_FOM_NONE) return u;
@@ -35,22 +28,33 @@ __device__ inline float epi32(float u, const rf_epilogue& e) {
     return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
 }
 
-// One 64 x 64 output tile (rows m0.., columns n0.., batch) of
-// D = A B; RealOnly forms only Re(D) (2 FMAs per complex product instead of
-// 4), as the last stage, whose output is the real field, needs.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// One 64 x 64 output tile (complex rows m0.., columns n0.., batch) of
+// D = A B on the matrix cores in FP32 (v_mfma_f32_32x32x2_f32): the complex
+// product is the real GEMM
+//   [Re D; Im D] = [[Re A, -Im A], [Im A, Re A]] [Re B; Im B]
+// with the real rows interleaved (2m: Re, 2m+1: Im), one complex k per
+// instruction (lanes 0-31 take the k' = 2k column, 32-63 the k' = 2k+1
+// column).  RealOnly forms Re D only: [Re A, -Im A] [Re B; Im B].  Operand
+// tiles of 16 complex k are staged in LDS as FP32 complex, as loaded.
+// Waves: complex output 128 real rows x 64 columns, wave w owns real rows
+// 32w.. and both 32-column halves; RealOnly 64 x 64, wave w one 32 x 32 tile.
 template <bool RealOnly>
 __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0, int batch,
                                           float2 (*sA)[kTM + 1], float2 (*sB)[kTN + 1]) {
     const int tid = threadIdx.x;
-    const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
     const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB;
     const float2* B = g.B + (int64_t)batch * g.bStrideB;
-    float2 acc[4][4];
+    f32x16 acc0, acc1;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = make_float2(0.f, 0.f);
-
+    for (int i = 0; i < 16; i++) {
+        acc0[i] = 0.0f;
+        acc1[i] = 0.0f;
+    }
+    // RealOnly tile of this wave
+    const int rt = wave >> 1, ct = wave & 1;
     for (int k0 = 0; k0 < g.K; k0 += kTK) {
         // stage A (kTM x kTK) and B (kTK x kTN), zero outside the matrices
         for (int e = tid; e < kTM * kTK; e += 256) {
@@ -66,41 +70,46 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
         __syncthreads();
 #pragma unroll 4
         for (int kk = 0; kk < kTK; kk++) {
-            float2 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = sA[kk][tm + i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = sB[kk][tn + j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (RealOnly) {
-                        acc[i][j].x = fmaf(a[i].x, b[j].x, acc[i][j].x);
-                        acc[i][j].x = fmaf(-a[i].y, b[j].y, acc[i][j].x);
-                    } else {
-                        acc[i][j] = cmul_add(a[i], b[j], acc[i][j]);
-                    }
-                }
+            if (!RealOnly) {
+                const float2 a = sA[kk][wave * 16 + (l32 >> 1)];
+                // row part (l32 & 1): 0 -> Re row (Re A, -Im A), 1 -> Im row (Im A, Re A)
+                const float av = (l32 & 1) ? (half ? a.x : a.y) : (half ? -a.y : a.x);
+                const float2 b0 = sB[kk][l32], b1 = sB[kk][32 + l32];
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b0.y : b0.x, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b1.y : b1.x, acc1, 0, 0, 0);
+            } else {
+                const float2 a = sA[kk][rt * 32 + l32];
+                const float2 b = sB[kk][ct * 32 + l32];
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(half ? -a.y : a.x, half ? b.y : b.x, acc0, 0, 0, 0);
+            }
         }
         __syncthreads();
     }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    if (!RealOnly) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int m = m0 + tm + i;
-        if (m >= g.M) continue;
+        for (int i = 0; i < 16; i += 2) {
+            const int r = 8 * (i >> 2) + 4 * half + (i & 3);       // even: the Re row
+            const int m = m0 + wave * 16 + (r >> 1);
+            if (m >= g.M) continue;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int n = n0 + tn + j;
-            if (n >= g.N) continue;
-            const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
-            if (!RealOnly) {
-                static_cast<float2*>(g.D)[o] = acc[i][j];
-            } else {
-                const float u = epi32(acc[i][j].x, g.epi);
-                if (g.outDtype == RF_DTYPE_F64) static_cast<double*>(g.D)[o] = (double)u;
-                else static_cast<float*>(g.D)[o] = u;
+            for (int c = 0; c < 2; c++) {
+                const int n = n0 + c * 32 + l32;
+                if (n >= g.N) continue;
+                const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
+                static_cast<float2*>(g.D)[o] = c ? make_float2(acc1[i], acc1[i + 1]) : make_float2(acc0[i], acc0[i + 1]);
             }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int m = m0 + rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+            const int n = n0 + ct * 32 + l32;
+            if (m >= g.M || n >= g.N) continue;
+            const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
+            const float u = epi32(acc0[i], g.epi);
+            if (g.outDtype == RF_DTYPE_F64) static_cast<double*>(g.D)[o] = (double)u;
+            else static_cast<float*>(g.D)[o] = u;
         }
     }
 }

@@ -40,9 +40,10 @@ constexpr int kKPad = 4 * kStepK;       // K padding: whole stages of up to 4 st
 // launch variants: waves per workgroup (points per block = 32 * waves),
 // K-steps per LDS stage (one barrier per stage) and MFMA issue order (0:
 // tiles in pairs, the three products of a pair interleaved; 1: one product
-// over all tiles, then the next)
+// over all tiles, then the next; 2: as 0 with the wavevectors read from LDS)
 struct Variant { int waves, steps, order; };
-constexpr Variant kVariants[] = {{4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {8, 2, 0}, {8, 4, 0}, {8, 2, 1}, {8, 4, 1}};
+constexpr Variant kVariants[] = {{4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {8, 2, 0}, {8, 4, 0}, {8, 2, 1}, {8, 4, 1},
+                                 {8, 2, 2}, {8, 4, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static_assert(kNumVariants == RF_MFMA_VARIANTS, "variant count of the C ABI");
 
@@ -120,6 +121,27 @@ __device__ inline void basis(const double* __restrict__ k, int h, const double* 
     }
 }
 
+// The same basis from the lane's own half of a step's wavevectors, read from
+// LDS (kslot: the step's 24 doubles; a lane in half h takes doubles 12h ..
+// 12h+11, a broadcast read): one dot product per wavevector and no select.
+template <typename V, int D>
+__device__ inline void basis_lds(const u32x4* kslot, int h, const double* xn, V* bh, V* bl) {
+    d2 k2[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) k2[i] = __builtin_bit_cast(d2, kslot[6 * h + i]);
+    const double* k = reinterpret_cast<const double*>(k2);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double s = k[3 * q] * xn[0];
+        if (D > 1) s = fma(k[3 * q + 1], xn[1], s);
+        if (D > 2) s = fma(k[3 * q + 2], xn[2], s);
+        const float f = (float)__builtin_amdgcn_fract(s);
+        const float c = __builtin_amdgcn_cosf(f);
+        const float sn = __builtin_amdgcn_sinf(f);
+        split_pair<V>(c, sn, q, bh, bl);
+    }
+}
+
 // Panels in HBM (16-byte units): [t][mtile][hilo][lane]; LDS double buffer of
 // S K-steps x MT x 2 x 64 units.  Wavevectors: separate stream [t][8][3]
 // doubles read with scalar loads, so the LDS serves only the coefficient
@@ -129,11 +151,15 @@ __device__ inline void basis(const double* __restrict__ k, int h, const double* 
 // M-tiles mt0 .. mt0 + MT - 1.  ldsA and ldsB are two distinct __shared__
 // objects of S * MT * 128 units or more (distinct, so the waitcnt pass can
 // tell the DMA target of the next stage from the buffer being read).
-template <typename V, int D, int MT, int W, int S, int ORD>
+// KS = 0: each step's wavevectors come through scalar loads; KS = 1: every
+// wave copies a stage's wavevectors into its own LDS slot (kqA/kqB, W x 12S
+// units each) with the stage's DMA and reads its lane half from there.
+template <typename V, int D, int MT, int W, int S, int ORD, int KS>
 __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSteps, int mtTotal,
                                                const u32x4* __restrict__ panels,
                                                const u32x4* __restrict__ kq, float outScale,
-                                               int64_t pt0, int mt0, u32x4* ldsA, u32x4* ldsB) {
+                                               int64_t pt0, int mt0, u32x4* ldsA, u32x4* ldsB,
+                                               u32x4* kqA, u32x4* kqB) {
     constexpr int kStepUnits = MT * 2 * 64;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -160,6 +186,7 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
     // not drain the DMA)
     constexpr int kChunks = 2 * MT * S;
     constexpr int kDmaPerWave = (kChunks + W - 1) / W;
+    static_assert(kDmaPerWave < 16, "vmcnt field of the wavevector wait");
     auto dma_stage = [&](int64_t u, u32x4* dst) {
 #pragma unroll
         for (int i = 0; i < kDmaPerWave; i++) {
@@ -173,20 +200,41 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
         }
     };
 
+    // this wave's copy of stage u's wavevectors (12 units per step)
+    auto dma_kq = [&](int64_t u, u32x4* dst) {
+        if (KS == 1 && lane < 12 * S)
+            __builtin_amdgcn_global_load_lds((const void*)(kq + u * S * 12 + lane),
+                                             (lds_void*)&dst[wave * 12 * S], 16, 0, 0);
+    };
+
     V bh, bl;
     const double* kqd = reinterpret_cast<const double*>(kq);
     // one stage: prefetch stage u+1 into nxt; per step, form the basis of the
     // next step while the MFMAs of this step read cur
-    auto stage = [&](int64_t u, const u32x4* cur, u32x4* nxt) {
+    auto stage = [&](int64_t u, const u32x4* cur, u32x4* nxt, const u32x4* kcur, u32x4* knxt) {
         // unconditional (the last stage re-reads itself into the unused
-        // buffer), so every path leaves the same number of loads in flight
-        dma_stage(u + 1 < nStages ? u + 1 : u, nxt);
+        // buffer), so every path leaves the same number of loads in flight;
+        // the wavevector copy goes first so a wait can single it out
+        const int64_t un = u + 1 < nStages ? u + 1 : u;
+        dma_kq(un, knxt);
+        dma_stage(un, nxt);
 #pragma unroll
         for (int sI = 0; sI < S; sI++) {
             const int64_t t = u * S + sI;
             const u32x4* c = cur + sI * kStepUnits;
             V nbh, nbl;
-            basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
+            if (KS == 1) {
+                if (sI + 1 < S) {
+                    basis_lds<V, D>(kcur + wave * 12 * S + (sI + 1) * 12, h, xn, &nbh, &nbl);
+                } else {
+                    // next stage's first step: this wave's own copy, issued
+                    // before the kDmaPerWave panel loads of this stage
+                    __builtin_amdgcn_s_waitcnt(0x0F70 | kDmaPerWave);
+                    basis_lds<V, D>(knxt + wave * 12 * S, h, xn, &nbh, &nbl);
+                }
+            } else {
+                basis<V, D>(kqd + (t + 1 < nSteps ? t + 1 : t) * 24, h, xn, &nbh, &nbl);
+            }
             if (ORD == 1) {
                 V ah[MT], al[MT];
 #pragma unroll
@@ -229,13 +277,15 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
         __syncthreads();   // stage u+1 landed in LDS; stage u's buffer is free
     };
 
+    dma_kq(0, kqA);
     dma_stage(0, ldsA);
-    basis<V, D>(kqd, h, xn, &bh, &bl);
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (KS == 1) basis_lds<V, D>(kqA + wave * 12 * S, h, xn, &bh, &bl);
+    else basis<V, D>(kqd, h, xn, &bh, &bl);
     __syncthreads();
     for (int64_t u = 0; u < nStages; u += 2) {
-        stage(u, ldsA, ldsB);
-        if (u + 1 < nStages) stage(u + 1, ldsB, ldsA);
+        stage(u, ldsA, ldsB, kqA, kqB);
+        if (u + 1 < nStages) stage(u + 1, ldsB, ldsA, kqB, kqA);
     }
     if (pt >= a.i1) return;
     // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
@@ -256,14 +306,17 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
 }
 
 
-template <typename V, int D, int MT, int W, int S, int ORD>
+template <typename V, int D, int MT, int W, int S, int ORD, int KS>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_sum_mfma(rf_sum_args a, int64_t nSteps, int mtTotal, const u32x4* __restrict__ panels,
                 const u32x4* __restrict__ kq, float outScale) {
     __shared__ u32x4 ldsA[S * MT * 128];
     __shared__ u32x4 ldsB[S * MT * 128];
-    sum_mfma_block<V, D, MT, W, S, ORD>(a, nSteps, mtTotal, panels, kq, outScale,
-                                   a.i0 + (int64_t)blockIdx.x * (32 * W), blockIdx.y * kMaxMT, ldsA, ldsB);
+    __shared__ u32x4 kqA[KS ? W * 12 * S : 1];
+    __shared__ u32x4 kqB[KS ? W * 12 * S : 1];
+    sum_mfma_block<V, D, MT, W, S, ORD, KS>(a, nSteps, mtTotal, panels, kq, outScale,
+                                            a.i0 + (int64_t)blockIdx.x * (32 * W), blockIdx.y * kMaxMT, ldsA,
+                                            ldsB, kqA, kqB);
 }
 
 // Grouped form for a batch of independent fields (requests of a serving
@@ -297,10 +350,10 @@ void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
     const u32x4* P = static_cast<const u32x4*>(q.ops0);
     const u32x4* K = static_cast<const u32x4*>(q.ops1);
     switch (mt) {
-        case 1: sum_mfma_block<V, 3, 1, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
-        case 2: sum_mfma_block<V, 3, 2, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
-        case 3: sum_mfma_block<V, 3, 3, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
-        default: sum_mfma_block<V, 3, kGroupMT, W, S, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB); break;
+        case 1: sum_mfma_block<V, 3, 1, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
+        case 2: sum_mfma_block<V, 3, 2, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
+        case 3: sum_mfma_block<V, 3, 3, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
+        default: sum_mfma_block<V, 3, kGroupMT, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
     }
 }
 
@@ -346,16 +399,17 @@ __global__ void k_mfma_prep(int nDim, int64_t kN, int64_t nSteps, int Nmc, int m
 template <typename V, int D, int W, int S, int ORD>
 hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, const void* B,
                      float outScale, hipStream_t s) {
+    constexpr int O = ORD == 2 ? 0 : ORD, KS = ORD == 2 ? 1 : 0;
     const u32x4* P = static_cast<const u32x4*>(B);
     const u32x4* K = P + nSteps * mtTotal * 128;
     const int64_t n = a.i1 - a.i0;
     const dim3 grid((unsigned)((n + 32 * W - 1) / (32 * W)), (unsigned)((mtTotal + kMaxMT - 1) / kMaxMT));
     const dim3 block(64 * W);
     switch (mt) {
-        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
-        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8, W, S, ORD>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 1: hipLaunchKernelGGL((k_sum_mfma<V, D, 1, W, S, O, KS>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 2: hipLaunchKernelGGL((k_sum_mfma<V, D, 2, W, S, O, KS>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        case 4: hipLaunchKernelGGL((k_sum_mfma<V, D, 4, W, S, O, KS>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
+        default: hipLaunchKernelGGL((k_sum_mfma<V, D, 8, W, S, O, KS>), grid, block, 0, s, a, nSteps, mtTotal, P, K, outScale); break;
     }
     return hipGetLastError();
 }
@@ -363,7 +417,7 @@ hipError_t launch_mt(int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal, 
 template <typename V, int D>
 hipError_t launch_variant(int variant, int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal,
                           const void* B, float outScale, hipStream_t s) {
-    static_assert(kNumVariants == 7, "variant table and dispatch agree");
+    static_assert(kNumVariants == 9, "variant table and dispatch agree");
     switch (variant) {
         case 1: return launch_mt<V, D, 4, 2, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
         case 2: return launch_mt<V, D, 8, 1, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
@@ -371,6 +425,8 @@ hipError_t launch_variant(int variant, int mt, const rf_sum_args& a, int64_t nSt
         case 4: return launch_mt<V, D, 8, 4, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
         case 5: return launch_mt<V, D, 8, 2, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
         case 6: return launch_mt<V, D, 8, 4, 1>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 7: return launch_mt<V, D, 8, 2, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
+        case 8: return launch_mt<V, D, 8, 4, 2>(mt, a, nSteps, mtTotal, B, outScale, s);
         default: return launch_mt<V, D, 4, 1, 0>(mt, a, nSteps, mtTotal, B, outScale, s);
     }
 }

@@ -386,7 +386,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
-    ap.add_argument("--rate", type=float, default=320.0, help="cfg 5: arrivals per second")
+    ap.add_argument("--rate", type=float, default=600.0, help="cfg 5: arrivals per second")
     ap.add_argument("--requests", type=int, default=2000, help="cfg 5: requests in the stream")
     ap.add_argument("--budget-gib", type=int, default=64, help="cfg 5: server memory budget")
     ap.add_argument("--config", type=int, default=3,

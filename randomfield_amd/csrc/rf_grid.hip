@@ -55,29 +55,63 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
     }
     // RealOnly tile of this wave
     const int rt = wave >> 1, ct = wave & 1;
+    // operand chunks of kTK complex k: each thread fetches kPer elements of
+    // A and of B into registers one chunk ahead (zero outside the matrices)
+    constexpr int kPer = kTM * kTK / 256;
+    static_assert(kTM * kTK == kTK * kTN && kPer * 256 == kTM * kTK, "staging split");
+    float2 ra[kPer], rb[kPer];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int e = tid + i * 256;
+            const int mm = e / kTK, ka = e % kTK;
+            const int m = m0 + mm, k = k0 + ka;
+            ra[i] = (m < g.M && k < g.K) ? A[m * g.aSm + k * g.aSk] : make_float2(0.f, 0.f);
+            const int kb = e / kTN, nn = e % kTN;
+            const int n = n0 + nn, k2 = k0 + kb;
+            rb[i] = (n < g.N && k2 < g.K) ? B[k2 * g.bSk + n * g.bSn] : make_float2(0.f, 0.f);
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int e = tid + i * 256;
+            sA[e % kTK][e / kTK] = ra[i];
+            sB[e / kTN][e % kTN] = rb[i];
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < g.K; k0 += kTK) {
-        // stage A (kTM x kTK) and B (kTK x kTN), zero outside the matrices
-        for (int e = tid; e < kTM * kTK; e += 256) {
-            const int mm = e / kTK, kk = e % kTK;
-            const int m = m0 + mm, k = k0 + kk;
-            sA[kk][mm] = (m < g.M && k < g.K) ? A[m * g.aSm + k * g.aSk] : make_float2(0.f, 0.f);
-        }
-        for (int e = tid; e < kTK * kTN; e += 256) {
-            const int kk = e / kTN, nn = e % kTN;
-            const int n = n0 + nn, k = k0 + kk;
-            sB[kk][nn] = (n < g.N && k < g.K) ? B[k * g.bSk + n * g.bSn] : make_float2(0.f, 0.f);
-        }
+        stash();
         __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        // 32 x 32 sub-tiles entirely outside D are skipped (wave-uniform), so
+        // edge tiles of the ~130-point grid axes cost only their valid part
+        if (!RealOnly) {
+            if (m0 + wave * 16 < g.M) {
+                if (n0 + 32 < g.N) {
 #pragma unroll 4
-        for (int kk = 0; kk < kTK; kk++) {
-            if (!RealOnly) {
-                const float2 a = sA[kk][wave * 16 + (l32 >> 1)];
-                // row part (l32 & 1): 0 -> Re row (Re A, -Im A), 1 -> Im row (Im A, Re A)
-                const float av = (l32 & 1) ? (half ? a.x : a.y) : (half ? -a.y : a.x);
-                const float2 b0 = sB[kk][l32], b1 = sB[kk][32 + l32];
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b0.y : b0.x, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b1.y : b1.x, acc1, 0, 0, 0);
-            } else {
+                    for (int kk = 0; kk < kTK; kk++) {
+                        const float2 a = sA[kk][wave * 16 + (l32 >> 1)];
+                        // row part (l32 & 1): 0 -> Re row (Re A, -Im A), 1 -> Im row (Im A, Re A)
+                        const float av = (l32 & 1) ? (half ? a.x : a.y) : (half ? -a.y : a.x);
+                        const float2 b0 = sB[kk][l32], b1 = sB[kk][32 + l32];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b0.y : b0.x, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b1.y : b1.x, acc1, 0, 0, 0);
+                    }
+                } else {
+#pragma unroll 4
+                    for (int kk = 0; kk < kTK; kk++) {
+                        const float2 a = sA[kk][wave * 16 + (l32 >> 1)];
+                        const float av = (l32 & 1) ? (half ? a.x : a.y) : (half ? -a.y : a.x);
+                        const float2 b0 = sB[kk][l32];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, half ? b0.y : b0.x, acc0, 0, 0, 0);
+                    }
+                }
+            }
+        } else if (m0 + rt * 32 < g.M && n0 + ct * 32 < g.N) {
+#pragma unroll 4
+            for (int kk = 0; kk < kTK; kk++) {
                 const float2 a = sA[kk][rt * 32 + l32];
                 const float2 b = sB[kk][ct * 32 + l32];
                 acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(half ? -a.y : a.x, half ? b.y : b.x, acc0, 0, 0, 0);

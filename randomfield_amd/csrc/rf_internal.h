@@ -144,7 +144,7 @@ struct rf_group_req {
     const void* ops1;     // MFMA: wavevector stream;  direct isotropic: pT
     const void* ops2;     //                           direct isotropic: aT
     int64_t tile0;        // first tile of the field
-    int64_t nSteps;       // MFMA K-steps (multiple of 2)
+    int64_t nSteps;       // MFMA K-steps (multiple of 4)
     int32_t mtTotal;      // MFMA M-tiles of the panels
     int32_t Rpad;         // direct isotropic: padded realizations (multiple of 16)
     float outScale;       // MFMA: 1 / panel scale

@@ -339,6 +339,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
     __shared__ u32x4 ldsA[S * kGroupMT * 128];
     __shared__ u32x4 ldsB[S * kGroupMT * 128];
+    __shared__ u32x4 kqA[W * 12 * S];
+    __shared__ u32x4 kqB[W * 12 * S];
     const int64_t b = blockIdx.x;
     const rf_group_req& q = reqs[group_find(reqs, nReq, b)];
     const int nSub = (q.mtTotal + kGroupMT - 1) / kGroupMT;
@@ -349,11 +351,14 @@ void k_group_mfma(const rf_group_req* __restrict__ reqs, int nReq) {
     const rf_sum_args a = q.a;
     const u32x4* P = static_cast<const u32x4*>(q.ops0);
     const u32x4* K = static_cast<const u32x4*>(q.ops1);
+    // wavevectors staged in LDS: with at most kGroupMT tiles per block the
+    // basis, not the matrix cores, bounds a step, and this form spends the
+    // least VALU on it
     switch (mt) {
-        case 1: sum_mfma_block<V, 3, 1, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
-        case 2: sum_mfma_block<V, 3, 2, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
-        case 3: sum_mfma_block<V, 3, 3, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
-        default: sum_mfma_block<V, 3, kGroupMT, W, S, 0, 0>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, nullptr, nullptr); break;
+        case 1: sum_mfma_block<V, 3, 1, W, S, 0, 1>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, kqA, kqB); break;
+        case 2: sum_mfma_block<V, 3, 2, W, S, 0, 1>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, kqA, kqB); break;
+        case 3: sum_mfma_block<V, 3, 3, W, S, 0, 1>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, kqA, kqB); break;
+        default: sum_mfma_block<V, 3, kGroupMT, W, S, 0, 1>(a, q.nSteps, q.mtTotal, P, K, q.outScale, pt0, mt0, ldsA, ldsB, kqA, kqB); break;
     }
 }
 
@@ -488,9 +493,9 @@ hipError_t rf_launch_group_mfma(int form, const rf_group_req* d_reqs, int nReq, 
     if (nTiles <= 0 || nReq <= 0) return hipSuccess;
     if (nTiles > 0x7fffffffLL) return hipErrorInvalidValue;
     if (form == RF_FORM_MFMA_BF16)
-        hipLaunchKernelGGL((k_group_mfma<b8, kGroupW, 2>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
+        hipLaunchKernelGGL((k_group_mfma<b8, kGroupW, 4>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
     else
-        hipLaunchKernelGGL((k_group_mfma<h8, kGroupW, 2>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
+        hipLaunchKernelGGL((k_group_mfma<h8, kGroupW, 4>), dim3((unsigned)nTiles), dim3(64 * kGroupW), 0, s, d_reqs, nReq);
     return hipGetLastError();
 }
 

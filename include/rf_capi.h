@@ -60,10 +60,11 @@ extern "C" {
 #define RF_E_UNSUPPORTED  (-5)   /* combination not implemented              */
 #define RF_E_NODEVICE     (-6)   /* no usable gfx950 device                  */
 
-/* methods: constants_RF.f90:12-15 (FFT = 4 is not provided) */
+/* methods: constants_RF.f90:12-15 */
 #define RF_ISOTROPIC      1
 #define RF_SHINOZUKA      2
 #define RF_RANDOMIZATION  3
+#define RF_FFT            4   /* structured grids only (rf_plan_execute_grid) */
 /* correlation models: constants_RF.f90:17-18 */
 #define RF_CM_GAUSSIAN    1
 #define RF_CM_COS         2
@@ -105,7 +106,8 @@ typedef struct rf_params {
     double  fieldVar;    /* marginal variance                                  */
     uint64_t seed;       /* device generator key when no uniforms are given    */
     int32_t isoShared;   /* isotropic: one direction set for all realizations  */
-    int32_t reserved[7]; /* zero                                               */
+    int32_t gridN[3];    /* method 4: grid points per axis (the DCT sizes)     */
+    int32_t reserved[4]; /* zero                                               */
 } rf_params;
 
 typedef struct rf_spectrum_info {

@@ -14,7 +14,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 
 REF_COMPAT, FIXED = 0, 1
-ISOTROPIC, SHINOZUKA, RANDOMIZATION = 1, 2, 3
+ISOTROPIC, SHINOZUKA, RANDOMIZATION, FFT = 1, 2, 3, 4
 KBOX, KDENSITY = 0, 1
 PI_REF = 3.1415926535898
 
@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
         L.rfo_interp.restype = C.c_int64
         L.rfo_corrL.argtypes = [vp, C.c_int, vp, vp, vp]
         L.rfo_iso_nvar.argtypes = [C.c_int]
+        L.rfo_fft_plan.argtypes = [C.c_int, C.c_int, vp, vp, C.POINTER(Plan)]
+        L.rfo_fft.argtypes = [C.POINTER(Plan), vp, C.c_int, vp, vp, vp, vp]
         _L = L
     return _L
 
@@ -81,6 +83,28 @@ def make_plan(nDim, method, mode, xRangeNorm, corrMod=1, ksamp=KBOX, nKDensity=0
     if rc != 0:
         raise ValueError("rfo_make_plan rejected the parameters")
     return p
+
+
+def fft_plan(nDim, xRangeNorm, gridN, corrMod=1) -> Plan:
+    """Method 4 (FFT) plan on the grid of gridN points per axis."""
+    p = Plan()
+    xr = np.ascontiguousarray(list(xRangeNorm) + [1.0] * (3 - len(xRangeNorm)), dtype=np.float64)
+    gn = np.ascontiguousarray(list(gridN) + [1] * (3 - len(gridN)), dtype=np.int32)
+    if lib().rfo_fft_plan(nDim, corrMod, _p(xr), _p(gn), C.byref(p)) != 0:
+        raise ValueError("rfo_fft_plan rejected the parameters")
+    return p
+
+
+def fft(plan: Plan, sk: np.ndarray, unif: np.ndarray, xStepNorm, Nmc: int, calc=None) -> np.ndarray:
+    """Method 4 fields (Nmc, prod gridN), x fastest; unif (Nmc, 3 kN - 1)."""
+    kN = plan.kNTotal
+    unif = np.ascontiguousarray(unif, dtype=np.float64).reshape(Nmc, 3 * kN - 1)
+    xs = np.ascontiguousarray(list(xStepNorm) + [1.0] * (3 - len(xStepNorm)), dtype=np.float64)
+    c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
+    out = np.zeros((Nmc, kN))
+    lib().rfo_fft(C.byref(plan), _p(np.ascontiguousarray(sk, dtype=np.float64)), Nmc, _p(unif), _p(c), _p(xs),
+                  _p(out))
+    return out
 
 
 def kpoints(plan: Plan, kU=None) -> np.ndarray:

@@ -129,9 +129,41 @@ int rfo_make_plan(int nDim, int method, int corrMod, int mode, int ksamp,
     return 0;
 }
 
+int rfo_fft_plan(int nDim, int corrMod, const double* xRangeNorm, const int32_t* gridN, rfo_plan* p)
+{
+    memset(p, 0, sizeof(*p));
+    if (nDim < 2 || nDim > 3) return -1;
+    p->nDim = nDim; p->method = RFO_FFT; p->corrMod = corrMod; p->mode = RFO_FIXED;
+    int64_t kN = 1;
+    for (int i = 0; i < nDim; i++) {
+        if (gridN[i] < 2) return -1;
+        p->kNStep[i] = gridN[i];
+        p->kDelta[i] = 2.0 * RFO_PI / (1.1 * xRangeNorm[i]);
+        p->kMax[i] = ((double)(gridN[i] - 1) * p->kDelta[i]) / 1.0;
+        kN *= gridN[i];
+    }
+    p->kNTotal = kN;
+    p->kN_unsigned = kN;
+    p->nSignBlocks = 1;
+    p->ampMult = 1.0;
+    return 0;
+}
+
 void rfo_kpoints(const rfo_plan* p, const double* kUnif, double* kPts)
 {
     const int d = p->nDim;
+    if (p->method == RFO_FFT) {
+        /* setGrid(kPoints, 0, kDelta, kNStep) (spectra_RF.f90:160), dim 1 fastest */
+        for (int64_t j = 0; j < p->kNTotal; j++) {
+            int64_t rem = j;
+            for (int dd = 0; dd < d; dd++) {
+                const int64_t idx = rem % p->kNStep[dd];
+                rem /= p->kNStep[dd];
+                kPts[j * d + dd] = 0.0 + p->kDelta[dd] * (double)idx;
+            }
+        }
+        return;
+    }
     if (p->method == RFO_ISOTROPIC) {
         /* intended get_Permutation(i, kMax, kNStep, snapExtremes) value
          * (math_RF.f90:137-145): ((i - 0.5) - 0.5) * kMax / (kNStep - 1) */
@@ -497,6 +529,60 @@ void rfo_blend(int nDim, const int32_t* groupN, int Nmc, int nSubdomains,
 }
 
 /* ---------------------------------------------------------- statistics */
+
+void rfo_fft(const rfo_plan* p, const double* Sk, int Nmc, const double* unif, const uint8_t* calc,
+             const double* xStepNorm, double* out)
+{
+    const int d = p->nDim;
+    const int64_t kN = p->kNTotal, nu = 3 * kN - 1;
+    const double pi = 3.14159265358979323846;   /* FFTW's transform is exact-math */
+    double* cosT[3] = {NULL, NULL, NULL};
+    int maxn = 1;
+    for (int a = 0; a < d; a++) {
+        const int n = p->kNStep[a];
+        if (n > maxn) maxn = n;
+        cosT[a] = (double*)malloc(sizeof(double) * (size_t)n * (size_t)n);
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                cosT[a][(size_t)j * n + i] = cos(pi * (double)j * (2.0 * i + 1.0) / (2.0 * n));
+    }
+    double vol = 1.0;
+    for (int a = 0; a < d; a++) vol *= xStepNorm[a];
+    const double scale = (d == 3 ? 2.0 : 1.0) * sqrt(vol);
+    double* line = (double*)malloc(sizeof(double) * (size_t)maxn);
+    for (int r = 0; r < Nmc; r++) {
+        double* f = out + (size_t)r * kN;
+        if (calc && !calc[r]) {
+            for (int64_t j = 0; j < kN; j++) f[j] = 0.0;
+            continue;
+        }
+        const double* U = unif + (size_t)r * nu;
+        for (int64_t j = 0; j < kN; j++) {
+            const double g = U[j] - 0.5;
+            f[j] = g * sqrt(Sk[j]) * cos(2.0 * RFO_PI * U[2 * kN - 1 + j]);
+        }
+        /* REDFT01 along axis a: lines of n[a] values at stride prod(n[<a]) */
+        int64_t stride = 1;
+        for (int a = 0; a < d; a++) {
+            const int n = p->kNStep[a];
+            const int64_t outer = kN / ((int64_t)n * stride);
+            for (int64_t o = 0; o < outer; o++)
+                for (int64_t in = 0; in < stride; in++) {
+                    double* base = f + o * n * stride + in;
+                    for (int i = 0; i < n; i++) {
+                        double y = base[0];
+                        for (int j = 1; j < n; j++) y += 2.0 * base[(int64_t)j * stride] * cosT[a][(size_t)j * n + i];
+                        line[i] = y;
+                    }
+                    for (int i = 0; i < n; i++) base[(int64_t)i * stride] = line[i];
+                }
+            stride *= n;
+        }
+        for (int64_t j = 0; j < kN; j++) f[j] *= scale;
+    }
+    free(line);
+    for (int a = 0; a < d; a++) free(cosT[a]);
+}
 
 void rfo_normalize(double* field, int64_t N, int Nmc)
 {

@@ -33,7 +33,7 @@ extern "C" {
 /* constants_RF.f90:5 -- not the full-precision pi; parity needs this literal */
 #define RFO_PI 3.1415926535898
 
-enum { RFO_ISOTROPIC = 1, RFO_SHINOZUKA = 2, RFO_RANDOMIZATION = 3 };
+enum { RFO_ISOTROPIC = 1, RFO_SHINOZUKA = 2, RFO_RANDOMIZATION = 3, RFO_FFT = 4 };
 enum { RFO_CM_GAUSSIAN = 1, RFO_CM_COS = 2 };
 enum { RFO_FOM_GAUSSIAN = 1, RFO_FOM_LOGNORMAL = 2 };
 enum { RFO_REF_COMPAT = 0, RFO_FIXED = 1 };
@@ -60,6 +60,10 @@ void rfo_kmax(int corrMod, int nDim, double* kMax);
  * already divided by corrL (randomFieldND.f90:72).  Returns 0 or -1. */
 int rfo_make_plan(int nDim, int method, int corrMod, int mode, int ksamp,
                   int64_t nKDensity, const double* xRangeNorm, rfo_plan* p);
+/* spectra_RF.f90:135-139 (case FFT) on one rank: kNStep = gridN (the grid
+ * points, find_xNStep(xRange, xStep) = 1 + nint(xRange/xStep)), kDelta =
+ * 2 pi / (1.1 xRange), lattice from 0, one block.  nDim 2 or 3; returns 0 or -1. */
+int rfo_fft_plan(int nDim, int corrMod, const double* xRangeNorm, const int32_t* gridN, rfo_plan* p);
 /* spectra_RF.f90:267-286 ; kSign is (2^(nDim-1)) x nDim column-major */
 void rfo_ksign(int nDim, double* kSign);
 /* spectra_RF.f90:75-133 + math_RF.f90:16-92,98-148.  kUnif holds the
@@ -86,6 +90,15 @@ void rfo_isotropic(const rfo_plan* p, const double* x, int64_t N,
                    const double* kRad, const double* Sk, int Nmc,
                    const double* unif, const uint8_t* calc, double* out,
                    int threads);
+
+/* gen_Std_Gauss_FFT, randomFieldND.f90:670-735, on one rank, per
+ * realization: unif[n*(3 kN - 1) + .] = gammaK (kN), kN - 1 discarded draws,
+ * phiK (kN); coefficient (gammaK - 0.5) sqrt(Sk) cos(2 PI phiK); the FFTW
+ * REDFT01 transform (DCT-III, y_i = x_0 + 2 sum_{j>=1} x_j cos(pi j (2i+1)/(2n)))
+ * along every axis as direct FP64 sums; times sqrt(prod xStepNorm), and 2 more
+ * in 3-D.  out[n*kN + i], x fastest; realizations with calc[n] == 0 are zero. */
+void rfo_fft(const rfo_plan* p, const double* Sk, int Nmc, const double* unif, const uint8_t* calc,
+             const double* xStepNorm, double* out);
 
 /* randomFieldND.f90:139-178.  Returns 0, or -2 for lognormal with avg <= 0. */
 int rfo_marginal(int margiFirst, double fieldAvg, double fieldVar,

@@ -21,7 +21,7 @@ import numpy as np
 
 from ._capi import (  # noqa: F401
     CM_COS, CM_GAUSSIAN, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
-    FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
+    FFT, FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
     MFMA_VARIANTS, RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
     RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, Params, RFError, SpectrumInfo, check, flib, lib,
     make_params,

@@ -13,7 +13,7 @@ LIBDIR = Path(__file__).resolve().parent / "lib"
 
 RF_OK = 0
 RF_E_ARG, RF_E_LOGNORMAL, RF_E_NOMEM, RF_E_HIP, RF_E_UNSUPPORTED, RF_E_NODEVICE = -1, -2, -3, -4, -5, -6
-ISOTROPIC, SHINOZUKA, RANDOMIZATION = 1, 2, 3
+ISOTROPIC, SHINOZUKA, RANDOMIZATION, FFT = 1, 2, 3, 4
 CM_GAUSSIAN, CM_COS = 1, 2
 FOM_NONE, FOM_GAUSSIAN, FOM_LOGNORMAL = 0, 1, 2
 REF_COMPAT, FIXED = 0, 1
@@ -44,7 +44,7 @@ class Params(C.Structure):
         ("nKDensity", C.c_int64), ("xSlab", C.c_int64),
         ("corrL", C.c_double * 3), ("xRange", C.c_double * 3),
         ("fieldAvg", C.c_double), ("fieldVar", C.c_double),
-        ("seed", C.c_uint64), ("isoShared", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("seed", C.c_uint64), ("isoShared", C.c_int32), ("gridN", C.c_int32 * 3), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -92,7 +92,7 @@ class ServerStats(C.Structure):
 def make_params(nDim=3, method=SHINOZUKA, corrMod=CM_GAUSSIAN, mode=REF_COMPAT, form=FORM_FP64,
                 kSampling=KSAMP_BOX, Nmc=1, margiFirst=FOM_NONE, nKDensity=0, xSlab=0,
                 corrL=(1.0, 1.0, 1.0), xRange=(1.0, 1.0, 1.0), fieldAvg=0.0, fieldVar=1.0,
-                seed=0, isoShared=0) -> Params:
+                seed=0, isoShared=0, gridN=(0, 0, 0)) -> Params:
     p = Params()
     p.nDim, p.method, p.corrMod, p.mode, p.form = nDim, method, corrMod, mode, form
     p.kSampling, p.Nmc, p.margiFirst, p.nKDensity, p.xSlab = kSampling, Nmc, margiFirst, nKDensity, xSlab
@@ -101,6 +101,8 @@ def make_params(nDim=3, method=SHINOZUKA, corrMod=CM_GAUSSIAN, mode=REF_COMPAT, 
     p.corrL = (C.c_double * 3)(*cl[:3])
     p.xRange = (C.c_double * 3)(*xr[:3])
     p.fieldAvg, p.fieldVar, p.seed, p.isoShared = fieldAvg, fieldVar, seed, isoShared
+    gn = list(gridN) + [0] * (3 - len(gridN))
+    p.gridN = (C.c_int32 * 3)(*gn[:3])
     return p
 
 

@@ -10,6 +10,7 @@
 // rf_grid_batch_execute then enqueues three grouped GEMM launches and nothing
 // else, so it can be captured into a graph and replayed.
 #include <cstring>
+#include <cmath>
 #include <memory>
 #include <new>
 #include <vector>
@@ -54,13 +55,17 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         if (!P || !d_outs[i]) return rf_set_error(ctx, RF_E_ARG, "NULL plan or output in batch");
         if (P->ctx != ctx) return rf_set_error(ctx, RF_E_ARG, "plan of another context in batch");
         const rf_params& p = P->p;
-        if (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2)
-            return rf_set_error(ctx, RF_E_UNSUPPORTED, "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3");
+        const bool fft = p.method == RF_FFT;
+        if (!fft && (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2))
+            return rf_set_error(ctx, RF_E_UNSUPPORTED,
+                                "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3, or method 4");
         const int32_t* ns = nsteps + 3 * i;
         const rf_spectrum_info& in = P->spec.info;
         int64_t rows1 = (int64_t)in.nSignBlocks * p.Nmc, plane = 1;
         for (int a = 0; a < p.nDim; a++) {
             if (ns[a] < 1 || !(steps[3 * i + a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "bad grid counts or steps");
+            if (fft && ns[a] != p.gridN[a])
+                return rf_set_error(ctx, RF_E_ARG, "method 4: the grid counts must equal params.gridN (the DCT sizes)");
             if (a > 0) rows1 *= in.kNStep[a];
             if (a + 1 < p.nDim) plane *= ns[a];
         }
@@ -91,9 +96,17 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
             ns[a] = nsteps[3 * i + a];
             corrL[a] = p.corrL[a];
         }
+        const bool fft = p.method == RF_FFT;
+        double ampMult = in.ampMult;
+        if (fft) {
+            // the DCT scale of randomFieldND.f90:707 and :732, as in rf_plan_execute_grid
+            double v = 1.0;
+            for (int a = 0; a < p.nDim; a++) v *= st[a] / corrL[a];
+            ampMult = (p.nDim == 3 ? 2.0 : 1.0) * std::sqrt(v);
+        }
         rf_grid_stages gs;
-        const hipError_t e = rf_grid_prepare(p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, o, st, ns, corrL,
-                                             P->d_amp, P->d_phase, in.ampMult, P->epi, d_outs[i], outDtype,
+        const hipError_t e = rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, o, st,
+                                             ns, corrL, P->d_amp, P->d_phase, ampMult, P->epi, d_outs[i], outDtype,
                                              reinterpret_cast<float2*>(B->d_mem + off[(size_t)i]), s, &gs);
         if (e != hipSuccess) {
             (void)hipFree(B->d_mem);

@@ -225,6 +225,17 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
     P->kNpad = mfma ? (kN + rf_mfma_kpad(p->form) - 1) / rf_mfma_kpad(p->form) * rf_mfma_kpad(p->form) : kN;
 
     double maxAmp = 0.0;
+    if (p->method == RF_FFT) {
+        // gen_Std_Gauss_FFT (randomFieldND.f90:683-689): real DCT coefficients,
+        // evaluated once; the grid form is its only executor
+        RF_HIP_TRY(ctx, dmalloc(&P->d_Sk, (size_t)kN));
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_Sk, P->spec.Sk.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
+        RF_HIP_TRY(ctx, rf_launch_fft_coeffs(kN, Nmc, P->d_Sk, P->d_uni, P->d_calc, P->d_amp, P->d_phase, s));
+        P->coeffsReady = true;
+        RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+        *out = P.release();
+        return RF_OK;
+    }
     if (p->method == RF_ISOTROPIC) {
         RF_HIP_TRY(ctx, dmalloc(&P->d_kIso, (size_t)(kN * Nmc * d)));
         RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)kN));
@@ -316,6 +327,8 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
     if (!d_x || !d_out) return rf_set_error(ctx, RF_E_ARG, "NULL device buffer");
     if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32)
         return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
+    if (P->p.method == RF_FFT)
+        return rf_set_error(ctx, RF_E_UNSUPPORTED, "method 4 (FFT) generates structured grids only: rf_plan_execute_grid");
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     const rf_params& p = P->p;
@@ -450,12 +463,15 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     if (!origin || !step || !nstep || !d_out) return rf_set_error(ctx, RF_E_ARG, "NULL grid argument");
     if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32)
         return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
-    if (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2)
+    const bool fft = p.method == RF_FFT;
+    if (!fft && (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2))
         return rf_set_error(ctx, RF_E_UNSUPPORTED,
-                            "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3");
+                            "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3, or method 4");
     for (int a = 0; a < p.nDim; a++) {
         if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
         if (!(step[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid steps must be > 0");
+        if (fft && nstep[a] != p.gridN[a])
+            return rf_set_error(ctx, RF_E_ARG, "method 4: the grid counts must equal params.gridN (the DCT sizes)");
     }
     {
         // every GEMM dimension of the three stages is an int32
@@ -480,9 +496,17 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     }
     double corrL[3] = {1.0, 1.0, 1.0};
     for (int a = 0; a < p.nDim; a++) corrL[a] = p.corrL[a];
+    double ampMult = in.ampMult;
+    if (fft) {
+        // randomFieldND.f90:707 (2-D) and :732 (3-D): the transform is scaled by
+        // sqrt(prod xStep) and, in 3-D, by 2 more; xStep in correlation lengths
+        double v = 1.0;
+        for (int a = 0; a < p.nDim; a++) v *= step[a] / corrL[a];
+        ampMult = (p.nDim == 3 ? 2.0 : 1.0) * std::sqrt(v);
+    }
     rf_grid_stages st;
-    RF_HIP_TRY(ctx, rf_grid_prepare(p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step, nstep,
-                                    corrL, P->d_amp, P->d_phase, in.ampMult, P->epi, d_out, outDtype,
+    RF_HIP_TRY(ctx, rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
+                                    nstep, corrL, P->d_amp, P->d_phase, ampMult, P->epi, d_out, outDtype,
                                     static_cast<float2*>(P->d_gridScratch), s, &st));
     RF_HIP_TRY(ctx, rf_grid_run(st, s));
     return RF_OK;

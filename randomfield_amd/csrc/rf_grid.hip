@@ -197,6 +197,19 @@ __global__ void k_phase_table(int nk, int nx, double dk, double o, double step, 
     if (variants > 1) E[(int64_t)nk * nx + e] = make_float2((float)c, (float)-s);
 }
 
+// Method 4: E[v][j][i] = (w_j cos(pi j (2i+1) / (2n)), 0), w_0 = 1, w_j = 2:
+// the REDFT01 (DCT-III) of FFTW that gen_Std_Gauss_FFT applies along each
+// axis (randomFieldND.f90:694-735), as a matrix; both variants equal.
+__global__ void k_dct_table(int n, float2* __restrict__ E) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int j = (int)(e / n), i = (int)(e % n);
+    const double w = j == 0 ? 1.0 : 2.0;
+    const double c = w * cos(3.14159265358979323846 * (double)j * (2.0 * i + 1.0) / (2.0 * n));
+    E[e] = make_float2((float)c, 0.0f);
+    E[(int64_t)n * n + e] = make_float2((float)c, 0.0f);
+}
+
 // Per-block table for the last axis: Ext[b][j][x] = E^{s_b}[j][x], so the
 // last stage contracts over (b, j) with a plain stride.
 __global__ void k_block_table(int nb, int nk, int nx, int axisBit, const float2* __restrict__ E,
@@ -264,7 +277,7 @@ size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, c
 // describes the GEMM stages: stage 1 rows (r, b, j_d..j_2) x j1 -> x1;
 // stage 2 (3-D) per (r, b, j3): x2 x j2 times j2 x x1; last stage per r:
 // x_d x (b, j_d) times (b, j_d) x plane, real part, epilogue, into d_out.
-hipError_t rf_grid_prepare(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
+hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
                            double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
@@ -277,8 +290,11 @@ hipError_t rf_grid_prepare(int nDim, int Nmc, const int32_t* kNStep, const doubl
         E[a] = p; p += 2LL * kNStep[a] * n[a];
         Ext[a] = p; p += (int64_t)nb * kNStep[a] * n[a];
         const int64_t cnt = (int64_t)kNStep[a] * n[a];
-        hipLaunchKernelGGL(k_phase_table, dim3(blocks_for(cnt)), dim3(256), 0, s, kNStep[a], n[a],
-                           kDelta[a], origin[a], step[a], 1.0 / corrL[a], 2, E[a]);
+        if (kind == 1)
+            hipLaunchKernelGGL(k_dct_table, dim3(blocks_for(cnt)), dim3(256), 0, s, n[a], E[a]);
+        else
+            hipLaunchKernelGGL(k_phase_table, dim3(blocks_for(cnt)), dim3(256), 0, s, kNStep[a], n[a],
+                               kDelta[a], origin[a], step[a], 1.0 / corrL[a], 2, E[a]);
         // sign of axis a in block b: kSign column a flips on bit (nDim-1-a) of b
         if (a > 0)
             hipLaunchKernelGGL(k_block_table, dim3(blocks_for(nb * cnt)), dim3(256), 0, s, nb, kNStep[a],

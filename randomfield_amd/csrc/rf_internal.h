@@ -89,6 +89,8 @@ hipError_t rf_launch_prep_fp32(int64_t kN, int Nmc, int Rpad, const double* d_ph
                                const double* d_amp, double ampMult, float* d_phT, float* d_aT,
                                hipStream_t s);
 hipError_t rf_launch_krev(int64_t count, const double* d_k, double* d_kRev, hipStream_t s);
+hipError_t rf_launch_fft_coeffs(int64_t kN, int Nmc, const double* d_Sk, const double* d_u,
+                                const uint8_t* d_calc, double* d_amp, double* d_phase, hipStream_t s);
 
 struct rf_sum_args {
     const double* x;      // nDim x N caller units
@@ -199,7 +201,9 @@ struct rf_grid_stages {
     int n;
 };
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
-hipError_t rf_grid_prepare(int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
+// kind 0: Shinozuka lattice, tables exp(i k_j x); kind 1: method 4, tables of
+// the DCT-III (FFTW REDFT01) of each axis, w_j cos(pi j (2i+1) / (2n))
+hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
                            double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,

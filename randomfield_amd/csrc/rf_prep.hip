@@ -123,6 +123,24 @@ __global__ void k_prep_fp32_iso(int nDim, int64_t kN, int Nmc, int Rpad,
     }
 }
 
+// gen_Std_Gauss_FFT coefficients (randomFieldND.f90:683-689) per realization:
+// uniforms u[r][0 .. 3 kN - 2] = gammaK (kN), kN - 1 discarded, phiK (kN);
+// amp = (gammaK - 0.5) * sqrt(S) * cos(2 pi phiK), phase 0.
+__global__ void k_fft_coeffs(int64_t kN, int Nmc, const double* __restrict__ Sk,
+                             const double* __restrict__ u, const uint8_t* __restrict__ calc,
+                             double* __restrict__ amp, double* __restrict__ phase) {
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kN * Nmc) return;
+    const int r = (int)(e / kN);
+    const int64_t j = e - (int64_t)r * kN;
+    const double* U = u + (int64_t)r * (3 * kN - 1);
+    const double g = U[j] - 0.5;
+    const double ph = U[2 * kN - 1 + j];
+    amp[e] = calc[r] ? g * sqrt(Sk[j]) * cos(2.0 * RF_PI_REF * ph) : 0.0;
+    phase[e] = 0.0;
+}
+
 __global__ void k_scale(int64_t n, const double* __restrict__ a, double s, double* __restrict__ b) {
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e < n) b[e] = a[e] * s;
@@ -176,5 +194,13 @@ hipError_t rf_launch_prep_fp32_iso(int nDim, int64_t kN, int Nmc, int Rpad, cons
     const int64_t n = kN * Rpad;
     hipLaunchKernelGGL(k_prep_fp32_iso, dim3(grid_for(n)), dim3(kBlock), 0, s, nDim, kN, Nmc, Rpad,
                        d_kIso, d_phase, d_amp, ampMult, d_kT, d_pT, d_aT);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_fft_coeffs(int64_t kN, int Nmc, const double* d_Sk, const double* d_u,
+                                const uint8_t* d_calc, double* d_amp, double* d_phase, hipStream_t s) {
+    const int64_t n = kN * Nmc;
+    hipLaunchKernelGGL(k_fft_coeffs, dim3(grid_for(n)), dim3(kBlock), 0, s, kN, Nmc, d_Sk, d_u, d_calc, d_amp,
+                       d_phase);
     return hipGetLastError();
 }

@@ -37,8 +37,13 @@ int rf_validate_params(const rf_params* p, std::string* err) {
     if (!p) { *err = "params is NULL"; return RF_E_ARG; }
     if (p->nDim < 1 || p->nDim > 3) { *err = "nDim must be 1, 2 or 3"; return RF_E_ARG; }
     if (p->Nmc < 1) { *err = "Nmc must be >= 1"; return RF_E_ARG; }
-    if (p->method == 4) { *err = "method 4 (FFT) is not provided by this library"; return RF_E_UNSUPPORTED; }
-    if (p->method < RF_ISOTROPIC || p->method > RF_RANDOMIZATION) { *err = "unknown method"; return RF_E_ARG; }
+    if (p->method < RF_ISOTROPIC || p->method > RF_FFT) { *err = "unknown method"; return RF_E_ARG; }
+    if (p->method == RF_FFT) {
+        // gen_Std_Gauss_FFT exists for 2-D and 3-D (randomFieldND.f90:565-597)
+        if (p->nDim < 2) { *err = "method 4 (FFT) needs nDim 2 or 3"; return RF_E_UNSUPPORTED; }
+        for (int i = 0; i < p->nDim; i++)
+            if (p->gridN[i] < 2) { *err = "method 4 needs gridN >= 2 points per axis"; return RF_E_ARG; }
+    }
     if (p->corrMod != RF_CM_GAUSSIAN && p->corrMod != RF_CM_COS) { *err = "unknown corrMod"; return RF_E_ARG; }
     if (p->mode != RF_REF_COMPAT && p->mode != RF_FIXED) { *err = "unknown mode"; return RF_E_ARG; }
     if (p->form < RF_FORM_FP64 || p->form > RF_FORM_MFMA_BF16) { *err = "unknown form"; return RF_E_ARG; }
@@ -104,6 +109,24 @@ int rf_spectrum_build_info(const rf_params* p, rf_spectrum_info* info, std::stri
         const double tpd = (d == 2) ? tp * tp : tp * tp * tp;
         info->ampMult = std::sqrt(info->isoStep / tpd);
         if (p->mode == RF_FIXED) info->ampMult *= (d == 3) ? 2.0 * RF_PI_REF : std::sqrt(4.0 * RF_PI_REF);
+        return RF_OK;
+    }
+    if (p->method == RF_FFT) {
+        // spectra_RF.f90:135-160: one wavevector per grid point, j * kDelta
+        // with kDelta = 2 pi / (1.1 xRange), kMax = (kNStep - 1) kDelta
+        int64_t kN = 1;
+        for (int i = 0; i < d; i++) {
+            info->kNStep[i] = p->gridN[i];
+            info->kDelta[i] = 2.0 * RF_PI_REF / (1.1 * xr[i]);
+            info->kMax[i] = ((double)(p->gridN[i] - 1) * info->kDelta[i]) / 1.0;
+            kN *= p->gridN[i];
+        }
+        info->kNTotal = kN;
+        info->kN_unsigned = kN;
+        info->nSignBlocks = 1;
+        // randomFieldND.f90:670-681: gammaK, kN - 1 discarded draws, phiK
+        info->nPhaseUniform = 3 * kN - 1;
+        info->ampMult = 1.0;            // the grid-step scale is applied at execution
         return RF_OK;
     }
     if (p->method == RF_RANDOMIZATION && p->kSampling == RF_KSAMP_DENSITY) {
@@ -178,7 +201,7 @@ int rf_spectrum_build_tables(const rf_params* p, const rf_spectrum_info& info,
             const int nb = info.nSignBlocks;
             for (int64_t j = 0; j < kNu; j++) {
                 double base[3];
-                if (p->method == RF_SHINOZUKA) {
+                if (p->method == RF_SHINOZUKA || p->method == RF_FFT) {
                     // setGrid(kPoints_base, 0, kDelta, kNStep), dim 1 fastest
                     int64_t rem = j;
                     for (int c = 0; c < d; c++) {
@@ -190,11 +213,13 @@ int rf_spectrum_build_tables(const rf_params* p, const rf_spectrum_info& info,
                     // random_number(kPoints_base): component fastest, * kMax(i)
                     for (int c = 0; c < d; c++) base[c] = kUniform[j * d + c] * info.kMax[c];
                 }
-                // block 0 is never written by the reference (:127)
+                // block 0 is never written by the reference (:127); method 4
+                // writes its one block directly (setGrid, :160)
+                const bool blank0 = p->mode == RF_REF_COMPAT && p->method != RF_FFT;
                 for (int b = 0; b < nb; b++) {
                     double* dst = &spec->k[(size_t)((b * kNu + j) * d)];
                     for (int c = 0; c < d; c++)
-                        dst[c] = (b == 0 && p->mode == RF_REF_COMPAT) ? 0.0 : base[c] * ksign(d, b, c);
+                        dst[c] = (b == 0 && blank0) ? 0.0 : base[c] * ksign(d, b, c);
                 }
             }
         }

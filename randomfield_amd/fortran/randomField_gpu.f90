@@ -42,7 +42,8 @@ module randomField_gpu
         real(c_double)     :: fieldAvg = 0d0, fieldVar = 1d0
         integer(c_int64_t) :: seed = 0
         integer(c_int32_t) :: isoShared = 0
-        integer(c_int32_t) :: reserved(7) = 0
+        integer(c_int32_t) :: gridN(3) = 0
+        integer(c_int32_t) :: reserved(4) = 0
     end type rf_params_c
 
     type, bind(C), public :: rf_spectrum_info_c
@@ -354,6 +355,10 @@ contains
                 do v = 1, nv
                     call random_number(phU((v - 1)*kN + 1:v*kN, n))
                 end do
+            else if (p%method == FFT) then
+                ! gen_Std_Gauss_FFT on one rank (randomFieldND.f90:670-681):
+                ! gammaK (kN), kN-1 discarded draws, phiK (kN), in one stream
+                call random_number(phU(1:info%nPhaseUniform, n))
             else
                 call random_number(phU(1:kN, n))
             end if

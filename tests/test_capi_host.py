@@ -51,6 +51,34 @@ def test_spectrum_tables_match_oracle(nDim, method, mode, xr):
     assert np.array_equal(k, ko) and np.array_equal(s, O.skvec(po, ko))
 
 
+@pytest.mark.parametrize("nDim,xr,gn", [(2, (5.0, 9.0), (11, 19)), (3, (2.0, 3.0, 1.5), (5, 7, 4))])
+def test_fft_tables_match_oracle(nDim, xr, gn):
+    corrL = (0.5, 1.5, 2.0)[:nDim]
+    p = R.make_params(nDim=nDim, method=R.FFT, mode=R.FIXED, Nmc=2, corrL=corrL, xRange=xr, gridN=gn)
+    info = R.plan_spectrum(p)
+    po = O.fft_plan(nDim, [xr[i] / corrL[i] for i in range(nDim)], gn)
+    assert info.kNTotal == po.kNTotal == int(np.prod(gn)) and info.nSignBlocks == 1
+    assert info.nPhaseUniform == 3 * info.kNTotal - 1
+    assert list(info.kDelta)[:nDim] == list(po.kDelta)[:nDim]
+    assert list(info.kNStep)[:nDim] == list(gn)
+    k, s = R.spectrum_tables(p)
+    ko = O.kpoints(po)
+    assert np.array_equal(k, ko) and np.array_equal(s, O.skvec(po, ko))
+    # lattice from 0 in steps kDelta = 2 pi / (1.1 xRange), x fastest
+    assert np.allclose(ko[1, 0], 2 * np.pi / (1.1 * xr[0] / corrL[0]), rtol=1e-12) and ko[0].tolist() == [0.0] * nDim
+    assert ko[gn[0], 1] > 0 and ko[gn[0], 0] == 0.0
+
+
+def test_fft_draws_continue_one_stream():
+    # gen_Std_Gauss_FFT draws 3 kN - 1 uniforms per realization from one stream
+    p = R.make_params(nDim=2, method=R.FFT, Nmc=2, xRange=(3, 3), gridN=(4, 5))
+    _, ph = R.reference_draws(p, [90])
+    p2 = R.make_params(nDim=2, method=R.SHINOZUKA, Nmc=1, xRange=(40, 40))
+    _, flat = R.reference_draws(p2, [90])
+    assert ph.shape == (2, 59)
+    assert np.array_equal(ph.reshape(-1), flat[0, :118])
+
+
 def test_randomization_tables_match_oracle():
     rng = np.random.default_rng(0)
     for ks, nk in ((R.KSAMP_BOX, 0), (R.KSAMP_DENSITY, 333)):
@@ -69,7 +97,9 @@ def test_randomization_tables_match_oracle():
 @pytest.mark.parametrize("kw,code", [
     (dict(Nmc=0), R.RF_E_ARG),
     (dict(nDim=4), R.RF_E_ARG),
-    (dict(method=4), R.RF_E_UNSUPPORTED),
+    (dict(method=4), R.RF_E_ARG),                       # gridN missing
+    (dict(method=4, nDim=1, xRange=(1.0,), gridN=(8,)), R.RF_E_UNSUPPORTED),
+    (dict(method=5), R.RF_E_ARG),
     (dict(corrL=(0.0, 1.0, 1.0)), R.RF_E_ARG),
     (dict(margiFirst=R.FOM_LOGNORMAL, fieldAvg=0.0), R.RF_E_LOGNORMAL),
     (dict(method=R.ISOTROPIC, nDim=1, xRange=(1.0,)), R.RF_E_UNSUPPORTED),

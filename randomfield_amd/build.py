@@ -2,6 +2,7 @@
 
     librf_hip.so          HIP kernels + C ABI (hipcc, gfx950 code objects)
     librandomfield_gpu.so Fortran facade over the C ABI (amdflang)
+    librf_h5.so           HDF5 / XDMF writer of structured fields (host, g++)
     tests/fortran/bin/*   Fortran drivers of the facade used by the test suite
     oracle/liboracle_rf.so  the CPU oracle (test infrastructure, gcc)
 
@@ -84,6 +85,24 @@ def build_fortran() -> Path:
     return out
 
 
+H5_INC = Path(os.environ.get("RF_HDF5_INC", "/opt/conda/include"))
+H5_LIB = Path(os.environ.get("RF_HDF5_LIB", "/opt/conda/lib"))
+
+
+def build_h5() -> Path | None:
+    """librf_h5.so: the HDF5/XDMF writer (host only).  Skipped, with a note,
+    where the HDF5 C library is absent."""
+    if not (H5_INC / "hdf5.h").exists() or not (H5_LIB / "libhdf5.so").exists():
+        sys.stderr.write(f"note: HDF5 not found under {H5_INC} / {H5_LIB}: librf_h5.so not built\n")
+        return None
+    out = LIB / "librf_h5.so"
+    tmp = LIB / "librf_h5.so.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{H5_INC}",
+          str(PKG / "h5" / "rf_h5.cpp"), "-o", str(tmp), f"-L{H5_LIB}", "-lhdf5", f"-Wl,-rpath,{H5_LIB}"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> Path:
     out = ORACLE / "liboracle_rf.so"
     tmp = ORACLE / "liboracle_rf.so.tmp"
@@ -96,6 +115,7 @@ def build_oracle() -> Path:
 def build_all(jobs: int = 8) -> None:
     build_oracle()
     build_hip(jobs)
+    build_h5()
     if shutil.which(FC) or Path(FC).exists():
         build_fortran()
     else:

@@ -2,6 +2,8 @@
 Shinozuka sum) and of the localization pipeline built on it: subdomains
 generated on their own grids and lattices, then blended.  The oracle sums the
 same wavevectors point by point in FP64 (randomFieldND.f90:276-330 order)."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -201,3 +203,29 @@ def test_graph_replay_of_batch_and_blend(ctx):
     batch.close()
     for pl in plans:
         pl.close()
+
+
+def test_generated_grid_to_hdf5(ctx, tmp_path):
+    """A generated grid's realization 1 through the HDF5 writer, read back by
+    h5dump, equals the device field (x fastest, HDF5 dims last axis first)."""
+    import re
+    import shutil
+    import subprocess
+    import torch
+    from randomfield_amd import h5io
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if not h5io.LIB.exists() or not Path(h5dump).exists():
+        pytest.skip("HDF5 C library or h5dump absent")
+    n, step = (20, 12, 9), (0.25, 0.25, 0.25)
+    p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=2, xRange=(5, 3, 2), seed=4)
+    plan = R.FieldPlan(ctx, p)
+    out = torch.empty((2, int(np.prod(n))), dtype=torch.float64, device="cuda")
+    plan.execute_grid((0.0, 0.0, 0.0), step, n, out)
+    ctx.sync()
+    plan.close()
+    h5, _ = h5io.write_simple(tmp_path, "gen", out[0], n, step, (0.0, 0.0, 0.0), n)
+    txt = subprocess.run([h5dump, "-d", "/samples", "-y", "-w", "0", "-m", "%.17g", h5], capture_output=True,
+                         text=True, check=True).stdout
+    body = txt.split("DATA {", 1)[1].rsplit("}", 3)[0]
+    vals = np.array([float(v) for v in re.split(r"[,\s]+", body.strip()) if v])
+    assert np.array_equal(vals, out[0].cpu().numpy())

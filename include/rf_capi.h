@@ -286,6 +286,29 @@ int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep,
                           const void* d_grid, int32_t gridDtype,
                           const double* d_nodes, int64_t nNodes, void* d_out,
                           int32_t outDtype, void* stream, int32_t* outside);
+/* interpolateToMesh on the bounding box it reads (sample_RF.f90:936-975):
+ * d_box holds global grid points minPos..maxPos (1-based, inclusive, x
+ * fastest, Nmc realizations back to back) of the grid that starts at xMinGlob;
+ * positions are measured from xMinGlob and the cell rule's border is maxPos,
+ * as in the reference.  Otherwise as rf_interpolate_device. */
+int rf_interpolate_box_device(rf_context* ctx, int32_t nDim, const int32_t* minPos,
+                              const int32_t* maxPos, const double* xMinGlob,
+                              const double* xStep, int32_t Nmc, const void* d_box,
+                              int32_t gridDtype, const double* d_nodes, int64_t nNodes,
+                              void* d_out, int32_t outDtype, void* stream, int32_t* outside);
+/* readUNV (readUNV_RF.f90:14-215, one rank): node coordinates of datasets
+ * 2411/781 and connectivity of 2412/780 of a universal file.  Host only.
+ * Files with element types other than FE descriptor 115 (8-node brick) are
+ * refused (RF_E_UNSUPPORTED): the reference misreads their records. */
+typedef struct rf_unv_mesh rf_unv_mesh;
+int rf_unv_read(const char* path, int32_t nDim, rf_unv_mesh** out);
+/* nNodes, nElem, maxConnect (= size(connectList,1)), monoType (0/1) */
+int rf_unv_sizes(const rf_unv_mesh* m, int64_t* nNodes, int64_t* nElem,
+                 int32_t* maxConnect, int32_t* monoType);
+/* coordList (nDim x nNodes), connectList (maxConnect x nElem, node labels as
+ * in the file, -1 padded), sizeList (nElem); any may be NULL. */
+int rf_unv_copy(const rf_unv_mesh* m, double* coords, int32_t* connect, int32_t* sizeList);
+int rf_unv_destroy(rf_unv_mesh* m);
 /* Subtract each realization's mean and divide by its RMS (FP64 sums). */
 int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N,
                         int32_t Nmc, int32_t dtype, void* stream);

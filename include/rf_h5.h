@@ -56,6 +56,22 @@ int rf_h5_write_xmf(const char* xmfPath, const char* h5FileName, const char* h5T
                     int32_t nDim, const double* xMin, const double* xMax, const double* xStep,
                     const char* dsetName);
 
+/* The read side of interpolateToMesh (sample_RF.f90:906-975): the file
+ * attributes nDim, Nmc, xMinGlob, xMaxGlob, xStep, the global grid
+ * xNStep = nint((xMaxGlob - xMinGlob)/xStep) + 1 and the bounding box of
+ * [xMinLoc, xMaxLoc]: minPos = floor((xMinLoc - xMinGlob)/xStep) + 1,
+ * maxPos = ceiling((xMaxLoc - xMinGlob)/xStep) + 1, clamped to the grid. */
+typedef struct rf_h5_box {
+    int32_t nDim, Nmc;
+    double  xMinGlob[3], xMaxGlob[3], xStep[3];
+    int32_t xNStep[3], minPos[3], maxPos[3];
+} rf_h5_box;
+int rf_h5_read_box_info(const char* h5Path, int32_t nDim, const double* xMinLoc,
+                        const double* xMaxLoc, rf_h5_box* box);
+/* The box minPos..maxPos of "samples" into out (prod(maxPos - minPos + 1)
+ * doubles, x fastest). */
+int rf_h5_read_box(const char* h5Path, const rf_h5_box* box, double* out);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* rf_h5_last_error(void);
 

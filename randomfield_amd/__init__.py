@@ -30,7 +30,8 @@ from ._capi import (  # noqa: F401
 __all__ = [
     "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate",
     "create_random_field", "multivariate_transformation", "reference_seed", "reference_draws",
-    "blend", "BlendPlan", "GridBatch", "Graph", "stats", "corr", "interpolate", "normalize",
+    "blend", "BlendPlan", "GridBatch", "Graph", "stats", "corr", "interpolate", "interpolate_box", "read_unv",
+    "normalize",
     "device_uniforms", "RFError", "MFMA_VARIANTS",
 ]
 
@@ -374,6 +375,38 @@ def interpolate(ctx: Context, nstep, xMin, xStep, Nmc: int, d_grid, d_nodes, d_o
                                       _dtype_code(d_grid.dtype), _ptr(d_nodes), int(d_nodes.shape[0]), _ptr(d_out),
                                       _dtype_code(d_out.dtype), _ptr(stream), C.byref(flag)), ctx.handle)
     return bool(flag.value)
+
+
+def interpolate_box(ctx: Context, minPos, maxPos, xMinGlob, xStep, Nmc: int, d_box, d_nodes, d_out,
+                    stream=None) -> bool:
+    """rf_interpolate_box_device: d_box holds global grid points minPos..maxPos
+    (1-based) of the grid starting at xMinGlob; as interpolate otherwise."""
+    lo = np.ascontiguousarray(minPos, dtype=np.int32)
+    hi = np.ascontiguousarray(maxPos, dtype=np.int32)
+    flag = C.c_int32(0)
+    check(lib().rf_interpolate_box_device(ctx.handle, len(lo), _ptr(lo), _ptr(hi),
+                                          _ptr(np.asarray(xMinGlob, np.float64)), _ptr(np.asarray(xStep, np.float64)),
+                                          Nmc, _ptr(d_box), _dtype_code(d_box.dtype), _ptr(d_nodes),
+                                          int(d_nodes.shape[0]), _ptr(d_out), _dtype_code(d_out.dtype), _ptr(stream),
+                                          C.byref(flag)), ctx.handle)
+    return bool(flag.value)
+
+
+def read_unv(path, nDim: int):
+    """readUNV on one rank: (coordList (nNodes, nDim), connectList (nElem,
+    maxConnect) with -1 padding, sizeList (nElem,), monoType)."""
+    h = C.c_void_p()
+    check(lib().rf_unv_read(str(path).encode(), nDim, C.byref(h)))
+    try:
+        nn, ne, mc, mono = C.c_int64(0), C.c_int64(0), C.c_int32(0), C.c_int32(0)
+        check(lib().rf_unv_sizes(h, C.byref(nn), C.byref(ne), C.byref(mc), C.byref(mono)))
+        coords = np.zeros((nn.value, nDim))
+        conn = np.zeros((ne.value, mc.value), dtype=np.int32)
+        sizes = np.zeros(ne.value, dtype=np.int32)
+        check(lib().rf_unv_copy(h, _ptr(coords), _ptr(conn), _ptr(sizes)))
+    finally:
+        lib().rf_unv_destroy(h)
+    return coords, conn, sizes, bool(mono.value)
 
 
 def normalize(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int, stream=None) -> None:

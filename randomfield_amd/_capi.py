@@ -27,7 +27,8 @@ EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
     "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
-    "rf_stats_device", "rf_corr_device", "rf_interpolate_device", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
+    "rf_stats_device", "rf_corr_device", "rf_interpolate_device", "rf_interpolate_box_device", "rf_unv_read", "rf_unv_sizes", "rf_unv_copy",
+    "rf_unv_destroy", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
     "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_read",
     "rf_server_release", "rf_server_get_stats",
@@ -142,6 +143,11 @@ def lib() -> C.CDLL:
     L.rf_normalize_device.argtypes = [vp, vp, i64, i32, i32, vp]
     L.rf_corr_device.argtypes = [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp]
     L.rf_interpolate_device.argtypes = [vp, i32, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
+    L.rf_interpolate_box_device.argtypes = [vp, i32, vp, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
+    L.rf_unv_read.argtypes = [C.c_char_p, i32, vp]
+    L.rf_unv_sizes.argtypes = [vp, vp, vp, vp, vp]
+    L.rf_unv_copy.argtypes = [vp, vp, vp, vp]
+    L.rf_unv_destroy.argtypes = [vp]
     L.rf_device_uniforms.argtypes = [vp, C.c_uint64, C.c_uint32, i64, vp, vp]
     L.rf_uniform_host.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
     L.rf_uniform_host.restype = C.c_double

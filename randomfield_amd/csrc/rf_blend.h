@@ -68,9 +68,9 @@ hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, 
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s);
 int rf_moments_parts(int64_t N);
 int rf_corr_parts(int64_t N);
-hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const double* xMin, const double* xStep, int Nmc,
-                            const void* grid, int gridDtype, const double* x, int64_t nNodes, void* out,
-                            int outDtype, int* d_flag, hipStream_t s);
+hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const int32_t* lo, const double* xMin,
+                            const double* xStep, int Nmc, const void* grid, int gridDtype, const double* x,
+                            int64_t nNodes, void* out, int outDtype, int* d_flag, hipStream_t s);
 hipError_t rf_launch_corr(const void* u, int dtype, int64_t N, int64_t strideA, int nA, int M, double avg,
                           double std, double* d_part, int nPart, double* d_R, hipStream_t s);
 hipError_t rf_launch_moments(const void* d, int64_t N, int Nmc, int dtype, double* d_part,

@@ -689,18 +689,14 @@ int rf_corr_device(rf_context* ctx, const void* d_field, int32_t dtype, int32_t 
     return RF_OK;
 }
 
-int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep, const double* xMin,
-                          const double* xStep, int32_t Nmc, const void* d_grid, int32_t gridDtype,
-                          const double* d_nodes, int64_t nNodes, void* d_out, int32_t outDtype, void* stream,
-                          int32_t* outside) {
-    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
-    if (nDim < 1 || nDim > 3 || !nstep || !xMin || !xStep || Nmc < 1 || nNodes < 0)
-        return rf_set_error(ctx, RF_E_ARG, "bad interpolation grid");
-    if (nNodes > 0 && (!d_grid || !d_nodes || !d_out)) return rf_set_error(ctx, RF_E_ARG, "NULL device buffer");
-    if ((gridDtype != RF_DTYPE_F64 && gridDtype != RF_DTYPE_F32) || (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32))
-        return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
-    for (int a = 0; a < nDim; a++)
-        if (nstep[a] < 2 || !(xStep[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid needs >= 2 points and step > 0 per axis");
+}  // extern "C"
+
+namespace {
+
+int interpolate_common(rf_context* ctx, int32_t nDim, const int32_t* nstep, const int32_t* lo, const double* xMin,
+                       const double* xStep, int32_t Nmc, const void* d_grid, int32_t gridDtype,
+                       const double* d_nodes, int64_t nNodes, void* d_out, int32_t outDtype, void* stream,
+                       int32_t* outside) {
     if (outside) *outside = 0;
     if (nNodes == 0) return RF_OK;
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -709,7 +705,7 @@ int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep, c
     if (rc) return rc;
     int* flag = static_cast<int*>(ctx->d_scratch);
     RF_HIP_TRY(ctx, hipMemsetAsync(flag, 0, sizeof(int), s));
-    RF_HIP_TRY(ctx, rf_launch_interp(nDim, nstep, xMin, xStep, Nmc, d_grid, gridDtype, d_nodes, nNodes, d_out,
+    RF_HIP_TRY(ctx, rf_launch_interp(nDim, nstep, lo, xMin, xStep, Nmc, d_grid, gridDtype, d_nodes, nNodes, d_out,
                                      outDtype, flag, s));
     if (outside) {
         int h = 0;
@@ -720,6 +716,53 @@ int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep, c
         RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // the flag word is context scratch
     }
     return RF_OK;
+}
+
+int interpolate_args(rf_context* ctx, int32_t nDim, const double* xMin, const double* xStep, int32_t Nmc,
+                     const void* d_grid, int32_t gridDtype, const double* d_nodes, int64_t nNodes, void* d_out,
+                     int32_t outDtype) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (nDim < 1 || nDim > 3 || !xMin || !xStep || Nmc < 1 || nNodes < 0)
+        return rf_set_error(ctx, RF_E_ARG, "bad interpolation grid");
+    if (nNodes > 0 && (!d_grid || !d_nodes || !d_out)) return rf_set_error(ctx, RF_E_ARG, "NULL device buffer");
+    if ((gridDtype != RF_DTYPE_F64 && gridDtype != RF_DTYPE_F32) || (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32))
+        return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    for (int a = 0; a < nDim; a++)
+        if (!(xStep[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid step must be > 0 per axis");
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rf_interpolate_device(rf_context* ctx, int32_t nDim, const int32_t* nstep, const double* xMin,
+                          const double* xStep, int32_t Nmc, const void* d_grid, int32_t gridDtype,
+                          const double* d_nodes, int64_t nNodes, void* d_out, int32_t outDtype, void* stream,
+                          int32_t* outside) {
+    int rc = interpolate_args(ctx, nDim, xMin, xStep, Nmc, d_grid, gridDtype, d_nodes, nNodes, d_out, outDtype);
+    if (rc) return rc;
+    if (!nstep) return rf_set_error(ctx, RF_E_ARG, "bad interpolation grid");
+    for (int a = 0; a < nDim; a++)
+        if (nstep[a] < 2) return rf_set_error(ctx, RF_E_ARG, "grid needs >= 2 points and step > 0 per axis");
+    return interpolate_common(ctx, nDim, nstep, nullptr, xMin, xStep, Nmc, d_grid, gridDtype, d_nodes, nNodes,
+                              d_out, outDtype, stream, outside);
+}
+
+int rf_interpolate_box_device(rf_context* ctx, int32_t nDim, const int32_t* minPos, const int32_t* maxPos,
+                              const double* xMinGlob, const double* xStep, int32_t Nmc, const void* d_box,
+                              int32_t gridDtype, const double* d_nodes, int64_t nNodes, void* d_out,
+                              int32_t outDtype, void* stream, int32_t* outside) {
+    int rc = interpolate_args(ctx, nDim, xMinGlob, xStep, Nmc, d_box, gridDtype, d_nodes, nNodes, d_out, outDtype);
+    if (rc) return rc;
+    if (!minPos || !maxPos) return rf_set_error(ctx, RF_E_ARG, "bad interpolation box");
+    int32_t ext[3] = {1, 1, 1};
+    for (int a = 0; a < nDim; a++) {
+        if (minPos[a] < 1 || maxPos[a] < minPos[a]) return rf_set_error(ctx, RF_E_ARG, "bad interpolation box");
+        ext[a] = maxPos[a] - minPos[a] + 1;
+    }
+    return interpolate_common(ctx, nDim, ext, minPos, xMinGlob, xStep, Nmc, d_box, gridDtype, d_nodes, nNodes,
+                              d_out, outDtype, stream, outside);
 }
 
 int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N, int32_t Nmc, int32_t dtype,

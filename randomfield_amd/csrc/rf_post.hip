@@ -160,9 +160,14 @@ __global__ void k_corr_final(const double* __restrict__ part, int nPart, int M, 
 // the upper border (:1024), corners in the reference's order (:996-1013),
 // weight prod(1 - |c - corner|), sum in corner order (:1072-1086).  Nodes
 // outside the grid get NaN and raise *flag.
+// The grid in memory may be a box of a larger grid: global indices lo..hi
+// (1-based, hi = lo + nstep - 1), positions still measured from the larger
+// grid's xMin, and hi is the border of the cell rule, as the reference reads
+// the bounding box of the nodes (:936-975).
 struct rf_interp_dev {
     int nDim, Nmc;
     int nstep[3];
+    int lo[3], hi[3];
     double xMin[3], xStep[3];
     int64_t Ng, nNodes;
 };
@@ -182,8 +187,8 @@ __global__ __launch_bounds__(kBlock) void k_interp(rf_interp_dev q, const TG* __
     for (int a = 0; a < q.nDim; a++) {
         c[a] = (x[i * q.nDim + a] - q.xMin[a]) / q.xStep[a] + 1.0;
         ci[a] = (int64_t)floor(c[a]);
-        if (ci[a] == q.nstep[a]) ci[a] -= 1;
-        if (ci[a] < 1 || ci[a] + 1 > q.nstep[a]) inside = false;
+        if (ci[a] == q.hi[a]) ci[a] -= 1;
+        if (ci[a] < q.lo[a] || ci[a] + 1 > q.hi[a]) inside = false;
     }
     if (!inside) {
         flag[0] = 1;
@@ -207,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(rf_interp_dev q, const TG* __
         for (int a = 0; a < q.nDim; a++) {
             const int off = q.nDim == 3 ? o3[j][a] : (q.nDim == 2 ? o2[j][a] : j);
             wj = wj * (1.0 - fabs(c[a] - (double)(ci[a] + off)));
-            id += (ci[a] + off - 1) * (a == 0 ? 1 : (a == 1 ? s1 : s2));
+            id += (ci[a] + off - q.lo[a]) * (a == 0 ? 1 : (a == 1 ? s1 : s2));
         }
         w[j] = wj;
         idx[j] = id;
@@ -473,9 +478,9 @@ hipError_t rf_launch_corr(const void* u, int dtype, int64_t N, int64_t strideA, 
     return hipGetLastError();
 }
 
-hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const double* xMin, const double* xStep, int Nmc,
-                            const void* grid, int gridDtype, const double* x, int64_t nNodes, void* out,
-                            int outDtype, int* d_flag, hipStream_t s) {
+hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const int32_t* lo, const double* xMin,
+                            const double* xStep, int Nmc, const void* grid, int gridDtype, const double* x,
+                            int64_t nNodes, void* out, int outDtype, int* d_flag, hipStream_t s) {
     if (nNodes <= 0) return hipSuccess;
     rf_interp_dev q{};
     q.nDim = nDim;
@@ -483,6 +488,8 @@ hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const double* xMin, 
     q.Ng = 1;
     for (int a = 0; a < 3; a++) {
         q.nstep[a] = a < nDim ? nstep[a] : 1;
+        q.lo[a] = (a < nDim && lo) ? lo[a] : 1;
+        q.hi[a] = q.lo[a] + q.nstep[a] - 1;
         q.xMin[a] = a < nDim ? xMin[a] : 0.0;
         q.xStep[a] = a < nDim ? xStep[a] : 1.0;
         q.Ng *= q.nstep[a];

@@ -232,3 +232,83 @@ extern "C" int rf_h5_write_attributes(const char* h5Path, const rf_h5_attrs* a) 
     }
     return RF_H5_OK;
 }
+
+namespace {
+
+int attr_read(hid_t file, const char* name, hid_t mtype, size_t n, void* data) {
+    if (H5Aexists(file, name) <= 0) return fail(RF_H5_E_IO, std::string("attribute ") + name + " missing");
+    H5Id at(H5Aopen(file, name, H5P_DEFAULT), H5Aclose);
+    if (at.id < 0) return fail(RF_H5_E_IO, std::string("cannot open attribute ") + name);
+    H5Id sp(H5Aget_space(at.id), H5Sclose);
+    const hssize_t cnt = sp.id < 0 ? -1 : H5Sget_simple_extent_npoints(sp.id);
+    if (cnt < 0 || (size_t)cnt < n) return fail(RF_H5_E_IO, std::string("attribute ") + name + " too short");
+    std::vector<char> buf((size_t)cnt * H5Tget_size(mtype));
+    if (H5Aread(at.id, mtype, buf.data()) < 0) return fail(RF_H5_E_IO, std::string("cannot read attribute ") + name);
+    std::memcpy(data, buf.data(), n * H5Tget_size(mtype));
+    return RF_H5_OK;
+}
+
+}  // namespace
+
+extern "C" int rf_h5_read_box_info(const char* h5Path, int32_t nDim, const double* xMinLoc, const double* xMaxLoc,
+                                   rf_h5_box* box) {
+    g_err.clear();
+    if (!h5Path || !xMinLoc || !xMaxLoc || !box) return fail(RF_H5_E_ARG, "NULL argument");
+    if (nDim < 1 || nDim > 3) return fail(RF_H5_E_ARG, "nDim must be 1, 2 or 3");
+    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    H5Id file(H5Fopen(h5Path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+    if (file.id < 0) return fail(RF_H5_E_IO, std::string("cannot open ") + h5Path);
+    rf_h5_box b{};
+    int rc = attr_read(file.id, "nDim", H5T_NATIVE_INT32, 1, &b.nDim);
+    if (!rc) rc = attr_read(file.id, "Nmc", H5T_NATIVE_INT32, 1, &b.Nmc);
+    if (rc) return rc;
+    if (b.nDim != nDim) return fail(RF_H5_E_ARG, "nDim differs from the file's attribute");
+    rc = attr_read(file.id, "xMinGlob", H5T_NATIVE_DOUBLE, (size_t)nDim, b.xMinGlob);
+    if (!rc) rc = attr_read(file.id, "xMaxGlob", H5T_NATIVE_DOUBLE, (size_t)nDim, b.xMaxGlob);
+    if (!rc) rc = attr_read(file.id, "xStep", H5T_NATIVE_DOUBLE, (size_t)nDim, b.xStep);
+    if (rc) return rc;
+    for (int i = 0; i < nDim; i++) {
+        if (!(b.xStep[i] > 0.0)) return fail(RF_H5_E_IO, "xStep attribute must be > 0");
+        b.xNStep[i] = (int32_t)std::llround((b.xMaxGlob[i] - b.xMinGlob[i]) / b.xStep[i]) + 1;
+        int64_t mn = (int64_t)std::floor((xMinLoc[i] - b.xMinGlob[i]) / b.xStep[i]) + 1;
+        int64_t mx = (int64_t)std::ceil((xMaxLoc[i] - b.xMinGlob[i]) / b.xStep[i]) + 1;
+        if (mn < 1) mn = 1;
+        if (mx > b.xNStep[i]) mx = b.xNStep[i];
+        if (mx < mn) return fail(RF_H5_E_ARG, "the nodes lie outside the grid");
+        b.minPos[i] = (int32_t)mn;
+        b.maxPos[i] = (int32_t)mx;
+    }
+    *box = b;
+    return RF_H5_OK;
+}
+
+extern "C" int rf_h5_read_box(const char* h5Path, const rf_h5_box* box, double* out) {
+    g_err.clear();
+    if (!h5Path || !box || !out) return fail(RF_H5_E_ARG, "NULL argument");
+    const int nDim = box->nDim;
+    if (nDim < 1 || nDim > 3) return fail(RF_H5_E_ARG, "nDim must be 1, 2 or 3");
+    hsize_t offset[3], count[3];
+    for (int i = 0; i < nDim; i++) {
+        if (box->minPos[i] < 1 || box->maxPos[i] < box->minPos[i]) return fail(RF_H5_E_ARG, "bad box");
+        offset[nDim - 1 - i] = (hsize_t)(box->minPos[i] - 1);
+        count[nDim - 1 - i] = (hsize_t)(box->maxPos[i] - box->minPos[i] + 1);
+    }
+    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    H5Id file(H5Fopen(h5Path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+    if (file.id < 0) return fail(RF_H5_E_IO, std::string("cannot open ") + h5Path);
+    H5Id dset(H5Dopen2(file.id, "samples", H5P_DEFAULT), H5Dclose);
+    if (dset.id < 0) return fail(RF_H5_E_IO, "dataset samples missing");
+    H5Id fspace(H5Dget_space(dset.id), H5Sclose);
+    if (fspace.id < 0 || H5Sget_simple_extent_ndims(fspace.id) != nDim)
+        return fail(RF_H5_E_IO, "samples has another rank");
+    hsize_t dims[3];
+    H5Sget_simple_extent_dims(fspace.id, dims, nullptr);
+    for (int c = 0; c < nDim; c++)
+        if (offset[c] + count[c] > dims[c]) return fail(RF_H5_E_ARG, "box outside the samples dataset");
+    H5Id mspace(H5Screate_simple(nDim, count, nullptr), H5Sclose);
+    if (mspace.id < 0 || H5Sselect_hyperslab(fspace.id, H5S_SELECT_SET, offset, nullptr, count, nullptr) < 0)
+        return fail(RF_H5_E_IO, "hyperslab of samples");
+    if (H5Dread(dset.id, H5T_NATIVE_DOUBLE, mspace.id, fspace.id, H5P_DEFAULT, out) < 0)
+        return fail(RF_H5_E_IO, "cannot read samples");
+    return RF_H5_OK;
+}

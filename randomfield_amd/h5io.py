@@ -26,6 +26,13 @@ class H5Attrs(C.Structure):
     ]
 
 
+class H5Box(C.Structure):
+    _fields_ = [
+        ("nDim", C.c_int32), ("Nmc", C.c_int32), ("xMinGlob", C.c_double * 3), ("xMaxGlob", C.c_double * 3),
+        ("xStep", C.c_double * 3), ("xNStep", C.c_int32 * 3), ("minPos", C.c_int32 * 3), ("maxPos", C.c_int32 * 3),
+    ]
+
+
 class H5Error(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"rf_h5 error {code}: {msg}")
@@ -46,6 +53,8 @@ def lib() -> C.CDLL:
                                          C.c_char_p, C.c_char_p, ip]
         L.rf_h5_write_attributes.argtypes = [C.c_char_p, C.POINTER(H5Attrs)]
         L.rf_h5_write_xmf.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, ip, vp, vp, vp, C.c_char_p]
+        L.rf_h5_read_box_info.argtypes = [C.c_char_p, ip, vp, vp, C.POINTER(H5Box)]
+        L.rf_h5_read_box.argtypes = [C.c_char_p, C.POINTER(H5Box), vp]
         L.rf_h5_last_error.restype = C.c_char_p
         _L = L
     return _L
@@ -100,3 +109,44 @@ def write_attributes(h5path, *, nDim, Nmc, method, seedStart, corrMod, margiFirs
     for fld, v in (("kNStep", kNStep or [0] * nDim), ("nFields", nFields or [1] * nDim)):
         setattr(a, fld, (C.c_int32 * 3)(*(list(v)[:nDim] + [0] * (3 - nDim))))
     _check(lib().rf_h5_write_attributes(str(h5path).encode(), C.byref(a)))
+
+
+def read_box_info(h5path, nDim: int, xMinLoc, xMaxLoc) -> H5Box:
+    """Attributes and the bounding box of [xMinLoc, xMaxLoc] (interpolateToMesh's read)."""
+    b = H5Box()
+    lo, hi = _f64(xMinLoc, nDim), _f64(xMaxLoc, nDim)
+    _check(lib().rf_h5_read_box_info(str(h5path).encode(), nDim, lo.ctypes.data, hi.ctypes.data, C.byref(b)))
+    return b
+
+
+def read_box(h5path, box: H5Box) -> np.ndarray:
+    """The box of the samples dataset, x fastest (flat)."""
+    n = int(np.prod([box.maxPos[i] - box.minPos[i] + 1 for i in range(box.nDim)]))
+    out = np.zeros(n)
+    _check(lib().rf_h5_read_box(str(h5path).encode(), C.byref(box), out.ctypes.data))
+    return out
+
+
+def interpolate_to_mesh(ctx, h5path, coordList, xMinLoc=None, xMaxLoc=None) -> np.ndarray:
+    """interpolateToMesh (sample_RF.f90:872-1106): the samples of the file at
+    the mesh nodes coordList (nNodes, nDim), read over the nodes' bounding
+    box and interpolated on the device in FP64.  Returns (nNodes,); nodes
+    outside the grid raise ValueError."""
+    import torch
+
+    from . import interpolate_box
+    x = np.ascontiguousarray(coordList, dtype=np.float64)
+    nDim = x.shape[1]
+    lo = x.min(axis=0) if xMinLoc is None else np.asarray(xMinLoc, np.float64)
+    hi = x.max(axis=0) if xMaxLoc is None else np.asarray(xMaxLoc, np.float64)
+    box = read_box_info(h5path, nDim, lo, hi)
+    data = read_box(h5path, box)
+    d_box = torch.from_numpy(data).cuda()
+    d_x = torch.from_numpy(x).cuda()
+    out = torch.empty((1, x.shape[0]), dtype=torch.float64, device="cuda")
+    mn = [box.minPos[i] for i in range(nDim)]
+    mx = [box.maxPos[i] for i in range(nDim)]
+    if interpolate_box(ctx, mn, mx, [box.xMinGlob[i] for i in range(nDim)], [box.xStep[i] for i in range(nDim)],
+                       1, d_box, d_x, out):
+        raise ValueError("mesh nodes outside the grid of " + str(h5path))
+    return out[0].cpu().numpy()

@@ -118,3 +118,68 @@ def test_interpolate_matches_oracle(ctx, nDim, gdt, odt):
     assert R.interpolate(ctx, nstep, xMin, xStep, Nmc, torch.from_numpy(grid).cuda(), torch.from_numpy(xo).cuda(),
                          out2)
     assert np.isnan(out2.cpu().numpy()[:, -1]).all()
+
+
+def test_interpolate_box_equals_full_grid(ctx):
+    """The bounding-box form (the reference reads only the nodes' box) gives
+    the bits of the whole-grid form for nodes inside the box, borders
+    included; nodes outside the box are flagged."""
+    import torch
+    rng = np.random.default_rng(11)
+    nstep, xMin, xStep, Nmc = (30, 22, 14), (-1.0, 2.0, 0.5), (0.25, 0.5, 0.75), 3
+    grid = rng.standard_normal((Nmc, int(np.prod(nstep))))
+    mn, mx = (4, 3, 2), (19, 15, 9)
+    lo = np.array([xMin[a] + (mn[a] - 1) * xStep[a] for a in range(3)])
+    hi = np.array([xMin[a] + (mx[a] - 1) * xStep[a] for a in range(3)])
+    x = rng.uniform(lo, hi, (5000, 3))
+    x[0], x[1] = hi, lo
+    x[2, 0] = hi[0]
+    G = grid.reshape(Nmc, nstep[2], nstep[1], nstep[0])
+    box = np.ascontiguousarray(G[:, mn[2] - 1:mx[2], mn[1] - 1:mx[1], mn[0] - 1:mx[0]].reshape(Nmc, -1))
+    d_x = torch.from_numpy(x).cuda()
+    full = torch.empty((Nmc, x.shape[0]), dtype=torch.float64, device="cuda")
+    part = torch.empty_like(full)
+    assert not R.interpolate(ctx, nstep, xMin, xStep, Nmc, torch.from_numpy(grid).cuda(), d_x, full)
+    assert not R.interpolate_box(ctx, mn, mx, xMin, xStep, Nmc, torch.from_numpy(box).cuda(), d_x, part)
+    assert np.array_equal(full.cpu().numpy(), part.cpu().numpy())
+    xo = torch.from_numpy(np.array([hi + 1.0])).cuda()
+    o = torch.empty((Nmc, 1), dtype=torch.float64, device="cuda")
+    assert R.interpolate_box(ctx, mn, mx, xMin, xStep, Nmc, torch.from_numpy(box).cuda(), xo, o)
+
+
+def test_interpolate_to_mesh_pipeline(ctx, tmp_path):
+    """Generation -> HDF5 samples + attributes -> UNV mesh -> interpolateToMesh
+    on the nodes' bounding box, against the oracle's interpolation of the
+    whole grid (FP64, bit for bit)."""
+    import torch
+    from randomfield_amd import h5io
+    if not h5io.LIB.exists():
+        pytest.skip("HDF5 C library absent")
+    nstep, step, x0 = (24, 18, 10), (0.25, 0.25, 0.5), (0.0, -1.0, 1.0)
+    p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=1, xRange=(6, 4.5, 4.5),
+                      seed=9)
+    plan = R.FieldPlan(ctx, p)
+    field = torch.empty((1, int(np.prod(nstep))), dtype=torch.float64, device="cuda")
+    plan.execute_grid(x0, step, nstep, field)
+    ctx.sync()
+    plan.close()
+    f = field[0].cpu().numpy()
+    h5, _ = h5io.write_simple(tmp_path, "gen", f, nstep, step, x0, nstep)
+    xmax = [x0[a] + (nstep[a] - 1) * step[a] for a in range(3)]
+    h5io.write_attributes(h5, nDim=3, Nmc=1, method=2, seedStart=9, corrMod=1, margiFirst=1, xMinGlob=x0,
+                          xMaxGlob=xmax, xStep=step, corrL=(1, 1, 1))
+    # a brick mesh inside the grid, written as a universal file
+    rng = np.random.default_rng(5)
+    nodes = rng.uniform([0.6, -0.4, 1.7], [4.1, 2.3, 4.2], (700, 3))
+    lines = ["    -1", "  2411"]
+    for i, c in enumerate(nodes):
+        lines += [f"{i + 1:10d}{1:10d}{1:10d}{11:10d}", "".join(f"{v:25.16E}".replace("E", "D") for v in c)]
+    lines += ["    -1", "    -1", "  2412", f"{1:10d}{115:10d}{2:10d}{1:10d}{7:10d}{8:10d}",
+              "".join(f"{v:10d}" for v in range(1, 9)), "    -1"]
+    unv = tmp_path / "mesh.unv"
+    unv.write_text("\n".join(lines) + "\n")
+    coords, *_ = R.read_unv(unv, 3)
+    got = h5io.interpolate_to_mesh(ctx, h5, coords)
+    ref, bad = O.interp(f[None, :], nstep, x0, step, coords)
+    assert bad == 0
+    assert np.array_equal(got, ref[0])

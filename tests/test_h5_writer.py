@@ -123,6 +123,31 @@ def test_xmf_text(tmp_path):
 def test_header_symbols_exported():
     hdr = (Path(__file__).resolve().parent.parent / "include" / "rf_h5.h").read_text()
     decl = set(re.findall(r"\b(rf_h5_[a-z0-9_]+)\s*\(", hdr))
-    assert decl == {"rf_h5_write_simple", "rf_h5_write_attributes", "rf_h5_write_xmf", "rf_h5_last_error"}
+    assert decl == {"rf_h5_write_simple", "rf_h5_write_attributes", "rf_h5_write_xmf", "rf_h5_last_error",
+                    "rf_h5_read_box_info", "rf_h5_read_box"}
     for name in decl:
         assert hasattr(h5io.lib(), name), name
+
+
+def test_read_box(tmp_path):
+    """interpolateToMesh's read: attributes, the bounding box (floor / ceiling
+    + 1, clamped to the grid) and the box values, x fastest."""
+    n, step, x0 = (11, 7, 5), (0.5, 0.25, 1.0), (-1.0, 0.0, 2.0)
+    f = np.random.default_rng(3).standard_normal(int(np.prod(n)))
+    h5, _ = h5io.write_simple(tmp_path, "b", f, n, step, x0, n)
+    xmax = [x0[i] + (n[i] - 1) * step[i] for i in range(3)]
+    h5io.write_attributes(h5, nDim=3, Nmc=1, method=2, seedStart=1, corrMod=1, margiFirst=1, xMinGlob=x0,
+                          xMaxGlob=xmax, xStep=step, corrL=(1, 1, 1))
+    lo, hi = (0.1, 0.3, 1.0), (2.0, 1.4, 9.0)
+    b = h5io.read_box_info(h5, 3, lo, hi)
+    assert b.nDim == 3 and b.Nmc == 1 and list(b.xNStep) == list(n)
+    mn = [int(np.floor((lo[i] - x0[i]) / step[i])) + 1 for i in range(3)]
+    mx = [min(int(np.ceil((hi[i] - x0[i]) / step[i])) + 1, n[i]) for i in range(3)]
+    mn = [max(v, 1) for v in mn]
+    assert list(b.minPos) == mn and list(b.maxPos) == mx and mn[2] == 1 and mx[2] == n[2]
+    got = h5io.read_box(h5, b)
+    F = f.reshape(n[2], n[1], n[0])
+    want = F[mn[2] - 1:mx[2], mn[1] - 1:mx[1], mn[0] - 1:mx[0]].reshape(-1)
+    assert np.array_equal(got, want)
+    with pytest.raises(h5io.H5Error):
+        h5io.read_box_info(h5, 2, lo[:2], hi[:2])          # nDim differs from the file

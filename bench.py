@@ -376,13 +376,62 @@ def run_cfg5(args):
                                "sample": "500 points of the first request of each method, all its realizations"})
 
 
+def run_cfg6(args):
+    """Method 4 (gen_Std_Gauss_FFT): 128^3 grid, spacing 0.1 corrL, Nmc 8,
+    DCT-III tables through the grid-form MFMA contractions; FP32 output;
+    realization 0 checked against the FP64 oracle.  CPU reference point:
+    scipy's FFT-based DCT-III of the same coefficients (the reference calls
+    FFTW), one realization."""
+    import numpy as np
+    import scipy.fft
+    import torch
+
+    import randomfield_amd as R
+    from oracle import oracle as O
+    gn, step, Nmc = (128, 128, 128), (0.1, 0.1, 0.1), 8
+    xr = [(n - 1) * s for n, s in zip(gn, step)]
+    kN = int(np.prod(gn))
+    ctx = R.Context(0)
+    sptr = torch.cuda.current_stream().cuda_stream
+    u = np.random.default_rng(4).random((Nmc, 3 * kN - 1))
+    p = R.make_params(nDim=3, method=R.FFT, mode=R.FIXED, form=R.FORM_FP32, Nmc=Nmc, xRange=xr, gridN=gn)
+    plan = R.FieldPlan(ctx, p, None, u)
+    out = torch.empty((Nmc, kN), dtype=torch.float32, device="cuda")
+    wall, ms = _timed(lambda: plan.execute_grid((0.0, 0.0, 0.0), step, gn, out, stream=sptr), args.steps,
+                      args.warmup)
+    po = O.fft_plan(3, xr, gn)
+    sk = O.skvec(po, O.kpoints(po))
+    ref = O.fft(po, sk, u[:1], step, 1)[0]
+    got = out[0].cpu().numpy().astype(np.float64)
+    coef = (u[0, :kN] - 0.5) * np.sqrt(sk) * np.cos(2.0 * O.PI_REF * u[0, 2 * kN - 1:])
+    t = time.perf_counter()
+    scipy.fft.dctn(coef.reshape(gn[::-1]), type=3, workers=_threads())
+    cpu_s = time.perf_counter() - t
+    plan.close()
+    ctx.close()
+    work = kN * Nmc
+    flop = Nmc * 8.0 * (gn[0] ** 2 * gn[1] * gn[2] + gn[0] * gn[1] ** 2 * gn[2] + 0.5 * gn[0] * gn[1] * gn[2] ** 2)
+    return _line("points x realizations / s (method 4, FFT/DCT-III grid)", work / (wall * 1e-3), args.steps,
+                 args.warmup, wall, "f32", "method 4: 128^3 grid spacing 0.1 corrL, Nmc 8, DCT-III per axis",
+                 kernel_ms=float(ms.mean()),
+                 grid_form={"bound": "mfma", "peak": 157.3, "unit": "TFLOP/s",
+                            "achieved": flop / (ms.mean() * 1e-3) / 1e12,
+                            "frac": flop / (ms.mean() * 1e-3) / 1e12 / 157.3,
+                            "note": "complex FP32 MFMA contractions (v_mfma_f32_32x32x2f32), 8 FLOP per complex MAC, "
+                                    "last stage real part only"},
+                 accuracy=_err(got, ref),
+                 cpu_reference={"value": kN / cpu_s, "unit": "points*realizations/s", "cores": _threads(),
+                                "kind": "scipy.fft.dctn type 3 (FFT-based, as FFTW REDFT01)",
+                                "seconds": cpu_s, "sample": "one realization"})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--direct-steps", type=int, default=2)
-    ap.add_argument("--cpu-points", type=int, default=3000)
+    ap.add_argument("--cpu-points", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
@@ -390,13 +439,14 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000, help="cfg 5: requests in the stream")
     ap.add_argument("--budget-gib", type=int, default=64, help="cfg 5: server memory budget")
     ap.add_argument("--config", type=int, default=3,
-                    help="benchmark configuration 1..5 (3, the headline workload, by default)")
+                    help="benchmark configuration 1..6 (3, the headline workload, by default; "
+                         "6 is method 4, the FFT method)")
     args = ap.parse_args()
     if args.config != 3 and not args.calibrate_copy:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
             return 2
-        runners = {1: run_cfg1, 2: run_cfg2, 4: run_cfg4, 5: run_cfg5}
+        runners = {1: run_cfg1, 2: run_cfg2, 4: run_cfg4, 5: run_cfg5, 6: run_cfg6}
         if args.config not in runners:
             print(f"unknown configuration {args.config}", file=sys.stderr)
             return 2

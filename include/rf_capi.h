@@ -182,6 +182,13 @@ int rf_plan_autotune(rf_field_plan* plan, const double* d_x, int64_t N,
 int rf_plan_execute_grid(rf_field_plan* plan, const double origin[3],
                          const double step[3], const int32_t nstep[3],
                          void* d_out, int32_t outDtype, void* stream);
+/* Host-buffer form of rf_plan_execute_grid (the Fortran facade's structured
+ * path; method 4 runs here, as gen_Std_Gauss_FFT does on the mesh's grid,
+ * randomFieldND.f90:520-763): plan, grid of nstep points from origin at step
+ * (caller units, x fastest), out N x Nmc; synchronous. */
+int rf_generate_grid(rf_context* ctx, const rf_params* p, const double* origin,
+                     const double* step, const int32_t* nstep, const double* phaseUniform,
+                     const uint8_t* calculate, void* out, int32_t outDtype);
 /* Host-buffer convenience: create, upload, execute, download, destroy.
  * x: nDim x N, out: N x Nmc (outDtype). */
 int rf_generate(rf_context* ctx, const rf_params* p, const double* x,

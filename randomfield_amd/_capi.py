@@ -26,7 +26,7 @@ MFMA_VARIANTS = 9
 EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
-    "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
+    "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_generate_grid", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
     "rf_stats_device", "rf_corr_device", "rf_interpolate_device", "rf_interpolate_box_device", "rf_unv_read", "rf_unv_sizes", "rf_unv_copy",
     "rf_unv_destroy", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
@@ -143,6 +143,7 @@ def lib() -> C.CDLL:
     L.rf_normalize_device.argtypes = [vp, vp, i64, i32, i32, vp]
     L.rf_corr_device.argtypes = [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp]
     L.rf_interpolate_device.argtypes = [vp, i32, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
+    L.rf_generate_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32]
     L.rf_interpolate_box_device.argtypes = [vp, i32, vp, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
     L.rf_unv_read.argtypes = [C.c_char_p, i32, vp]
     L.rf_unv_sizes.argtypes = [vp, vp, vp, vp, vp]

@@ -512,6 +512,32 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     return RF_OK;
 }
 
+int rf_generate_grid(rf_context* ctx, const rf_params* p, const double* origin, const double* step,
+                     const int32_t* nstep, const double* phaseUniform, const uint8_t* calculate, void* out,
+                     int32_t outDtype) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!p || !origin || !step || !nstep || !out) return rf_set_error(ctx, RF_E_ARG, "NULL grid argument");
+    if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
+    int64_t N = 1;
+    for (int a = 0; a < p->nDim && a < 3; a++) N *= nstep[a] > 0 ? nstep[a] : 0;
+    if (N < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
+    rf_field_plan* P = nullptr;
+    int rc = rf_plan_create(ctx, p, nullptr, phaseUniform, calculate, &P);
+    if (rc) return rc;
+    std::unique_ptr<rf_field_plan, int (*)(rf_field_plan*)> guard(P, rf_plan_destroy);
+    const size_t bytes = (size_t)N * p->Nmc * (outDtype == RF_DTYPE_F64 ? 8 : 4);
+    void* d_out = nullptr;
+    if (hipMalloc(&d_out, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "output allocation failed");
+    rc = rf_plan_execute_grid(P, origin, step, nstep, d_out, outDtype, ctx->stream);
+    hipError_t e = hipSuccess;
+    if (rc == RF_OK) e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    if (rc) return rc;
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("rf_generate_grid: ") + hipGetErrorString(e));
+    return RF_OK;
+}
+
 int rf_generate(rf_context* ctx, const rf_params* p, const double* x, int64_t N,
                 const double* kUniform, const double* phaseUniform, const uint8_t* calculate,
                 void* out, int32_t outDtype) {

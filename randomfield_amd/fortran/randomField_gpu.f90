@@ -139,6 +139,16 @@ module randomField_gpu
             integer(c_int64_t), value :: N
             integer(c_int32_t), value :: dtype
         end function
+        integer(c_int) function c_rf_generate_grid(ctx, p, origin, step, nstep, phU, calc, out, dtype) &
+                bind(C, name="rf_generate_grid")
+            import :: c_int, c_int32_t, c_double, c_ptr, rf_params_c
+            type(c_ptr), value :: ctx
+            type(rf_params_c), intent(in) :: p
+            real(c_double), intent(in) :: origin(*), step(*)
+            integer(c_int32_t), intent(in) :: nstep(*)
+            type(c_ptr), value :: phU, calc, out
+            integer(c_int32_t), value :: dtype
+        end function
         integer(c_int) function c_rf_marginal_host(ctx, margi, avg, var, field, count) &
                 bind(C, name="rf_marginal_host")
             import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
@@ -436,6 +446,11 @@ contains
         MSH%xNTotal = N
 
         p = rf_c_params(RDF)
+        if (RDF%method == FFT) then
+            do i = 1, RDF%nDim
+                p%gridN(i) = MSH%xNStep(i)
+            end do
+        end if
         rc = c_rf_plan_spectrum(p, info)
         call rf_check(rc, "rf_plan_spectrum")
 
@@ -453,7 +468,30 @@ contains
             phU_p = c_loc(phU)
         end if
 
-        if (is_contiguous(RDF%xPoints) .and. is_contiguous(RDF%randField)) then
+        if (RDF%method == FFT) then
+            ! gen_Std_Gauss_FFT (randomFieldND.f90:520-763): the whole structured
+            ! mesh MSH%xNStep from MSH%xMinGlob at MSH%xStep, x fastest
+            if (.not. allocated(MSH%xStep) .or. .not. allocated(MSH%xMinGlob)) &
+                call fail("method 4 needs MSH%xStep and MSH%xMinGlob")
+            block
+                real(c_double) :: org(3), stp(3)
+                integer(c_int32_t) :: ns(3)
+                double precision, allocatable, target :: fs(:,:)
+                org = 0d0
+                stp = 1d0
+                ns = 1
+                do i = 1, RDF%nDim
+                    org(i) = MSH%xMinGlob(i)
+                    stp(i) = MSH%xStep(i)
+                    ns(i) = MSH%xNStep(i)
+                end do
+                allocate(fs(N, RDF%Nmc))
+                rc = c_rf_generate_grid(rf_ctx(RDF%device), p, org, stp, ns, phU_p, c_loc(calc), c_loc(fs), &
+                                        0_c_int32_t)
+                if (rc == 0) RDF%randField = fs
+            end block
+            call rf_check(rc, "rf_generate_grid")
+        else if (is_contiguous(RDF%xPoints) .and. is_contiguous(RDF%randField)) then
             rc = c_rf_generate(rf_ctx(RDF%device), p, c_loc(RDF%xPoints), int(N, c_int64_t), &
                                kU_p, phU_p, c_loc(calc), c_loc(RDF%randField), 0_c_int32_t)
         else
@@ -467,7 +505,7 @@ contains
                 if (rc == 0) RDF%randField = fs
             end block
         end if
-        call rf_check(rc, "rf_generate")
+        if (RDF%method /= FFT) call rf_check(rc, "rf_generate")
 
         ! spectral state the reference leaves in RDF (spectra_RF.f90:68-133)
         RDF%kNTotal = info%kNTotal

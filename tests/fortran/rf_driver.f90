@@ -9,6 +9,8 @@
 !>           calculate(2) = .false.
 !>   case 4: isotropic, 3-D, FIXED, FP64, 300 points, Nmc 2, seed 7
 !>   case 5: createStandardGaussianFieldUnstruct, lognormal, FIXED FP64
+!>   case 6: method 4 (FFT), 3-D grid 12 x 10 x 8 at steps (0.25, 0.3, 0.4)
+!>           from (1, -2, 0.5), corrL (1, 1.5, 0.8), Nmc 2, seed 5
 !> out.bin (stream): int32 N, int32 Nmc, int32 nDim, int64 kNTotal,
 !>   xPoints(nDim,N), randField(N,Nmc), kPoints(rows,kNTotal), SkVec(kNTotal),
 !>   int32 seed(1)
@@ -29,6 +31,7 @@ program rf_driver
     select case (icase)
         case (1, 2, 5); nDim = 3; N = 500; Nmc = 3
         case (3);       nDim = 2; N = 400; Nmc = 2
+        case (6);       nDim = 3; N = 12 * 10 * 8; Nmc = 2
         case default;   nDim = 3; N = 300; Nmc = 2
     end select
     allocate(x(nDim, N), field(N, Nmc))
@@ -38,6 +41,15 @@ program rf_driver
         x(2, i) = 10d0 * modulo(0.5698402909980532d0 * i, 1d0)
         if (nDim == 3) x(3, i) = 10d0 * modulo(0.3819660112501051d0 * i, 1d0)
     end do
+
+    if (icase == 6) then
+        ! the grid points themselves, x fastest
+        do i = 1, N
+            x(1, i) = 1d0 + 0.25d0 * modulo(i - 1, 12)
+            x(2, i) = -2d0 + 0.3d0 * modulo((i - 1) / 12, 10)
+            x(3, i) = 0.5d0 + 0.4d0 * ((i - 1) / 120)
+        end do
+    end if
 
     if (icase == 5) then
         call createStandardGaussianFieldUnstruct(x, "gaussian", "lognormal", [1d0, 1.5d0, 0.8d0], &
@@ -59,6 +71,10 @@ program rf_driver
             call init_RF(RDF, nDim, Nmc, RANDOMIZATION, cm_GAUSSIAN, fom_GAUSSIAN, [1d0, 2d0], 0d0, 1d0)
             call calculate_random_seed(seed, 11)
             RDF%calculate(2) = .false.
+        case (6)
+            call init_RF(RDF, nDim, Nmc, FFT, cm_GAUSSIAN, fom_GAUSSIAN, [1d0, 1.5d0, 0.8d0], 0d0, 1d0)
+            call calculate_random_seed(seed, 5)
+            RDF%compatMode = RF_FIXED
         case default
             call init_RF(RDF, nDim, Nmc, ISOTROPIC, cm_GAUSSIAN, fom_GAUSSIAN, [0.5d0, 0.5d0, 0.5d0], 0d0, 1d0)
             call calculate_random_seed(seed, 7)
@@ -69,6 +85,12 @@ program rf_driver
     MSH%xNStep = 1
     MSH%xNStep(1) = N
     RDF%xRange = 10d0
+    if (icase == 6) then
+        MSH%xNStep = [12, 10, 8]
+        MSH%xStep = [0.25d0, 0.3d0, 0.4d0]
+        MSH%xMinGlob = [1d0, -2d0, 0.5d0]
+        RDF%xRange = (MSH%xNStep - 1) * MSH%xStep
+    end if
     RDF%xPoints => x
     RDF%randField => field
     call createRandomField(RDF, MSH)

@@ -287,7 +287,7 @@ def test_stats_and_normalize(ctx):
 
 
 # ------------------------------------------------------------ Fortran host
-@pytest.mark.parametrize("case", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("case", [1, 2, 3, 4, 5, 6])
 def test_fortran_driver(tmp_path, case):
     exe = ROOT / "tests" / "fortran" / "bin" / "rf_driver"
     out = tmp_path / f"case{case}.bin"
@@ -310,6 +310,21 @@ def test_fortran_driver(tmp_path, case):
         k = O.kpoints(po)
         ref = O.marginal(2, 1.0, 0.25, O.shinozuka(po, x / corrL, k, O.skvec(po, k), phU))
         assert report("fortran createStandardGaussianFieldUnstruct", field, ref) <= 1e-12
+        return
+    if case == 6:
+        # method 4 through createRandomField on the structured mesh: draws of
+        # the facade (random_number after the seed of seedStart 5), the oracle's
+        # DCT on the normalised steps
+        gn, step, corrL = (12, 10, 8), np.array([0.25, 0.3, 0.4]), np.array([1.0, 1.5, 0.8])
+        xr = (np.array(gn) - 1) * step
+        p = R.make_params(nDim=3, method=R.FFT, mode=R.FIXED, Nmc=Nmc, corrL=corrL, xRange=tuple(xr), gridN=gn)
+        _, U = R.reference_draws(p, R.reference_seed(5))
+        po = O.fft_plan(3, list(xr / corrL), gn)
+        kt = O.kpoints(po)
+        kP = np.frombuffer(raw[off: off + 8 * 3 * kN], np.float64).reshape(kN, 3)
+        assert kN == int(np.prod(gn)) and np.array_equal(kP, kt)
+        ref = O.fft(po, O.skvec(po, kt), U, list(step / corrL), Nmc)
+        assert report("fortran driver case 6 (method 4)", field, ref) <= 1e-5
         return
     rows = 1 if case == 4 else nDim
     kP = np.frombuffer(raw[off: off + 8 * rows * kN], np.float64).reshape(kN, rows)

@@ -425,6 +425,18 @@ def run_cfg6(args):
                                 "seconds": cpu_s, "sample": "one realization"})
 
 
+def measured_peaks():
+    """Runs lib/rf_peaks (a child process) and returns its JSON, or None."""
+    exe = ROOT / "randomfield_amd" / "lib" / "rf_peaks"
+    if not exe.exists():
+        return None
+    import subprocess
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,10 +450,14 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=600.0, help="cfg 5: arrivals per second")
     ap.add_argument("--requests", type=int, default=2000, help="cfg 5: requests in the stream")
     ap.add_argument("--budget-gib", type=int, default=64, help="cfg 5: server memory budget")
+    ap.add_argument("--peaks", action="store_true", help="only print the measured unit ceilings (JSON)")
     ap.add_argument("--config", type=int, default=3,
                     help="benchmark configuration 1..6 (3, the headline workload, by default; "
                          "6 is method 4, the FFT method)")
     args = ap.parse_args()
+    if args.peaks:
+        print(json.dumps(measured_peaks()), flush=True)
+        return 0
     if args.config != 3 and not args.calibrate_copy:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
@@ -579,6 +595,15 @@ def main() -> int:
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
 
+    # the f16 MFMA ceiling measured on this device with operands that change
+    # every instruction (the datasheet peak assumes a clock the chip does not
+    # hold under that load); executed work = 3 x algorithmic (hi/lo split)
+    ceiling = None
+    pk = measured_peaks()
+    if pk is not None:
+        ceiling = {"mfma_f16_toggling_TFLOPs": pk["mfma_f16_toggling_TFLOPs"],
+                   "mfma_f16_constant_TFLOPs": pk["mfma_f16_constant_TFLOPs"],
+                   "executed_frac": 3 * achieved / 1e12 / pk["mfma_f16_toggling_TFLOPs"]}
     line = {
         "metric": "points x realizations / s (cfg3 randomization, <=1e-5 rel RMS vs FP64)",
         "value": value,
@@ -600,7 +625,8 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": achieved / PEAK_F16_DENSE, "traffic": traffic,
                      "kernel_ms": float(kms.mean()),
                      "flop_per_launch": flop_alg, "executed_mfma_flop_per_launch": 3 * flop_alg,
-                     "variant": chosen, "variants_kernel_ms": variants_ms},
+                     "variant": chosen, "variants_kernel_ms": variants_ms,
+                     "measured_ceiling": ceiling},
         "latency_ms": {"p50": float(np.percentile(kms, 50)), "p99": float(np.percentile(kms, 99))},
         "direct_fp32": direct,
         "accuracy": acc,

@@ -3,6 +3,7 @@
     librf_hip.so          HIP kernels + C ABI (hipcc, gfx950 code objects)
     librandomfield_gpu.so Fortran facade over the C ABI (amdflang)
     librf_h5.so           HDF5 / XDMF writer of structured fields (host, g++)
+    rf_peaks              measured MFMA / transcendental ceilings (bench.py --peaks)
     tests/fortran/bin/*   Fortran drivers of the facade used by the test suite
     oracle/liboracle_rf.so  the CPU oracle (test infrastructure, gcc)
 
@@ -103,6 +104,16 @@ def build_h5() -> Path | None:
     return out
 
 
+def build_peaks() -> Path:
+    """lib/rf_peaks: measured ceilings of the MFMA and transcendental units
+    (bench.py reports them beside the datasheet peaks)."""
+    out = LIB / "rf_peaks"
+    tmp = LIB / "rf_peaks.tmp"
+    _run([HIPCC, *HIP_FLAGS, str(CSRC / "probe" / "rf_peaks.hip"), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> Path:
     out = ORACLE / "liboracle_rf.so"
     tmp = ORACLE / "liboracle_rf.so.tmp"
@@ -115,6 +126,7 @@ def build_oracle() -> Path:
 def build_all(jobs: int = 8) -> None:
     build_oracle()
     build_hip(jobs)
+    build_peaks()
     build_h5()
     if shutil.which(FC) or Path(FC).exists():
         build_fortran()

@@ -58,12 +58,6 @@ __device__ inline f16x mfma(const b8& a, const b8& b, const f16x& c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ inline float epi32(float u, const rf_epilogue& e) {
-    if (e.margi == RF_FOM_NONE) return u;
-    const float y = fmaf(u, (float)e.scale, (float)e.shift);
-    return (e.margi == RF_FOM_LOGNORMAL) ? expf(y) : y;
-}
-
 typedef __attribute__((address_space(3))) void lds_void;
 
 // (c, s) -> hi and lo halves at positions 2q, 2q+1 of the basis fragments.
@@ -288,20 +282,36 @@ __device__ __forceinline__ void sum_mfma_block(const rf_sum_args& a, int64_t nSt
         if (u + 1 < nStages) stage(u + 1, ldsB, ldsA, kqB, kqA);
     }
     if (pt >= a.i1) return;
-    // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4
+    // D layout of a 32x32 tile: lane column n, rows 8*(r/4) + 4*h + r%4.
+    // The marginal map and the output type are uniform: one store loop each,
+    // so the common cases carry no exp evaluation or dtype select per value.
+    auto store = [&](auto map, auto* out) {
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
+        for (int m = 0; m < MT; m++) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int real = (mt0 + m) * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            if (real < a.Nmc) {
-                const float u = epi32(acc[m][r] * outScale, a.epi);
-                if (a.outDtype == RF_DTYPE_F64)
-                    static_cast<double*>(a.out)[(int64_t)real * a.N + pt] = (double)u;
-                else
-                    static_cast<float*>(a.out)[(int64_t)real * a.N + pt] = u;
+            for (int r = 0; r < 16; r++) {
+                const int real = (mt0 + m) * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (real < a.Nmc) out[(int64_t)real * a.N + pt] = map(acc[m][r] * outScale);
             }
         }
+    };
+    const float sc = (float)a.epi.scale, sh = (float)a.epi.shift;
+    auto none = [](float u) { return u; };
+    auto gauss = [=](float u) { return fmaf(u, sc, sh); };
+    auto logn = [=](float u) { return expf(fmaf(u, sc, sh)); };
+    auto none64 = [](float u) { return (double)u; };
+    auto gauss64 = [=](float u) { return (double)fmaf(u, sc, sh); };
+    auto logn64 = [=](float u) { return (double)expf(fmaf(u, sc, sh)); };
+    float* o32 = static_cast<float*>(a.out);
+    double* o64 = static_cast<double*>(a.out);
+    if (a.outDtype == RF_DTYPE_F64) {
+        if (a.epi.margi == RF_FOM_NONE) store(none64, o64);
+        else if (a.epi.margi == RF_FOM_LOGNORMAL) store(logn64, o64);
+        else store(gauss64, o64);
+    } else {
+        if (a.epi.margi == RF_FOM_NONE) store(none, o32);
+        else if (a.epi.margi == RF_FOM_LOGNORMAL) store(logn, o32);
+        else store(gauss, o32);
     }
 }
 

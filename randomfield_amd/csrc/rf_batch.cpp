@@ -106,7 +106,8 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         }
         rf_grid_stages gs;
         const hipError_t e = rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, o, st,
-                                             ns, corrL, P->d_amp, P->d_phase, ampMult, P->epi, d_outs[i], outDtype,
+                                             ns, corrL, P->d_amp, P->d_phase, ampMult, ampMult * P->ampMax, P->epi,
+                                             d_outs[i], outDtype,
                                              reinterpret_cast<float2*>(B->d_mem + off[(size_t)i]), s, &gs);
         if (e != hipSuccess) {
             (void)hipFree(B->d_mem);

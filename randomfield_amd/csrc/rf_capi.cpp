@@ -232,6 +232,9 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
         RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_Sk, P->spec.Sk.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
         RF_HIP_TRY(ctx, rf_launch_fft_coeffs(kN, Nmc, P->d_Sk, P->d_uni, P->d_calc, P->d_amp, P->d_phase, s));
         P->coeffsReady = true;
+        double sMax = 0.0;
+        for (double v : P->spec.Sk) sMax = std::max(sMax, v);
+        P->ampMax = 0.5 * std::sqrt(sMax);          // |gamma - 1/2| <= 1/2, |cos| <= 1
         RF_HIP_TRY(ctx, hipStreamSynchronize(s));
         *out = P.release();
         return RF_OK;
@@ -297,6 +300,7 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
             RF_HIP_TRY(ctx, hipMemsetAsync(P->d_kRev, 0, (size_t)(P->kNpad * d) * sizeof(double), s));
         RF_HIP_TRY(ctx, rf_launch_krev(kN * d, P->d_k, P->d_kRev, s));
     }
+    P->ampMax = maxAmp;
     if (mfma) {
         P->bScale = rf_mfma_bscale(P->spec.info.ampMult, maxAmp);
         RF_HIP_TRY(ctx, hipMalloc(&P->d_B, rf_mfma_bbytes(p->form, P->kNpad, P->Rpad)));
@@ -506,7 +510,8 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     }
     rf_grid_stages st;
     RF_HIP_TRY(ctx, rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
-                                    nstep, corrL, P->d_amp, P->d_phase, ampMult, P->epi, d_out, outDtype,
+                                    nstep, corrL, P->d_amp, P->d_phase, ampMult, ampMult * P->ampMax, P->epi, d_out,
+                                    outDtype,
                                     static_cast<float2*>(P->d_gridScratch), s, &st));
     RF_HIP_TRY(ctx, rf_grid_run(st, s));
     return RF_OK;

@@ -131,7 +131,8 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
                 const int n = n0 + c * 32 + l32;
                 if (n >= g.N) continue;
                 const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
-                static_cast<float2*>(g.D)[o] = c ? make_float2(acc1[i], acc1[i + 1]) : make_float2(acc0[i], acc0[i + 1]);
+                static_cast<float2*>(g.D)[o] = c ? make_float2(acc1[i] * g.outScale, acc1[i + 1] * g.outScale)
+                                                 : make_float2(acc0[i] * g.outScale, acc0[i + 1] * g.outScale);
             }
         }
     } else {
@@ -141,30 +142,215 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
             const int n = n0 + ct * 32 + l32;
             if (m >= g.M || n >= g.N) continue;
             const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
-            const float u = epi32(acc0[i], g.epi);
+            const float u = epi32(acc0[i] * g.outScale, g.epi);
             if (g.outDtype == RF_DTYPE_F64) static_cast<double*>(g.D)[o] = (double)u;
             else static_cast<float*>(g.D)[o] = u;
         }
     }
 }
 
+// The same tile on the FP16 matrix cores (v_mfma_f32_32x32x16_f16), operands
+// split into FP16 hi + lo and D += A_hi B_hi + A_hi B_lo + A_lo B_hi (FP32
+// accumulate): ~22 significant bits per operand, 3x the instructions of one
+// product at 10x the FP32 MFMA rate.  A chunk of 16 complex k is converted
+// once while staged: A into planes Re, Im, -Im (hi and lo each) as [row][k],
+// B into Re, Im (hi, lo) as [column][k], so every MFMA operand is one 16-byte
+// LDS read.  In the complex product a 32-row MFMA tile holds 16 complex rows,
+// Re D in rows 0-15 ([Re A | -Im A] against [Re B ; Im B]) and Im D in rows
+// 16-31 ([Im A | Re A]); K' = 16 covers 8 complex k, lanes 0-31 the Re half.
+typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
+typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
+struct HStage {
+    _Float16 a[6][kTM][kHP];                  // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
+    _Float16 b[4][kTN][kHP];                  // Re hi, Re lo, Im hi, Im lo
+};
+
+__device__ inline void split4(const float* v, hv4* hi, hv4* lo) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const _Float16 h = (_Float16)v[i];
+        (*hi)[i] = h;
+        (*lo)[i] = (_Float16)(v[i] - (float)h);
+    }
+}
+
+template <bool RealOnly>
+__device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n0, int batch, HStage* st) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    // tile base pointers in 64-bit, offsets inside the tile in 32-bit (the
+    // launcher keeps K * stride and 64 * stride below 2^31)
+    const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm;
+    const float2* B = g.B + (int64_t)batch * g.bStrideB + (int64_t)n0 * g.bSn;
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk, bSn = (int)g.bSn;
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        acc0[i] = 0.0f;
+        acc1[i] = 0.0f;
+    }
+    const int rt = wave >> 1, ct = wave & 1;
+    // staging: A row t/4, k 4(t%4)..+3; B column t%64, k 4(t/64)..+3
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 63, bK = (tid >> 6) * 4;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
+    const int aOff = aRow * aSm, bOff = bCol * bSn;
+    float2 ra[4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i, kb = k0 + bK + i;
+            ra[i] = (aIn && ka < g.K) ? A[aOff + ka * aSk] : make_float2(0.f, 0.f);
+            rb[i] = (bIn && kb < g.K) ? B[kb * bSk + bOff] : make_float2(0.f, 0.f);
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4], nim[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = ra[i].x;
+            im[i] = ra[i].y;
+            nim[i] = -ra[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[1][aRow][aK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = lo;
+        split4(nim, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[4][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[5][aRow][aK]) = lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = rb[i].x;
+            im[i] = rb[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->b[0][bCol][bK]) = hi;
+        *reinterpret_cast<hv4*>(&st->b[1][bCol][bK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->b[2][bCol][bK]) = hi;
+        *reinterpret_cast<hv4*>(&st->b[3][bCol][bK]) = lo;
+    };
+    // this lane's operand planes: complex product, tile row l32 = Re (< 16)
+    // or Im row of complex row (l32 & 15); K' half 0 reads Re B, 1 reads Im B
+    const int isIm = l32 >> 4;
+    const int aPlane = RealOnly ? (half ? 4 : 0) : (isIm ? (half ? 0 : 2) : (half ? 4 : 0));
+    const int aRowL = RealOnly ? rt * 32 + l32 : wave * 16 + (l32 & 15);
+    const int bPlane = half ? 2 : 0;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+#pragma unroll
+        for (int kk = 0; kk < kTK; kk += 8) {
+            if (!RealOnly) {
+                if (m0 + wave * 16 < g.M) {
+                    const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                    const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+                    const hv8 bh0 = *reinterpret_cast<const hv8*>(&st->b[bPlane][l32][kk]);
+                    const hv8 bl0 = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][l32][kk]);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, acc0, 0, 0, 0);
+                    if (n0 + 32 < g.N) {
+                        const hv8 bh1 = *reinterpret_cast<const hv8*>(&st->b[bPlane][32 + l32][kk]);
+                        const hv8 bl1 = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][32 + l32][kk]);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, acc1, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, acc1, 0, 0, 0);
+                    }
+                }
+            } else if (m0 + rt * 32 < g.M && n0 + ct * 32 < g.N) {
+                const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+                const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][ct * 32 + l32][kk]);
+                const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][ct * 32 + l32][kk]);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc0, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + (int64_t)n0 * g.dSn;
+    const int dSm = (int)g.dSm, dSn = (int)g.dSn;
+    if (!RealOnly) {
+        float2* D = static_cast<float2*>(g.D) + dBase;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int r = wave * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
+            if (m0 + r >= g.M) continue;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int nn = c * 32 + l32;
+                if (n0 + nn >= g.N) continue;
+                D[r * dSm + nn * dSn] = c ? make_float2(acc1[i] * os, acc1[i + 8] * os)
+                                          : make_float2(acc0[i] * os, acc0[i + 8] * os);
+            }
+        }
+    } else {
+        const int nn = ct * 32 + l32;
+        const bool nIn = n0 + nn < g.N;
+        auto store = [&](auto map, auto* D) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+                if (m0 + r < g.M && nIn) D[r * dSm + nn * dSn] = map(acc0[i] * os);
+            }
+        };
+        const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
+        float* D32 = static_cast<float*>(g.D) + dBase;
+        double* D64 = static_cast<double*>(g.D) + dBase;
+        if (g.outDtype == RF_DTYPE_F64) {
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
+            else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+        } else {
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
+            else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+        }
+    }
+}
+
+struct FStage {
+    float2 a[kTK][kTM + 1];
+    float2 b[kTK][kTN + 1];
+};
+constexpr size_t kStageBytes = sizeof(HStage) > sizeof(FStage) ? sizeof(HStage) : sizeof(FStage);
+
+template <bool RealOnly>
+__device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, int batch, char* smem) {
+    if (g.split16) {
+        gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage*>(smem));
+    } else {
+        FStage* f = reinterpret_cast<FStage*>(smem);
+        gemm_tile<RealOnly>(g, m0, n0, batch, f->a, f->b);
+    }
+}
+
 template <bool RealOnly>
 __global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g) {
-    __shared__ float2 sA[kTK][kTM + 1];
-    __shared__ float2 sB[kTK][kTN + 1];
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
     // grid y and z are capped at 65535 by the launcher; larger row-tile and
     // batch counts are covered by these block-uniform loops
     for (int batch = blockIdx.z; batch < g.batches; batch += gridDim.z)
         for (int m0 = blockIdx.y * kTM; m0 < g.M; m0 += gridDim.y * kTM)
-            gemm_tile<RealOnly>(g, m0, blockIdx.x * kTN, batch, sA, sB);
+            run_tile<RealOnly>(g, m0, blockIdx.x * kTN, batch, smem);
 }
 
 // Grouped form: the GEMMs of several fields in one launch; GEMM q owns tiles
 // tile0 .. tile0 + tiles(q) - 1, ordered (batch, row tile, column tile).
 template <bool RealOnly>
 __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG) {
-    __shared__ float2 sA[kTK][kTM + 1];
-    __shared__ float2 sB[kTK][kTN + 1];
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
     const int64_t b = blockIdx.x;
     int lo = 0, hi = nG - 1;
     while (lo < hi) {
@@ -172,14 +358,14 @@ __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restr
         if (gs[mid].tile0 <= b) lo = mid;
         else hi = mid - 1;
     }
-    const rf_gemm_desc& g = gs[lo];
+    const rf_gemm_desc g = gs[lo];                 // uniform: scalar registers
     const int64_t local = b - g.tile0;
     const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + kTM - 1) / kTM;
     const int nI = (int)(local % ntN);
     const int64_t r = local / ntN;
     const int mI = (int)(r % ntM);
     const int batch = (int)(r / ntM);
-    gemm_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, sA, sB);
+    run_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, smem);
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -280,9 +466,24 @@ size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, c
 hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
-                           double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                           double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st) {
     const int64_t R = (int64_t)nb * Nmc;
+    // power-of-two scales for the FP16 operands: coefficients into [1/2, 1),
+    // the middle contraction's output below 2^14 (|T1| <= eMax K1 and
+    // |T2| <= eMax^2 K1 K2 for |table| <= eMax); the last stage undoes both
+    const double eMax = kind == 1 ? 2.0 : 1.0;
+    double sc = 1.0;
+    if (coefMax > 0.0 && std::isfinite(coefMax)) {
+        int e = 0;
+        std::frexp(coefMax, &e);
+        sc = std::ldexp(1.0, -e);
+    }
+    const double bound1 = eMax * kNStep[0];
+    const double bound2 = nDim == 3 ? eMax * eMax * kNStep[0] * (double)kNStep[1] : bound1;
+    int m2 = 0;
+    while (nDim == 3 && std::ldexp(bound2, -m2) > 16384.0) m2++;
+    int split = bound1 <= 16384.0 ? 1 : 0;
     float2* p = scratch;
     float2* E[3];
     float2* Ext[3];
@@ -304,7 +505,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int a = 0; a < nDim; a++) nkAll *= kNStep[a];
     float2* Cf = p; p += R * nkAll;
     hipLaunchKernelGGL(k_coeffs, dim3(blocks_for(R * nkAll)), dim3(256), 0, s, R * nkAll, d_amp, d_phase,
-                       ampMult, Cf);
+                       ampMult * sc, Cf);
     *st = rf_grid_stages{};
     const int64_t rows1 = R * nkAll / kNStep[0];
     float2* T1 = p; p += rows1 * n[0];
@@ -313,6 +514,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;
     g.D = T1; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
     g.M = (int)rows1; g.N = n[0]; g.K = kNStep[0]; g.batches = 1; g.realOut = 0;
+    g.outScale = 1.0f; g.split16 = split;
     const float2* last = T1;
     int64_t plane = n[0];                      // points already resolved
     if (nDim == 3) {
@@ -322,6 +524,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
         h.B = T1; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)kNStep[1] * n[0];
         h.D = T2; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
         h.M = n[1]; h.N = n[0]; h.K = kNStep[1]; h.batches = (int)(R * kNStep[2]); h.realOut = 0;
+        h.outScale = (float)std::ldexp(1.0, -m2); h.split16 = split;
         // batch index = (r*nb + b)*nk3 + j3; the axis-2 sign of block b is
         // bit 1 of b (3-D, nb = 4), i.e. (batch / (2 nk3)) % 2
         h.aDiv = kNStep[2] * 2;
@@ -336,6 +539,15 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     f.D = d_out; f.dSm = plane; f.dSn = 1; f.dStrideB = plane * n[ad];
     f.M = n[ad]; f.N = (int)plane; f.K = nb * kNStep[ad]; f.batches = Nmc; f.realOut = 1;
     f.outDtype = outDtype; f.epi = epi;
+    f.outScale = (float)(1.0 / (sc * std::ldexp(1.0, -m2))); f.split16 = split;
+    // the split-precision tile addresses inside a tile with 32-bit offsets
+    for (int i = 0; i < st->n; i++) {
+        rf_gemm_desc& q = st->g[i];
+        const int64_t lim = (int64_t)1 << 30;
+        if ((int64_t)q.K * q.aSk + 64 * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn >= lim ||
+            64 * q.dSm + 64 * q.dSn >= lim)
+            for (int j = 0; j < st->n; j++) st->g[j].split16 = 0;
+    }
     return hipGetLastError();
 }
 

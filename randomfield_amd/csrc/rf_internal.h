@@ -61,6 +61,7 @@ struct rf_field_plan {
     int Rpad = 0;
     int64_t kNpad = 0;
     double bScale = 1.0;         // power-of-two scale folded into B
+    double ampMax = 1.0;         // bound of |amp| over all realizations
     // REF_COMPAT: sqrt chain of Sk, row c = Sk^(2^-(c+1)), until it saturates
     std::vector<std::vector<double>> sqrtChain;
     bool coeffsReady = false;    // FIXED: coefficients independent of N
@@ -195,6 +196,8 @@ struct rf_gemm_desc {
     int outDtype;
     rf_epilogue epi;
     int64_t tile0;        // grouped launches: first tile of this GEMM
+    float outScale;       // D = outScale * (A B) (power of two)
+    int split16;          // 1: FP16 hi/lo operands (3 products), 0: FP32 operands
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)
@@ -203,10 +206,12 @@ struct rf_grid_stages {
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
 // kind 0: Shinozuka lattice, tables exp(i k_j x); kind 1: method 4, tables of
 // the DCT-III (FFTW REDFT01) of each axis, w_j cos(pi j (2i+1) / (2n))
+// coefMax bounds |ampMult * amp| (it sets the power-of-two scales that keep
+// the FP16 operands of the split-precision contractions in range).
 hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
-                           double ampMult, const rf_epilogue& epi, void* d_out, int outDtype,
+                           double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st);
 hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);
 int64_t rf_gemm_tiles(const rf_gemm_desc& g);

@@ -396,28 +396,42 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
     E[(int64_t)n * n + e] = make_float2((float)c, 0.0f);
 }
 
-// Per-block table for the last axis: Ext[b][j][x] = E^{s_b}[j][x], so the
-// last stage contracts over (b, j) with a plain stride.
-__global__ void k_block_table(int nb, int nk, int nx, int axisBit, const float2* __restrict__ E,
-                              float2* __restrict__ Ext) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t per = (int64_t)nk * nx;
-    if (e >= (int64_t)nb * per) return;
-    const int b = (int)(e / per);
-    const int64_t r = e - b * per;
-    const bool neg = (b >> axisBit) & 1;
-    Ext[e] = neg ? E[per + r] : E[r];
-}
-
-// C[r][j] = ampMult amp[r][j] exp(i phase[r][j]) (FP64 -> FP32 complex).
-__global__ void k_coeffs(int64_t n, const double* __restrict__ amp, const double* __restrict__ ph,
-                         double ampMult, float2* __restrict__ C) {
+// Sign-block folding.  The field is Re of the sum, and the tables of a
+// negative sign are conjugates, Re(E3^- E2^s2 T) = Re(E3^+ E2^-s2 conj(T)):
+// each block with s3 = - folds onto s3 = + with its stage-1 result
+// conjugated, and conj(sum_j1 E1 C) = sum_j1 conj(E1) conj(C).  In 3-D
+// (blocks b = 2 f2 + f3, f = 1 for a negative sign):
+//   V_q[j3][j2][x1] = sum_{h,j1} E1^{h}[j1][x1] Cv[j3][q][j2][h][j1],
+//     q = 0: Cv[.][0] = C_0, Cv[.][1] = conj(C_3);  q = 1: C_2, conj(C_1)
+//   W[j3][x2][x1] = sum_{q,j2} E2^{q}[j2][x2] V_q[j3][j2][x1]
+//   u[x3][x2][x1] = Re sum_j3 E3^+[j3][x3] W[j3][x2][x1]
+// In 2-D: V[j2][x1] = sum_{h,j1} E1^h (C_0 | conj(C_1)), u = Re sum_j2 E2^+ V.
+// The middle intermediate W is a quarter of the per-block one and the last
+// contraction runs over nk3 instead of 4 nk3.  Without sign blocks (method 4)
+// Cv = C.
+//
+// Cv[r][j3][q][j2][h][j1] = (conj if h) ampMult amp[r][b][j][j1] exp(i phase).
+__global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, int nk3, int64_t kNu,
+                              int64_t kNr, const double* __restrict__ amp, const double* __restrict__ ph,
+                              double ampMult, float2* __restrict__ C) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    double s, c;
-    sincos(ph[e], &s, &c);
-    const double a = amp[e] * ampMult;
-    C[e] = make_float2((float)(a * c), (float)(a * s));
+    const int H = fold ? 2 : 1, Q = (fold && nDim == 3) ? 2 : 1;
+    int64_t rem = e;
+    const int j1 = (int)(rem % nk1); rem /= nk1;
+    const int h = (int)(rem % H); rem /= H;
+    const int j2 = (int)(rem % nk2); rem /= nk2;
+    const int q = (int)(rem % Q); rem /= Q;
+    const int j3 = nDim == 3 ? (int)(rem % nk3) : 0;
+    const int64_t r = nDim == 3 ? rem / nk3 : rem;
+    int b = 0;
+    if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
+    const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
+    const int64_t src = r * kNr + b * kNu + j;
+    double sn, cs;
+    sincos(ph[src], &sn, &cs);
+    const double a = amp[src] * ampMult;
+    C[e] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -446,17 +460,18 @@ hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, i
     return hipGetLastError();
 }
 
-// Scratch for one grid execution (float2 elements).
+// Scratch for one grid execution (float2 elements): tables (both sign
+// variants per axis), folded coefficients, V and (3-D) W.
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n) {
-    const int64_t R = (int64_t)nb * Nmc;
     int64_t tables = 0;
-    for (int a = 0; a < nDim; a++) tables += 2LL * kNStep[a] * n[a] + (int64_t)nb * kNStep[a] * n[a];
-    int64_t coeff = R;
+    for (int a = 0; a < nDim; a++) tables += 2LL * kNStep[a] * n[a];
+    int64_t coeff = (int64_t)nb * Nmc;
     for (int a = 0; a < nDim; a++) coeff *= kNStep[a];
-    int64_t t1 = R * n[0];
-    for (int a = 1; a < nDim; a++) t1 *= kNStep[a];
-    int64_t t2 = (nDim == 3) ? R * n[0] * n[1] * kNStep[2] : 0;
-    return (size_t)(tables + coeff + t1 + t2);
+    const int fold = nb > 1, Q = (fold && nDim == 3) ? 2 : 1;
+    int64_t v = (int64_t)Nmc * Q * n[0];
+    for (int a = 1; a < nDim; a++) v *= kNStep[a];
+    const int64_t w = nDim == 3 ? (int64_t)Nmc * kNStep[2] * n[1] * n[0] : 0;
+    return (size_t)(tables + coeff + v + w);
 }
 
 // Builds the phase tables and coefficients in scratch (launches on s) and
@@ -468,10 +483,11 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
                            const double* corrL, const double* d_amp, const double* d_phase,
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st) {
-    const int64_t R = (int64_t)nb * Nmc;
     // power-of-two scales for the FP16 operands: coefficients into [1/2, 1),
-    // the middle contraction's output below 2^14 (|T1| <= eMax K1 and
-    // |T2| <= eMax^2 K1 K2 for |table| <= eMax); the last stage undoes both
+    // the middle contraction's output below 2^14 (|V| <= eMax K1 and
+    // |W| <= eMax^2 K1 K2 for |table| <= eMax); the last stage undoes both
+    const int fold = nb > 1 ? 1 : 0;
+    const int H = fold ? 2 : 1, Q = (fold && nDim == 3) ? 2 : 1;
     const double eMax = kind == 1 ? 2.0 : 1.0;
     double sc = 1.0;
     if (coefMax > 0.0 && std::isfinite(coefMax)) {
@@ -479,65 +495,65 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
         std::frexp(coefMax, &e);
         sc = std::ldexp(1.0, -e);
     }
-    const double bound1 = eMax * kNStep[0];
-    const double bound2 = nDim == 3 ? eMax * eMax * kNStep[0] * (double)kNStep[1] : bound1;
+    const int K1 = H * kNStep[0];
+    const int K2 = nDim == 3 ? Q * kNStep[1] : kNStep[1];
+    const double bound1 = eMax * K1;
+    const double bound2 = nDim == 3 ? eMax * eMax * K1 * (double)K2 : bound1;
     int m2 = 0;
     while (nDim == 3 && std::ldexp(bound2, -m2) > 16384.0) m2++;
     int split = bound1 <= 16384.0 ? 1 : 0;
+
     float2* p = scratch;
     float2* E[3];
-    float2* Ext[3];
     for (int a = 0; a < nDim; a++) {
         E[a] = p; p += 2LL * kNStep[a] * n[a];
-        Ext[a] = p; p += (int64_t)nb * kNStep[a] * n[a];
         const int64_t cnt = (int64_t)kNStep[a] * n[a];
         if (kind == 1)
             hipLaunchKernelGGL(k_dct_table, dim3(blocks_for(cnt)), dim3(256), 0, s, n[a], E[a]);
         else
             hipLaunchKernelGGL(k_phase_table, dim3(blocks_for(cnt)), dim3(256), 0, s, kNStep[a], n[a],
                                kDelta[a], origin[a], step[a], 1.0 / corrL[a], 2, E[a]);
-        // sign of axis a in block b: kSign column a flips on bit (nDim-1-a) of b
-        if (a > 0)
-            hipLaunchKernelGGL(k_block_table, dim3(blocks_for(nb * cnt)), dim3(256), 0, s, nb, kNStep[a],
-                               n[a], nDim - 1 - a, E[a], Ext[a]);
     }
-    int64_t nkAll = 1;
-    for (int a = 0; a < nDim; a++) nkAll *= kNStep[a];
-    float2* Cf = p; p += R * nkAll;
-    hipLaunchKernelGGL(k_coeffs, dim3(blocks_for(R * nkAll)), dim3(256), 0, s, R * nkAll, d_amp, d_phase,
+    int64_t kNu = 1;
+    for (int a = 0; a < nDim; a++) kNu *= kNStep[a];
+    const int64_t nC = (int64_t)Nmc * nb * kNu;          // = Nmc x nk3 x Q x nk2 x H x nk1
+    float2* Cf = p; p += nC;
+    hipLaunchKernelGGL(k_coeffs_fold, dim3(blocks_for(nC)), dim3(256), 0, s, nC, nDim, fold, kNStep[0],
+                       nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp, d_phase,
                        ampMult * sc, Cf);
     *st = rf_grid_stages{};
-    const int64_t rows1 = R * nkAll / kNStep[0];
-    float2* T1 = p; p += rows1 * n[0];
+    // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
+    const int64_t rows1 = nC / K1;
+    float2* V = p; p += rows1 * n[0];
     rf_gemm_desc& g = st->g[st->n++];
-    g.A = Cf; g.aSm = kNStep[0]; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
-    g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;
-    g.D = T1; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
-    g.M = (int)rows1; g.N = n[0]; g.K = kNStep[0]; g.batches = 1; g.realOut = 0;
+    g.A = Cf; g.aSm = K1; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
+    g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;     // K (h, j1): variant h of E1
+    g.D = V; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
+    g.M = (int)rows1; g.N = n[0]; g.K = K1; g.batches = 1; g.realOut = 0;
     g.outScale = 1.0f; g.split16 = split;
-    const float2* last = T1;
+    const float2* last = V;
     int64_t plane = n[0];                      // points already resolved
+    int64_t lastStrideB = (int64_t)kNStep[1] * n[0];
     if (nDim == 3) {
-        float2* T2 = p; p += R * kNStep[2] * n[1] * n[0];
+        // stage 2, batch (r, j3): W[x2][x1] = sum_{q,j2} E2^q[j2][x2] V[q][j2][x1]
+        float2* W = p; p += (int64_t)Nmc * kNStep[2] * n[1] * n[0];
         rf_gemm_desc& h = st->g[st->n++];
-        h.A = E[1]; h.aSm = 1; h.aSk = n[1]; h.aStrideB = (int64_t)kNStep[1] * n[1];
-        h.B = T1; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)kNStep[1] * n[0];
-        h.D = T2; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
-        h.M = n[1]; h.N = n[0]; h.K = kNStep[1]; h.batches = (int)(R * kNStep[2]); h.realOut = 0;
+        h.A = E[1]; h.aSm = 1; h.aSk = n[1]; h.aStrideB = 0; h.aDiv = 1; h.aMod = 1;
+        h.B = V; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)K2 * n[0];
+        h.D = W; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
+        h.M = n[1]; h.N = n[0]; h.K = K2; h.batches = Nmc * kNStep[2]; h.realOut = 0;
         h.outScale = (float)std::ldexp(1.0, -m2); h.split16 = split;
-        // batch index = (r*nb + b)*nk3 + j3; the axis-2 sign of block b is
-        // bit 1 of b (3-D, nb = 4), i.e. (batch / (2 nk3)) % 2
-        h.aDiv = kNStep[2] * 2;
-        h.aMod = 2;
-        last = T2;
+        last = W;
         plane = (int64_t)n[0] * n[1];
+        lastStrideB = (int64_t)kNStep[2] * plane;
     }
+    // last stage per realization: u[x_d][plane] = Re sum_j E_d^+[j][x_d] last[j][plane]
     const int ad = nDim - 1;
     rf_gemm_desc& f = st->g[st->n++];
-    f.A = Ext[ad]; f.aSm = 1; f.aSk = n[ad]; f.aStrideB = 0; f.aDiv = 1; f.aMod = 1;
-    f.B = last; f.bSk = plane; f.bSn = 1; f.bStrideB = (int64_t)nb * kNStep[ad] * plane;
+    f.A = E[ad]; f.aSm = 1; f.aSk = n[ad]; f.aStrideB = 0; f.aDiv = 1; f.aMod = 1;
+    f.B = last; f.bSk = plane; f.bSn = 1; f.bStrideB = lastStrideB;
     f.D = d_out; f.dSm = plane; f.dSn = 1; f.dStrideB = plane * n[ad];
-    f.M = n[ad]; f.N = (int)plane; f.K = nb * kNStep[ad]; f.batches = Nmc; f.realOut = 1;
+    f.M = n[ad]; f.N = (int)plane; f.K = kNStep[ad]; f.batches = Nmc; f.realOut = 1;
     f.outDtype = outDtype; f.epi = epi;
     f.outScale = (float)(1.0 / (sc * std::ldexp(1.0, -m2))); f.split16 = split;
     // the split-precision tile addresses inside a tile with 32-bit offsets

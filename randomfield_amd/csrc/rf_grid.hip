@@ -336,22 +336,53 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
     }
 }
 
-template <bool RealOnly>
-__global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g) {
-    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
-    // grid y and z are capped at 65535 by the launcher; larger row-tile and
-    // batch counts are covered by these block-uniform loops
-    for (int batch = blockIdx.z; batch < g.batches; batch += gridDim.z)
-        for (int m0 = blockIdx.y * kTM; m0 < g.M; m0 += gridDim.y * kTM)
-            run_tile<RealOnly>(g, m0, blockIdx.x * kTN, batch, smem);
+// Tile order and placement.  A GEMM's tiles are numbered so that the tiles
+// sharing the larger operand are consecutive: row tiles fastest when there
+// are no more of them than column tiles (they share a column slab of B, the
+// previous stage's output), column tiles fastest otherwise (they share a row
+// slab of A, the coefficients).  Workgroups go round-robin to the 8 XCDs, each
+// with its own L2; block b runs logical tile (b % 8) * P + b / 8, so every XCD
+// walks a contiguous range of P tiles and a shared slab is re-read from its
+// L2, not from HBM.
+constexpr int kXcd = 8;
+
+__device__ inline void tile_coords(const rf_gemm_desc& g, int64_t local, int* mI, int* nI, int* batch) {
+    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + kTM - 1) / kTM;
+    if (ntM <= ntN) {
+        *mI = (int)(local % ntM);
+        const int64_t r = local / ntM;
+        *nI = (int)(r % ntN);
+        *batch = (int)(r / ntN);
+    } else {
+        *nI = (int)(local % ntN);
+        const int64_t r = local / ntN;
+        *mI = (int)(r % ntM);
+        *batch = (int)(r / ntM);
+    }
 }
 
-// Grouped form: the GEMMs of several fields in one launch; GEMM q owns tiles
-// tile0 .. tile0 + tiles(q) - 1, ordered (batch, row tile, column tile).
+__device__ inline int64_t xcd_tile(int64_t b, int64_t nTiles) {
+    const int64_t per = (nTiles + kXcd - 1) / kXcd;
+    return (b % kXcd) * per + b / kXcd;
+}
+
 template <bool RealOnly>
-__global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG) {
+__global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
     __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
-    const int64_t b = blockIdx.x;
+    const int64_t L = xcd_tile(blockIdx.x, nTiles);
+    if (L >= nTiles) return;
+    int mI, nI, batch;
+    tile_coords(g, L, &mI, &nI, &batch);
+    run_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, smem);
+}
+
+// Grouped form: the GEMMs of several fields in one launch; GEMM q owns
+// logical tiles tile0 .. tile0 + tiles(q) - 1.
+template <bool RealOnly>
+__global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
+    const int64_t b = xcd_tile(blockIdx.x, nTiles);
+    if (b >= nTiles) return;
     int lo = 0, hi = nG - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -359,12 +390,8 @@ __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restr
         else hi = mid - 1;
     }
     const rf_gemm_desc g = gs[lo];                 // uniform: scalar registers
-    const int64_t local = b - g.tile0;
-    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + kTM - 1) / kTM;
-    const int nI = (int)(local % ntN);
-    const int64_t r = local / ntN;
-    const int mI = (int)(r % ntM);
-    const int batch = (int)(r / ntM);
+    int mI, nI, batch;
+    tile_coords(g, b - g.tile0, &mI, &nI, &batch);
     run_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, smem);
 }
 
@@ -442,21 +469,26 @@ int64_t rf_gemm_tiles(const rf_gemm_desc& g) {
     return (int64_t)((g.N + kTN - 1) / kTN) * ((g.M + kTM - 1) / kTM) * g.batches;
 }
 
+// blocks of a launch over nTiles logical tiles (a multiple of the XCD count)
+inline int64_t xcd_blocks(int64_t nTiles) { return (nTiles + kXcd - 1) / kXcd * kXcd; }
+
 hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
-    const int64_t mt = ((int64_t)g.M + kTM - 1) / kTM;
-    dim3 grid((unsigned)((g.N + kTN - 1) / kTN), (unsigned)(mt < 65535 ? mt : 65535),
-              (unsigned)(g.batches < 65535 ? g.batches : 65535));
-    if (g.realOut) hipLaunchKernelGGL(k_cgemm<true>, grid, dim3(256), 0, s, g);
-    else hipLaunchKernelGGL(k_cgemm<false>, grid, dim3(256), 0, s, g);
+    const int64_t nTiles = rf_gemm_tiles(g);
+    if (nTiles <= 0) return hipSuccess;
+    const int64_t blocks = xcd_blocks(nTiles);
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (g.realOut) hipLaunchKernelGGL(k_cgemm<true>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else hipLaunchKernelGGL(k_cgemm<false>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
 }
 
 hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles,
                                 hipStream_t s) {
     if (nG <= 0 || nTiles <= 0) return hipSuccess;
-    if (nTiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (realOut) hipLaunchKernelGGL(k_cgemm_group<true>, dim3((unsigned)nTiles), dim3(256), 0, s, d_gs, nG);
-    else hipLaunchKernelGGL(k_cgemm_group<false>, dim3((unsigned)nTiles), dim3(256), 0, s, d_gs, nG);
+    const int64_t blocks = xcd_blocks(nTiles);
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (realOut) hipLaunchKernelGGL(k_cgemm_group<true>, dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else hipLaunchKernelGGL(k_cgemm_group<false>, dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     return hipGetLastError();
 }
 

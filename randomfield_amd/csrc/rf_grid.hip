@@ -166,13 +166,29 @@ struct HStage {
     _Float16 b[4][kTN][kHP];                  // Re hi, Re lo, Im hi, Im lo
 };
 
+// v -> FP16 hi (round to nearest) and lo = v - hi: packed conversions for
+// hi and lo, the residuals by mixed-precision FMAs reading the f16 halves in
+// place (v_fma_mix_f32), two instructions per value.
 __device__ inline void split4(const float* v, hv4* hi, hv4* lo) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const _Float16 h = (_Float16)v[i];
-        (*hi)[i] = h;
-        (*lo)[i] = (_Float16)(v[i] - (float)h);
+    for (int i = 0; i < 4; i += 2) {
+        const h2 h = {(_Float16)v[i], (_Float16)v[i + 1]};       // v_cvt_pk_f16_f32
+        const unsigned hb = __builtin_bit_cast(unsigned, h);
+        float l0, l1;
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[0,0,0] op_sel_hi:[1,0,0]" : "=v"(l0) : "v"(hb), "v"(v[i]));
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(l1) : "v"(hb), "v"(v[i + 1]));
+        (*hi)[i] = h.x;
+        (*hi)[i + 1] = h.y;
+        (*lo)[i] = (_Float16)l0;
+        (*lo)[i + 1] = (_Float16)l1;
     }
+}
+
+// -x of FP16 values: the sign bits flipped
+__device__ inline hv4 neg4(hv4 x) {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(hv4, __builtin_bit_cast(u2, x) ^ 0x80008000u);
 }
 
 template <bool RealOnly>
@@ -206,13 +222,12 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         }
     };
     auto stash = [&]() {
-        float re[4], im[4], nim[4];
+        float re[4], im[4];
         hv4 hi, lo;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             re[i] = ra[i].x;
             im[i] = ra[i].y;
-            nim[i] = -ra[i].y;
         }
         split4(re, &hi, &lo);
         *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
@@ -220,9 +235,8 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         split4(im, &hi, &lo);
         *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = hi;
         *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = lo;
-        split4(nim, &hi, &lo);
-        *reinterpret_cast<hv4*>(&st->a[4][aRow][aK]) = hi;
-        *reinterpret_cast<hv4*>(&st->a[5][aRow][aK]) = lo;
+        *reinterpret_cast<hv4*>(&st->a[4][aRow][aK]) = neg4(hi);   // split(-x) = -split(x)
+        *reinterpret_cast<hv4*>(&st->a[5][aRow][aK]) = neg4(lo);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             re[i] = rb[i].x;

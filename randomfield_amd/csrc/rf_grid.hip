@@ -452,6 +452,7 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // Cv = C.
 //
 // Cv[r][j3][q][j2][h][j1] = (conj if h) ampMult amp[r][b][j][j1] exp(i phase).
+// ph == NULL: real coefficients (method 4), no phase factor.
 __global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, int nk3, int64_t kNu,
                               int64_t kNr, const double* __restrict__ amp, const double* __restrict__ ph,
                               double ampMult, float2* __restrict__ C) {
@@ -469,9 +470,13 @@ __global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, i
     if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
     const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
     const int64_t src = r * kNr + b * kNu + j;
+    const double a = amp[src] * ampMult;
+    if (!ph) {
+        C[e] = make_float2((float)a, 0.0f);
+        return;
+    }
     double sn, cs;
     sincos(ph[src], &sn, &cs);
-    const double a = amp[src] * ampMult;
     C[e] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
 }
 
@@ -565,8 +570,8 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     const int64_t nC = (int64_t)Nmc * nb * kNu;          // = Nmc x nk3 x Q x nk2 x H x nk1
     float2* Cf = p; p += nC;
     hipLaunchKernelGGL(k_coeffs_fold, dim3(blocks_for(nC)), dim3(256), 0, s, nC, nDim, fold, kNStep[0],
-                       nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp, d_phase,
-                       ampMult * sc, Cf);
+                       nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp,
+                       kind == 1 ? nullptr : d_phase, ampMult * sc, Cf);
     *st = rf_grid_stages{};
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;

@@ -318,26 +318,18 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
 #pragma clang fp contract(off)
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= b.groupTot) return;
-    int gi0, gi1, gi2;
-    if (b.groupTot <= 0x7fffffffLL) {
-        const uint32_t u = (uint32_t)g, n0 = (uint32_t)b.groupN[0], n1 = (uint32_t)b.groupN[1];
-        const uint32_t t = u / n0;
-        gi0 = (int)(u - t * n0);
-        gi2 = (int)(t / n1);
-        gi1 = (int)(t - (uint32_t)gi2 * n1);
-    } else {
-        const int64_t t = g / b.groupN[0];
-        gi0 = (int)(g - t * b.groupN[0]);
-        gi2 = (int)(t / b.groupN[1]);
-        gi1 = (int)(t - (int64_t)gi2 * b.groupN[1]);
-    }
+    // block (x segment, gi1, gi2): the axis-1 and axis-2 entries are
+    // block-uniform (scalar loads), one axis-0 entry per thread
+    const int gi0 = blockIdx.x * kBlock + threadIdx.x;
+    const int gi1 = blockIdx.y, gi2 = blockIdx.z;
+    if (gi0 >= b.groupN[0]) return;
+    const int64_t g = gi0 + (int64_t)b.groupN[0] * (gi1 + (int64_t)b.groupN[1] * gi2);
     const rf_blend_axis A0 = b.ax[0][gi0];
     const rf_blend_axis A1 = b.ax[1][gi1];
     const rf_blend_axis A2 = b.ax[2][gi2];
+    const bool single = A0.iv[1] < 0 && A1.iv[1] < 0 && A2.iv[1] < 0;
     // fused with the batch: points of a single subdomain are already written
-    if (b.skipSingle && A0.iv[1] < 0 && A1.iv[1] < 0 && A2.iv[1] < 0) return;
+    if (b.skipSingle && single) return;
     const double pu1 = (A0.pu1 * A1.pu1) * A2.pu1;
     constexpr int RB = 4;                 // realizations per pass
     for (int r0 = 0; r0 < b.Nmc; r0 += RB) {
@@ -512,7 +504,8 @@ hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const int32_t* lo, c
 }
 
 hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s) {
-    const unsigned grid = (unsigned)((b.groupTot + kBlock - 1) / kBlock);
+    if (b.groupN[1] > 65535 || b.groupN[2] > 65535) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((b.groupN[0] + kBlock - 1) / kBlock), (unsigned)b.groupN[1], (unsigned)b.groupN[2]);
     if (dtype == RF_DTYPE_F64)
         hipLaunchKernelGGL(k_blend_sep<double>, dim3(grid), dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
     else

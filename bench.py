@@ -224,30 +224,9 @@ def run_cfg4(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps, np.array([a.elapsed_time(b) for a, b in ev])
 
-    wall_u, ms_u = timed(lambda: graph.launch(sptr), args.steps, args.warmup)
+    wall_s, ms_s = timed(lambda: graph.launch(sptr), args.steps, args.warmup)
     _, ms_g = timed(lambda: batch.execute(sptr), args.steps, 1)
     _, ms_b = timed(lambda: bplan.execute(g, sptr), args.steps, 1)
-    # fused: the last contraction writes the single-subdomain points into the
-    # group grid and the blend adds only the overlaps (bit-identical group)
-    torch.cuda.synchronize()
-    g_unfused = g.clone()
-    acc_fields0 = fields[0].clone()
-    batch_f = R.GridBatch(ctx, plans, [x[0] for x in geo], [step] * len(geo), [x[1] for x in geo], fields,
-                          R.DTYPE_F32)
-    bplan_f = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields, R.DTYPE_F32)
-    batch_f.fuse_blend(bplan_f, g)
-    graph_f = R.Graph(ctx, sptr)
-    with graph_f:
-        batch_f.execute(sptr)
-        bplan_f.execute(g, sptr)
-    g.zero_()
-    torch.cuda.synchronize()
-    wall_s, ms_s = timed(lambda: graph_f.launch(sptr), args.steps, args.warmup)
-    _, ms_gf = timed(lambda: batch_f.execute(sptr), args.steps, 1)
-    _, ms_bf = timed(lambda: bplan_f.execute(g, sptr), args.steps, 1)
-    torch.cuda.synchronize()
-    fused_equal = bool(torch.equal(g, g_unfused))
-    del g_unfused
     nglob = float(np.prod(groupN))
     nsub = float(sum(np.prod(s) for s in sizes))
     work = nglob * Nmc
@@ -268,26 +247,19 @@ def run_cfg4(args):
     t = time.perf_counter()
     ref = O.shinozuka(po, xs[sel], k, sk, phU, threads=_threads())
     cpu_s = time.perf_counter() - t
-    got = acc_fields0[:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
+    got = fields[0][:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
     graph.close()
-    graph_f.close()
     bplan.close()
     batch.close()
-    bplan_f.close()
-    batch_f.close()
     for pl in plans:
         pl.close()
     ctx.close()
     return _line("points x realizations / s (cfg4 localization: generation + blend)", work / (wall_s * 1e-3),
                  args.steps, args.warmup, wall_s, "f32",
                  "cfg4: 512x512x256 grid, 4x4x2 uneven subdomains, 7-point overlap, Nmc 16, "
-                 f"N_k {info.kNTotal} (subdomain 0), grid form in grouped launches + variance-preserving blend "
-                 "fused into the last contraction, one captured graph per step",
-                 step_kernel_ms=float(ms_s.mean()),
-                 fused={"generation_ms": float(ms_gf.mean()), "overlap_blend_ms": float(ms_bf.mean()),
-                        "group_bit_identical_to_unfused": fused_equal},
-                 unfused={"step_ms": wall_u, "step_kernel_ms": float(ms_u.mean())},
-                 generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
+                 f"N_k {info.kNTotal} (subdomain 0), grid form in grouped launches + variance-preserving blend, "
+                 "one captured graph per step",
+                 step_kernel_ms=float(ms_s.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
                  blend_roofline={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
                                  "frac": gbs / 8000.0, "kernel_ms": float(ms_b.mean()),
                                  "bytes_per_launch": bytes_blend},

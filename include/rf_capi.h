@@ -252,18 +252,6 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
                          const int32_t* nsteps, void* const* d_outs,
                          int32_t outDtype, rf_grid_batch** batch);
 int rf_grid_batch_execute(rf_grid_batch* batch, void* stream);
-/* Fuse the blend into the batch's last contraction (no counterpart in the
- * reference, which writes every subdomain and then adds them into the group,
- * localization_RF.f90:199-263): batch field s must be blend subdomain s
- * (same output buffer and sizes, separable arrangement, split-precision
- * contractions).  Afterwards rf_grid_batch_execute writes every group point
- * covered by one subdomain straight into d_group (its weight, and the
- * REF_COMPAT boundary division of realization 1, applied exactly as the blend
- * applies them) and only the overlap points into the fields, and
- * rf_blend_plan_execute (on d_group only) adds the overlap points: the group
- * result is bit-identical to the unfused pair of calls.  The subdomain
- * fields then hold their overlap points only. */
-int rf_grid_batch_fuse_blend(rf_grid_batch* batch, rf_blend_plan* blend, void* d_group);
 int rf_grid_batch_destroy(rf_grid_batch* batch);
 
 /* Graph capture of the launches a thread enqueues on an explicit stream

@@ -300,12 +300,6 @@ class GridBatch:
     def execute(self, stream=None) -> None:
         check(lib().rf_grid_batch_execute(self._h, _ptr(stream)), self.ctx.handle)
 
-    def fuse_blend(self, blend: "BlendPlan", d_group) -> None:
-        """rf_grid_batch_fuse_blend: single-covered group points written by the
-        last contraction; the blend plan then adds the overlap points."""
-        self._fused = (blend, d_group)
-        check(lib().rf_grid_batch_fuse_blend(self._h, blend._h, _ptr(d_group)), self.ctx.handle)
-
     def close(self) -> None:
         if self._h:
             lib().rf_grid_batch_destroy(self._h)

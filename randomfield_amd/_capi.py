@@ -33,7 +33,7 @@ EXPORTS = (
     "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_read",
     "rf_server_release", "rf_server_get_stats",
     "rf_blend_plan_create", "rf_blend_plan_execute", "rf_blend_plan_destroy",
-    "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy", "rf_grid_batch_fuse_blend",
+    "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy",
     "rf_graph_capture_begin", "rf_graph_capture_end", "rf_graph_launch", "rf_graph_destroy",
 )
 
@@ -144,7 +144,6 @@ def lib() -> C.CDLL:
     L.rf_corr_device.argtypes = [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp]
     L.rf_interpolate_device.argtypes = [vp, i32, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
     L.rf_generate_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32]
-    L.rf_grid_batch_fuse_blend.argtypes = [vp, vp, vp]
     L.rf_interpolate_box_device.argtypes = [vp, i32, vp, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
     L.rf_unv_read.argtypes = [C.c_char_p, i32, vp]
     L.rf_unv_sizes.argtypes = [vp, vp, vp, vp, vp]

@@ -16,8 +16,15 @@
 #include <vector>
 
 #include "rf_internal.h"
-#include "rf_plans.h"
 
+struct rf_grid_batch {
+    rf_context* ctx = nullptr;
+    char* d_mem = nullptr;              // scratch of every field, then descriptors
+    rf_gemm_desc* d_desc[3] = {nullptr, nullptr, nullptr};
+    int nDesc[3] = {0, 0, 0};
+    int64_t tiles[3] = {0, 0, 0};
+    int realOut[3] = {0, 0, 1};
+};
 
 struct rf_graph {
     rf_context* ctx = nullptr;
@@ -73,8 +80,6 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
     std::unique_ptr<rf_grid_batch> B(new (std::nothrow) rf_grid_batch());
     if (!B) return rf_set_error(ctx, RF_E_NOMEM, "batch allocation failed");
     B->ctx = ctx;
-    B->outDtype = outDtype;
-    B->nstep.assign(nsteps, nsteps + 3 * (size_t)n);
     if (hipMalloc((void**)&B->d_mem, total) != hipSuccess)
         return rf_set_error(ctx, RF_E_NOMEM, "batch scratch allocation failed");
     const hipStream_t s = ctx->stream;
@@ -125,7 +130,6 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         B->nDesc[k] = (int)stage[k].size();
         B->d_desc[k] = reinterpret_cast<rf_gemm_desc*>(B->d_mem + descOff) + dOff;
         all.insert(all.end(), stage[k].begin(), stage[k].end());
-        B->desc[k] = stage[k];
         dOff += stage[k].size();
         if (t > 0x7fffffffLL) {
             (void)hipFree(B->d_mem);
@@ -156,7 +160,6 @@ int rf_grid_batch_destroy(rf_grid_batch* B) {
     if (!B) return RF_OK;
     (void)hipSetDevice(B->ctx->device);
     (void)hipFree(B->d_mem);
-    if (B->d_fuse) (void)hipFree(B->d_fuse);
     delete B;
     return RF_OK;
 }

@@ -27,7 +27,6 @@
 
 #include "rf_blend.h"
 #include "rf_internal.h"
-#include "rf_plans.h"
 
 namespace {
 
@@ -54,6 +53,14 @@ void axis_factors(int N, int n, bool lower, bool upper, int mode, double* low, d
 
 }  // namespace
 
+struct rf_blend_plan {
+    rf_context* ctx = nullptr;
+    char* d_blob = nullptr;
+    rf_blend_dev b{};
+    bool separable = false;
+    rf_blend_sep sep{};
+    int dtype = RF_DTYPE_F64;
+};
 
 extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
                                     int32_t nSub, const int32_t* origins, const int32_t* sizes,
@@ -299,14 +306,7 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
     b.lowF = reinterpret_cast<const double*>(d_blob + oLow);
     b.upF = reinterpret_cast<const double*>(d_blob + oUp);
     b.maxSize = maxSize;
-    P->nDim = nDim; P->nSub = nSub; P->mode = mode;
-    for (int a = 0; a < nDim; a++) P->groupN[a] = groupN[a];
-    P->org.assign(origins, origins + (size_t)nSub * nDim);
-    P->siz.assign(sizes, sizes + (size_t)nSub * nDim);
-    P->subInt = subInt;
-    P->fields.assign(d_fields, d_fields + nSub);
     if (separable) {
-        for (int a = 0; a < 3; a++) P->axH[a] = axT[a];
         P->separable = true;
         rf_blend_sep& q = P->sep;
         q.nDim = nDim; q.Nmc = Nmc; q.mode = mode;
@@ -330,8 +330,6 @@ extern "C" int rf_blend_plan_execute(rf_blend_plan* P, void* d_group, void* stre
     RF_HIP_TRY(ctx, hipGetDevice(&dev));
     if (dev != ctx->device) RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    if (P->fusedGroup && d_group != P->fusedGroup)
-        return rf_set_error(ctx, RF_E_ARG, "blend plan fused with a batch: execute on the group buffer given to the fuse");
     if (P->separable) RF_HIP_TRY(ctx, rf_launch_blend_sep(P->sep, d_group, P->dtype, s));
     else RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
     return RF_OK;
@@ -362,87 +360,5 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
     rf_blend_plan_destroy(P);
     if (rc) return rc;
     if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("blend: ") + hipGetErrorString(e));
-    return RF_OK;
-}
-
-extern "C" int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) {
-    if (!B || !P) return rf_set_error(nullptr, RF_E_ARG, "batch or blend plan is NULL");
-    rf_context* ctx = B->ctx;
-    if (!d_group) return rf_set_error(ctx, RF_E_ARG, "NULL group buffer");
-    if (P->ctx != ctx) return rf_set_error(ctx, RF_E_ARG, "batch and blend plan of different contexts");
-    if (!P->separable) return rf_set_error(ctx, RF_E_UNSUPPORTED, "fusing needs a separable subdomain arrangement");
-    if (P->fusedGroup) return rf_set_error(ctx, RF_E_ARG, "blend plan already fused");
-    const int nSub = P->nSub, d = P->nDim;
-    std::vector<rf_gemm_desc>& last = B->desc[2];
-    if ((int)last.size() != nSub) return rf_set_error(ctx, RF_E_ARG, "batch and blend plan hold different field counts");
-    if (B->outDtype != P->dtype) return rf_set_error(ctx, RF_E_ARG, "batch and blend plan of different dtypes");
-    for (int s = 0; s < nSub; s++) {
-        if (last[(size_t)s].D != P->fields[(size_t)s])
-            return rf_set_error(ctx, RF_E_ARG, "batch field s is not blend subdomain s");
-        if (!last[(size_t)s].split16)
-            return rf_set_error(ctx, RF_E_UNSUPPORTED, "fusing needs the split-precision contractions");
-        for (int a = 0; a < d; a++)
-            if (B->nstep[(size_t)s * 3 + a] != P->siz[(size_t)s * d + a])
-                return rf_set_error(ctx, RF_E_ARG, "batch field and blend subdomain sizes differ");
-    }
-    // per field: the fuse header, then its axis tables
-    std::vector<char> blob;
-    std::vector<size_t> hOff((size_t)nSub), tOff((size_t)nSub * 3);
-    auto put = [&](const void* src, size_t bytes) {
-        const size_t off = (blob.size() + 15) / 16 * 16;
-        blob.resize(off + bytes);
-        std::memcpy(blob.data() + off, src, bytes);
-        return off;
-    };
-    std::vector<rf_blend_fuse> hdr((size_t)nSub);
-    for (int s = 0; s < nSub; s++) {
-        hOff[(size_t)s] = put(&hdr[(size_t)s], sizeof(rf_blend_fuse));      // filled below
-        for (int a = 0; a < 3; a++) {
-            std::vector<rf_fuse_axis> t;
-            if (a >= d) {
-                t.push_back(rf_fuse_axis{1.0, 1.0, 1, 0});
-            } else {
-                const int o = P->org[(size_t)s * d + a], z = P->siz[(size_t)s * d + a];
-                const int iv = P->subInt[(size_t)s * 3 + a];
-                for (int i = 0; i < z; i++) {
-                    const rf_blend_axis& e = P->axH[a][(size_t)(o + i)];
-                    const int c = e.iv[0] == iv ? 0 : 1;
-                    t.push_back(rf_fuse_axis{e.f[c], e.pu1, e.iv[1] < 0 ? 1 : 0, 0});
-                }
-            }
-            tOff[(size_t)s * 3 + a] = put(t.data(), t.size() * sizeof(rf_fuse_axis));
-        }
-    }
-    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    char* d_mem = nullptr;
-    RF_HIP_TRY(ctx, hipMalloc((void**)&d_mem, blob.size()));
-    for (int s = 0; s < nSub; s++) {
-        rf_blend_fuse& h = hdr[(size_t)s];
-        h = rf_blend_fuse{};
-        for (int a = 0; a < 3; a++) {
-            h.ax[a] = reinterpret_cast<const rf_fuse_axis*>(d_mem + tOff[(size_t)s * 3 + a]);
-            h.org[a] = a < d ? P->org[(size_t)s * d + a] : 0;
-            h.groupN[a] = P->groupN[a];
-        }
-        h.n0 = P->siz[(size_t)s * d];
-        h.nDim = d;
-        h.mode = P->mode;
-        h.groupTot = P->sep.groupTot;
-        h.group = d_group;
-        std::memcpy(blob.data() + hOff[(size_t)s], &h, sizeof(h));
-        last[(size_t)s].fuse = reinterpret_cast<const rf_blend_fuse*>(d_mem + hOff[(size_t)s]);
-    }
-    hipError_t e = hipMemcpy(d_mem, blob.data(), blob.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(B->d_desc[2], last.data(), last.size() * sizeof(rf_gemm_desc), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        for (auto& g : last) g.fuse = nullptr;
-        (void)hipFree(d_mem);
-        return rf_set_error(ctx, RF_E_HIP, std::string("fuse tables: ") + hipGetErrorString(e));
-    }
-    if (B->d_fuse) (void)hipFree(B->d_fuse);
-    B->d_fuse = d_mem;
-    P->sep.skipSingle = 1;
-    P->fusedGroup = d_group;
     return RF_OK;
 }

@@ -62,7 +62,6 @@ struct rf_blend_sep {
     int64_t groupTot;
     const rf_blend_axis* ax[3];   // [groupN[a]] (one unit entry for a >= nDim)
     const rf_blend_sub* sub;      // [i0 + nInt0 * (i1 + nInt1 * i2)]
-    int skipSingle;               // fused batch: single-covered points are written
 };
 hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s);
 

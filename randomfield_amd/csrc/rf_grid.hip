@@ -191,52 +191,6 @@ __device__ inline hv4 neg4(hv4 x) {
     return __builtin_bit_cast(hv4, __builtin_bit_cast(u2, x) ^ 0x80008000u);
 }
 
-// The last contraction's store with the blend fused (rf_blend_fuse): for
-// tile rows mRow.. (x_d) and this lane's column n (the plane index), a point
-// that no other subdomain covers is weighted, (REF_COMPAT, realization 0)
-// divided by the group taper, and written into the group grid with the
-// blend's operation order; any other point goes to the field for the blend.
-template <typename Map, typename T>
-__device__ inline void fused_store(const rf_gemm_desc& g, const rf_blend_fuse& F, Map map, T* D,
-                                   const f32x16& acc, float os, int mRow, int n, bool nIn, int half,
-                                   int batch, int64_t dBase, int rLocal0, int nLocal) {
-#pragma clang fp contract(off)
-    if (!nIn) return;
-    int i0, i1;
-    if (F.nDim == 3) {
-        i1 = n / F.n0;
-        i0 = n - i1 * F.n0;
-    } else {
-        i0 = n;
-        i1 = 0;
-    }
-    const rf_fuse_axis a0 = F.ax[0][i0];
-    const rf_fuse_axis a1 = F.nDim == 3 ? F.ax[1][i1] : F.ax[2][0];
-    T* G = static_cast<T*>(F.group) + (int64_t)batch * F.groupTot;
-    const int dSm = (int)g.dSm, dSn = (int)g.dSn;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const int rl = 8 * (i >> 2) + 4 * half + (i & 3);
-        const int m = mRow + rl;
-        if (m >= g.M) continue;
-        const T u = map(acc[i] * os);
-        // the row is axis 2 in 3-D, axis 1 in 2-D
-        const rf_fuse_axis ar = F.nDim == 3 ? F.ax[2][m] : F.ax[1][m];
-        const rf_fuse_axis& b1 = F.nDim == 3 ? a1 : ar;
-        const rf_fuse_axis& b2 = F.nDim == 3 ? ar : a1;
-        if (a0.single & b1.single & b2.single) {
-            const double w = (a0.w * b1.w) * b2.w;
-            double v = (double)u * w + 0.0;
-            if (F.mode == RF_REF_COMPAT && batch == 0) v = v / ((a0.pu1 * b1.pu1) * b2.pu1);
-            const int g1 = F.nDim == 3 ? F.org[1] + i1 : F.org[1] + m;
-            const int g2 = F.nDim == 3 ? F.org[2] + m : 0;
-            G[(F.org[0] + i0) + (int64_t)F.groupN[0] * (g1 + (int64_t)F.groupN[1] * g2)] = (T)v;
-        } else {
-            D[(rLocal0 + rl) * dSm + nLocal * dSn] = u;
-        }
-    }
-}
-
 template <bool RealOnly>
 __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n0, int batch, HStage* st) {
     const int tid = threadIdx.x;
@@ -359,11 +313,6 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         const int nn = ct * 32 + l32;
         const bool nIn = n0 + nn < g.N;
         auto store = [&](auto map, auto* D) {
-            if (g.fuse) {
-                fused_store(g, *g.fuse, map, D, acc0, os, m0 + rt * 32, n0 + nn, nIn, half, batch, dBase,
-                            rt * 32, nn);
-                return;
-            }
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);

@@ -187,27 +187,6 @@ inline rf_epilogue rf_make_epilogue(int margi, double avg, double var) {
 
 // structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice:
 // batched complex FP32 GEMMs D[b] = A[(b / aDiv) % aMod] B[b]
-// Blend fused into the last contraction of a localization batch: a field point
-// covered by no other subdomain (single along every axis) is weighted and
-// written straight into the group grid, exactly as the blend computes it;
-// the others go to the field for the blend, which then skips the single ones.
-struct rf_fuse_axis {
-    double w;             // taper factor of this field at the local index
-    double pu1;           // REF_COMPAT reverse taper of the group grid there
-    int32_t single;       // 1: no other interval covers the group index
-    int32_t pad;
-};
-struct rf_blend_fuse {
-    const rf_fuse_axis* ax[3];   // [local index] per axis (one unit entry beyond nDim)
-    int32_t org[3];              // field origin in the group grid
-    int32_t n0;                  // field points along axis 0
-    int32_t nDim, mode;
-    int32_t groupN[3];
-    int32_t pad;
-    int64_t groupTot;
-    void* group;                 // [Nmc][groupTot], the field's dtype
-};
-
 struct rf_gemm_desc {
     const float2* A; int64_t aSm, aSk, aStrideB; int aDiv, aMod;
     const float2* B; int64_t bSk, bSn, bStrideB;
@@ -219,7 +198,6 @@ struct rf_gemm_desc {
     int64_t tile0;        // grouped launches: first tile of this GEMM
     float outScale;       // D = outScale * (A B) (power of two)
     int split16;          // 1: FP16 hi/lo operands (3 products), 0: FP32 operands
-    const rf_blend_fuse* fuse;   // last contraction: fused blend (device), or NULL
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)

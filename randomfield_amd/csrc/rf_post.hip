@@ -327,9 +327,6 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
     const rf_blend_axis A0 = b.ax[0][gi0];
     const rf_blend_axis A1 = b.ax[1][gi1];
     const rf_blend_axis A2 = b.ax[2][gi2];
-    const bool single = A0.iv[1] < 0 && A1.iv[1] < 0 && A2.iv[1] < 0;
-    // fused with the batch: points of a single subdomain are already written
-    if (b.skipSingle && single) return;
     const double pu1 = (A0.pu1 * A1.pu1) * A2.pu1;
     constexpr int RB = 4;                 // realizations per pass
     for (int r0 = 0; r0 < b.Nmc; r0 += RB) {

@@ -229,65 +229,3 @@ def test_generated_grid_to_hdf5(ctx, tmp_path):
     body = txt.split("DATA {", 1)[1].rsplit("}", 3)[0]
     vals = np.array([float(v) for v in re.split(r"[,\s]+", body.strip()) if v])
     assert np.array_equal(vals, out[0].cpu().numpy())
-
-
-@pytest.mark.parametrize("mode", [R.FIXED, R.REF_COMPAT])
-@pytest.mark.parametrize("dt", ["f32", "f64"])
-@pytest.mark.parametrize("nDim", [3, 2])
-def test_fused_batch_blend_equals_unfused(ctx, mode, dt, nDim):
-    """rf_grid_batch_fuse_blend: the last contraction writes the points of a
-    single subdomain into the group grid and the blend adds the overlaps; the
-    group field is bit-identical to generation followed by the full blend,
-    also under graph replay."""
-    import torch
-    tdt = torch.float32 if dt == "f32" else torch.float64
-    code = R.DTYPE_F32 if dt == "f32" else R.DTYPE_F64
-    groupN, parts, nOvl, step, Nmc = (40, 36, 20)[:nDim], (3, 2, 2)[:nDim], 5, (0.25, 0.25, 0.25)[:nDim], 3
-    origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
-    specs = []
-    for s in range(len(sizes)):
-        n = tuple(int(v) for v in sizes[s])
-        specs.append((n, step, tuple(origins[s][a] * step[a] for a in range(nDim)), (1.0,) * nDim, Nmc, 100 + s + 1))
-    plans, outs, geo = _grid_fields(ctx, specs, tdt)
-    args = ([x[0] for x in geo], [x[1] for x in geo], [x[2] for x in geo])
-    # unfused reference
-    ref_batch = R.GridBatch(ctx, plans, *args, outs, code)
-    ref_bp = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * nDim, masks, mode, outs, code)
-    g_ref = torch.empty((Nmc, int(np.prod(groupN))), dtype=tdt, device="cuda")
-    ref_batch.execute()
-    ref_bp.execute(g_ref)
-    ctx.sync()
-    want = g_ref.cpu().numpy().copy()
-    ref_bp.close()
-    ref_batch.close()
-    # fused
-    batch = R.GridBatch(ctx, plans, *args, outs, code)
-    bp = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * nDim, masks, mode, outs, code)
-    g = torch.full((Nmc, int(np.prod(groupN))), 9.0, dtype=tdt, device="cuda")
-    batch.fuse_blend(bp, g)
-    for o in outs:
-        o.zero_()
-    batch.execute()
-    bp.execute(g)
-    ctx.sync()
-    assert np.array_equal(g.cpu().numpy(), want)
-    with pytest.raises(R.RFError):
-        bp.execute(torch.empty_like(g))                     # fused: only the fused group buffer
-    import torch as T
-    stream = T.cuda.Stream()
-    sp = stream.cuda_stream
-    graph = R.Graph(ctx, sp)
-    with graph:
-        batch.execute(sp)
-        bp.execute(g, sp)
-    for rep in range(2):
-        g.fill_(float(rep + 3))
-        T.cuda.synchronize()
-        graph.launch(sp)
-        T.cuda.synchronize()
-        assert np.array_equal(g.cpu().numpy(), want), rep
-    graph.close()
-    bp.close()
-    batch.close()
-    for pl in plans:
-        pl.close()

@@ -176,9 +176,13 @@ int rf_plan_autotune(rf_field_plan* plan, const double* d_x, int64_t N,
  * units, x fastest; the point order of set_XPoints/setGrid, mesh_RF.f90:44,
  * math_RF.f90:70-88), evaluated axis by axis: the Shinozuka lattice is a
  * tensor product per sign block, so sum_j C_j exp(i k.x) factorises into
- * three small contractions.  Method 2, FIXED mode, nDim 2 or 3 (otherwise
- * RF_E_UNSUPPORTED).  d_out: prod(nstep) x Nmc.  The plan keeps a scratch
- * buffer, grown on the first call with a larger grid. */
+ * three small contractions.  Method 2, FIXED mode, nDim 2 or 3, or method 4
+ * (gen_Std_Gauss_FFT: nstep = params.gridN, DCT-III tables); otherwise
+ * RF_E_UNSUPPORTED.  d_out: prod(nstep) x Nmc.  The plan keeps a scratch
+ * buffer, grown on the first call with a larger grid, and the folded
+ * coefficients, formed on the first call and reused while the scale is the
+ * same: calls on one plan are ordered by their stream, so calls on different
+ * streams must not overlap. */
 int rf_plan_execute_grid(rf_field_plan* plan, const double origin[3],
                          const double step[3], const int32_t nstep[3],
                          void* d_out, int32_t outDtype, void* stream);

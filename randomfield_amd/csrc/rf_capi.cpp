@@ -160,6 +160,7 @@ int rf_plan_destroy(rf_field_plan* P) {
     dfree(P->d_k); dfree(P->d_kRev); dfree(P->d_kIso); dfree(P->d_kIsoT);
     dfree(P->d_phase); dfree(P->d_amp); dfree(P->d_uni); dfree(P->d_phT);
     dfree(P->d_aT); dfree(P->d_B); dfree(P->d_Sk); dfree(P->d_calc); dfree(P->d_gridScratch);
+    if (P->d_gridCoef) (void)hipFree(P->d_gridCoef);
     delete P;
     return RF_OK;
 }
@@ -509,10 +510,18 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
         ampMult = (p.nDim == 3 ? 2.0 : 1.0) * std::sqrt(v);
     }
     rf_grid_stages st;
+    if (!P->d_gridCoef) {
+        int64_t nC = (int64_t)in.nSignBlocks * p.Nmc;
+        for (int a = 0; a < p.nDim; a++) nC *= in.kNStep[a];
+        RF_HIP_TRY(ctx, hipMalloc((void**)&P->d_gridCoef, (size_t)nC * sizeof(float2)));
+        P->gridCoefKey = NAN;
+    }
+    rf_grid_coef coef{P->d_gridCoef, P->gridCoefKey};
     RF_HIP_TRY(ctx, rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
                                     nstep, corrL, P->d_amp, P->d_phase, ampMult, ampMult * P->ampMax, P->epi, d_out,
                                     outDtype,
-                                    static_cast<float2*>(P->d_gridScratch), s, &st));
+                                    static_cast<float2*>(P->d_gridScratch), s, &st, &coef));
+    P->gridCoefKey = coef.key;
     RF_HIP_TRY(ctx, rf_grid_run(st, s));
     return RF_OK;
 }

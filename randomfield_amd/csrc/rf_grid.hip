@@ -533,7 +533,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
-                           float2* scratch, hipStream_t s, rf_grid_stages* st) {
+                           float2* scratch, hipStream_t s, rf_grid_stages* st, rf_grid_coef* coef) {
     // power-of-two scales for the FP16 operands: coefficients into [1/2, 1),
     // the middle contraction's output below 2^14 (|V| <= eMax K1 and
     // |W| <= eMax^2 K1 K2 for |table| <= eMax); the last stage undoes both
@@ -569,9 +569,18 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int a = 0; a < nDim; a++) kNu *= kNStep[a];
     const int64_t nC = (int64_t)Nmc * nb * kNu;          // = Nmc x nk3 x Q x nk2 x H x nk1
     float2* Cf = p; p += nC;
-    hipLaunchKernelGGL(k_coeffs_fold, dim3(blocks_for(nC)), dim3(256), 0, s, nC, nDim, fold, kNStep[0],
-                       nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp,
-                       kind == 1 ? nullptr : d_phase, ampMult * sc, Cf);
+    // a caller-owned coefficient buffer is reused while its scale matches
+    // (the coefficients depend only on the plan and the scale)
+    bool fresh = false;
+    if (coef) {
+        Cf = coef->buf;
+        fresh = coef->key == ampMult * sc;
+        coef->key = ampMult * sc;
+    }
+    if (!fresh)
+        hipLaunchKernelGGL(k_coeffs_fold, dim3(blocks_for(nC)), dim3(256), 0, s, nC, nDim, fold, kNStep[0],
+                           nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp,
+                           kind == 1 ? nullptr : d_phase, ampMult * sc, Cf);
     *st = rf_grid_stages{};
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;

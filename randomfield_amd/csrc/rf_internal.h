@@ -68,6 +68,8 @@ struct rf_field_plan {
     rf_epilogue epi{};
     void* d_gridScratch = nullptr;   // structured-grid form (grown on demand)
     size_t gridScratchBytes = 0;
+    float2* d_gridCoef = nullptr;    // folded grid coefficients (first grid execution)
+    double gridCoefKey = NAN;
     int mfmaVariant = 0;         // MFMA launch shape (rf_plan_set_variant / rf_plan_autotune)
 };
 
@@ -206,13 +208,20 @@ struct rf_grid_stages {
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
 // kind 0: Shinozuka lattice, tables exp(i k_j x); kind 1: method 4, tables of
 // the DCT-III (FFTW REDFT01) of each axis, w_j cos(pi j (2i+1) / (2n))
+// Folded coefficients kept across executions: buf holds Nmc * nb * prod(kNStep)
+// elements, valid for the scale key (NaN: not yet computed).
+struct rf_grid_coef {
+    float2* buf;
+    double key;
+};
 // coefMax bounds |ampMult * amp| (it sets the power-of-two scales that keep
-// the FP16 operands of the split-precision contractions in range).
+// the FP16 operands of the split-precision contractions in range).  coef may
+// be NULL (coefficients formed in scratch on every call).
 hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
                            const double* corrL, const double* d_amp, const double* d_phase,
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
-                           float2* scratch, hipStream_t s, rf_grid_stages* st);
+                           float2* scratch, hipStream_t s, rf_grid_stages* st, rf_grid_coef* coef = nullptr);
 hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);
 int64_t rf_gemm_tiles(const rf_gemm_desc& g);
 hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s);

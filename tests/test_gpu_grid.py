@@ -229,3 +229,36 @@ def test_generated_grid_to_hdf5(ctx, tmp_path):
     body = txt.split("DATA {", 1)[1].rsplit("}", 3)[0]
     vals = np.array([float(v) for v in re.split(r"[,\s]+", body.strip()) if v])
     assert np.array_equal(vals, out[0].cpu().numpy())
+
+
+@pytest.mark.parametrize("method", ["shinozuka", "fft"])
+def test_repeated_grid_calls_reuse_coefficients(ctx, method):
+    """A plan keeps its folded coefficients between grid executions: a second
+    grid (other origin and step; for method 4 another step, hence another
+    scale) gives the bits of a fresh plan."""
+    import torch
+
+    def make():
+        if method == "fft":
+            p = R.make_params(nDim=3, method=R.FFT, mode=R.FIXED, form=R.FORM_FP32, Nmc=2, xRange=(3.0, 2.0, 1.5),
+                              gridN=(13, 9, 7), seed=8)
+        else:
+            p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.FIXED, form=R.FORM_FP32, Nmc=2, xRange=(4, 3, 2),
+                              seed=8)
+        return R.FieldPlan(ctx, p)
+    grids = [((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), (13, 9, 7)), ((0.5, -1.0, 2.0), (0.3, 0.2, 0.4), (13, 9, 7))]
+    plan = make()
+    outs = []
+    for org, st, ns in grids:
+        o = torch.empty((2, int(np.prod(ns))), dtype=torch.float32, device="cuda")
+        plan.execute_grid(org, st, ns, o)
+        ctx.sync()
+        outs.append(o.cpu().numpy())
+    plan.close()
+    for (org, st, ns), got in zip(grids, outs):
+        fresh = make()
+        o = torch.empty((2, int(np.prod(ns))), dtype=torch.float32, device="cuda")
+        fresh.execute_grid(org, st, ns, o)
+        ctx.sync()
+        fresh.close()
+        assert np.array_equal(o.cpu().numpy(), got)

@@ -328,21 +328,30 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
     const rf_blend_axis A1 = b.ax[1][gi1];
     const rf_blend_axis A2 = b.ax[2][gi2];
     const double pu1 = (A0.pu1 * A1.pu1) * A2.pu1;
+    // the interval combinations covering this point (bit k: interval k & 1 of
+    // axis 0, (k >> 1) & 1 of axis 1, k >> 2 of axis 2), found once
+    uint32_t live = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int i0 = (k & 1) ? A0.iv[1] : A0.iv[0];
+        const int i1 = ((k >> 1) & 1) ? A1.iv[1] : A1.iv[0];
+        const int i2 = (k >> 2) ? A2.iv[1] : A2.iv[0];
+        if (i0 >= 0 && i1 >= 0 && i2 >= 0) live |= 1u << k;
+    }
     constexpr int RB = 4;                 // realizations per pass
     for (int r0 = 0; r0 < b.Nmc; r0 += RB) {
         double acc[RB];
 #pragma unroll
         for (int j = 0; j < RB; j++) acc[j] = 0.0;
-        // contributions in interval order; a combination is recomputed per
-        // pass (a few cached loads) to keep the register count low
-#pragma unroll 1
-        for (int k = 0; k < 8; k++) {
+        // contributions in interval order (ascending bits); a combination's
+        // address is recomputed per pass (a few cached loads)
+        for (uint32_t lm = live; lm; lm &= lm - 1) {
+            const int k = __builtin_ctz(lm);
             // selects, not indexing: the tables stay in registers
             const bool c0 = k & 1, c1 = (k >> 1) & 1, c2 = k >> 2;
             const int i0 = c0 ? A0.iv[1] : A0.iv[0];
             const int i1 = c1 ? A1.iv[1] : A1.iv[0];
             const int i2 = c2 ? A2.iv[1] : A2.iv[0];
-            if (i0 < 0 || i1 < 0 || i2 < 0) continue;
             const rf_blend_sub S = b.sub[i0 + b.nInt[0] * (i1 + b.nInt[1] * i2)];
             if (!S.field) continue;
             const int64_t s0 = c0 ? A0.sz[1] : A0.sz[0], s1 = c1 ? A1.sz[1] : A1.sz[0];

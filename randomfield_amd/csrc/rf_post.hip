@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
         const int i2 = (k >> 2) ? A2.iv[1] : A2.iv[0];
         if (i0 >= 0 && i1 >= 0 && i2 >= 0) live |= 1u << k;
     }
-    constexpr int RB = 4;                 // realizations per pass
+    constexpr int RB = 8;                 // realizations per pass
     for (int r0 = 0; r0 < b.Nmc; r0 += RB) {
         double acc[RB];
 #pragma unroll

@@ -310,13 +310,28 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
 }
 
 // Separable form: one thread per group point, all realizations; the <= 2^d
-// contributions come from the per-axis tables, the realization loop issues
-// 8 independent loads per contribution.  FP64 sums, each term v * PU
+// contributions come from the per-axis tables.  FP64 sums, each term v * PU
 // then + (localization_RF.f90:228,253); the PU product is taken per axis
 // (a reordering of the reference's lower-then-upper face product, equal up
 // to rounding) and contributions are added in interval order.
+//
+// Every field load is unconditional (realization index clamped, absent
+// contributions read a dummy address with stride 0 and are masked out of
+// the sum): a load under a per-element condition becomes a branch that waits
+// for its value before the next load issues, which leaves one load in
+// flight per wave.  A wave whose points have at most two contributions
+// takes a straight-line path chosen wave-uniformly (16 realizations of
+// loads in flight for one contribution, 2 x 8 for two); larger counts (the
+// corners where 3-D overlaps meet) take the general loop.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
+__device__ __forceinline__ void blend_store(T* p, double u) {
+    __builtin_nontemporal_store((T)u, p);   // written once, read by the next kernel
+}
+
+// 7 waves per SIMD (at most 72 VGPRs, no spills; unconstrained the
+// allocation is 81 VGPRs, 5 waves)
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
 #pragma clang fp contract(off)
     // block (x segment, gi1, gi2): the axis-1 and axis-2 entries are
     // block-uniform (scalar loads), one axis-0 entry per thread
@@ -328,8 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
     const rf_blend_axis A1 = b.ax[1][gi1];
     const rf_blend_axis A2 = b.ax[2][gi2];
     const double pu1 = (A0.pu1 * A1.pu1) * A2.pu1;
+    const bool fix0 = b.mode == RF_REF_COMPAT;      // realization 0 divided by pu1 (calls_RF.f90:355)
+    const int Nmc = b.Nmc;
     // the interval combinations covering this point (bit k: interval k & 1 of
-    // axis 0, (k >> 1) & 1 of axis 1, k >> 2 of axis 2), found once
+    // axis 0, (k >> 1) & 1 of axis 1, k >> 2 of axis 2)
     uint32_t live = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -338,16 +355,79 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
         const int i2 = (k >> 2) ? A2.iv[1] : A2.iv[0];
         if (i0 >= 0 && i1 >= 0 && i2 >= 0) live |= 1u << k;
     }
-    constexpr int RB = 8;                 // realizations per pass
-    for (int r0 = 0; r0 < b.Nmc; r0 += RB) {
+    // the first two contributions in interval order; absent ones read the
+    // axis-0 table entry with stride 0
+    const T* dummy = reinterpret_cast<const T*>(b.ax[0]);
+    const T* F0 = dummy; const T* F1 = dummy;
+    int64_t st0 = 0, st1 = 0;
+    double w0 = 0.0, w1 = 0.0;
+    int n = 0;
+    for (uint32_t lm = live; lm; lm &= lm - 1) {
+        const int k = __builtin_ctz(lm);
+        // selects, not indexing: the tables stay in registers
+        const bool c0 = k & 1, c1 = (k >> 1) & 1, c2 = k >> 2;
+        const int i0 = c0 ? A0.iv[1] : A0.iv[0];
+        const int i1 = c1 ? A1.iv[1] : A1.iv[0];
+        const int i2 = c2 ? A2.iv[1] : A2.iv[0];
+        const rf_blend_sub S = b.sub[i0 + b.nInt[0] * (i1 + b.nInt[1] * i2)];
+        if (!S.field) continue;
+        if (n == 2) { n = 3; break; }
+        const int64_t s0 = c0 ? A0.sz[1] : A0.sz[0], s1 = c1 ? A1.sz[1] : A1.sz[0];
+        const int64_t p0 = c0 ? A0.pos[1] : A0.pos[0], p1 = c1 ? A1.pos[1] : A1.pos[0];
+        const int64_t p2 = c2 ? A2.pos[1] : A2.pos[0];
+        const T* F = static_cast<const T*>(S.field) + p0 + p1 * s0 + p2 * s0 * s1;
+        const double w = ((c0 ? A0.f[1] : A0.f[0]) * (c1 ? A1.f[1] : A1.f[0])) * (c2 ? A2.f[1] : A2.f[0]);
+        if (n == 0) { F0 = F; st0 = S.subTot; w0 = w; }
+        else { F1 = F; st1 = S.subTot; w1 = w; }
+        n++;
+    }
+    if (!__any(n > 2)) {
+        if (!__any(n > 1)) {
+            constexpr int RB = 16;
+            for (int r0 = 0; r0 < Nmc; r0 += RB) {
+                T v[RB];
+#pragma unroll
+                for (int j = 0; j < RB; j++) v[j] = F0[(int64_t)min(r0 + j, Nmc - 1) * st0];
+#pragma unroll
+                for (int j = 0; j < RB; j++)
+                    if (r0 + j < Nmc) {
+                        double u = 0.0;
+                        if (n == 1) u = (double)v[j] * w0 + u;
+                        if (fix0 && r0 + j == 0) u = u / pu1;
+                        blend_store(group + (int64_t)(r0 + j) * b.groupTot + g, u);
+                    }
+            }
+        } else {
+            constexpr int RB = 8;
+            for (int r0 = 0; r0 < Nmc; r0 += RB) {
+                T v0[RB], v1[RB];
+#pragma unroll
+                for (int j = 0; j < RB; j++) {
+                    const int64_t r = min(r0 + j, Nmc - 1);
+                    v0[j] = F0[r * st0];
+                    v1[j] = F1[r * st1];
+                }
+#pragma unroll
+                for (int j = 0; j < RB; j++)
+                    if (r0 + j < Nmc) {
+                        double u = 0.0;
+                        if (n >= 1) u = (double)v0[j] * w0 + u;
+                        if (n >= 2) u = (double)v1[j] * w1 + u;
+                        if (fix0 && r0 + j == 0) u = u / pu1;
+                        blend_store(group + (int64_t)(r0 + j) * b.groupTot + g, u);
+                    }
+            }
+        }
+        return;
+    }
+    // general loop: every contribution, 8 realizations per pass
+    constexpr int RB = 8;
+    for (int r0 = 0; r0 < Nmc; r0 += RB) {
         double acc[RB];
 #pragma unroll
         for (int j = 0; j < RB; j++) acc[j] = 0.0;
-        // contributions in interval order (ascending bits); a combination's
-        // address is recomputed per pass (a few cached loads)
         for (uint32_t lm = live; lm; lm &= lm - 1) {
             const int k = __builtin_ctz(lm);
-            // selects, not indexing: the tables stay in registers
             const bool c0 = k & 1, c1 = (k >> 1) & 1, c2 = k >> 2;
             const int i0 = c0 ? A0.iv[1] : A0.iv[0];
             const int i1 = c1 ? A1.iv[1] : A1.iv[0];
@@ -361,16 +441,16 @@ __global__ __launch_bounds__(kBlock) void k_blend_sep(rf_blend_sep b, T* __restr
             const double w = ((c0 ? A0.f[1] : A0.f[0]) * (c1 ? A1.f[1] : A1.f[0])) * (c2 ? A2.f[1] : A2.f[0]);
             T v[RB];
 #pragma unroll
-            for (int j = 0; j < RB; j++) v[j] = (r0 + j < b.Nmc) ? F[(int64_t)(r0 + j) * S.subTot] : T(0);
+            for (int j = 0; j < RB; j++) v[j] = F[(int64_t)min(r0 + j, Nmc - 1) * S.subTot];
 #pragma unroll
             for (int j = 0; j < RB; j++) acc[j] = (double)v[j] * w + acc[j];
         }
 #pragma unroll
         for (int j = 0; j < RB; j++)
-            if (r0 + j < b.Nmc) {
+            if (r0 + j < Nmc) {
                 double u = acc[j];
-                if (b.mode == RF_REF_COMPAT && r0 + j == 0) u = u / pu1;   // calls_RF.f90:355
-                group[(int64_t)(r0 + j) * b.groupTot + g] = (T)u;
+                if (fix0 && r0 + j == 0) u = u / pu1;
+                blend_store(group + (int64_t)(r0 + j) * b.groupTot + g, u);
             }
     }
 }
@@ -513,8 +593,8 @@ hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, 
     if (b.groupN[1] > 65535 || b.groupN[2] > 65535) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((b.groupN[0] + kBlock - 1) / kBlock), (unsigned)b.groupN[1], (unsigned)b.groupN[2]);
     if (dtype == RF_DTYPE_F64)
-        hipLaunchKernelGGL(k_blend_sep<double>, dim3(grid), dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
+        hipLaunchKernelGGL(k_blend_sep<double>, grid, dim3(kBlock), 0, s, b, static_cast<double*>(d_group));
     else
-        hipLaunchKernelGGL(k_blend_sep<float>, dim3(grid), dim3(kBlock), 0, s, b, static_cast<float*>(d_group));
+        hipLaunchKernelGGL(k_blend_sep<float>, grid, dim3(kBlock), 0, s, b, static_cast<float*>(d_group));
     return hipGetLastError();
 }

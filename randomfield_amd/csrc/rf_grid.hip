@@ -158,6 +158,11 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
 // LDS read.  In the complex product a 32-row MFMA tile holds 16 complex rows,
 // Re D in rows 0-15 ([Re A | -Im A] against [Re B ; Im B]) and Im D in rows
 // 16-31 ([Im A | Re A]); K' = 16 covers 8 complex k, lanes 0-31 the Re half.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef const f2v __attribute__((address_space(1))) gf2;      // global-memory views
+typedef f2v __attribute__((address_space(1))) gf2o;
+typedef float __attribute__((address_space(1))) gfo;
+typedef double __attribute__((address_space(1))) gdo;
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
@@ -197,8 +202,12 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
     // tile base pointers in 64-bit, offsets inside the tile in 32-bit (the
     // launcher keeps K * stride and 64 * stride below 2^31)
-    const float2* A = g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm;
-    const float2* B = g.B + (int64_t)batch * g.bStrideB + (int64_t)n0 * g.bSn;
+    // global address space: the descriptor of a grouped launch is read from
+    // memory, and generic (flat) loads would also count against the LDS
+    // wait counter, so every LDS read of the K-step would wait for the
+    // prefetch of the next one
+    const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + (int64_t)n0 * g.bSn);
     const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk, bSn = (int)g.bSn;
     f32x16 acc0, acc1;
 #pragma unroll
@@ -211,17 +220,21 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     const int aRow = tid >> 2, aK = (tid & 3) * 4;
     const int bCol = tid & 63, bK = (tid >> 6) * 4;
     const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
-    const int aOff = aRow * aSm, bOff = bCol * bSn;
-    float2 ra[4], rb[4];
+    // rows / columns / k outside the operand read an in-range element (row 0
+    // of the tile, k = K - 1) and are zeroed when staged: no load sits under
+    // a per-lane branch, and nothing waits for the prefetch before the MFMAs
+    const int aOff = (aIn ? aRow : 0) * aSm, bOff = (bIn ? bCol : 0) * bSn;
+    const int kLast = g.K - 1;
+    f2v ra[4], rb[4];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int ka = k0 + aK + i, kb = k0 + bK + i;
-            ra[i] = (aIn && ka < g.K) ? A[aOff + ka * aSk] : make_float2(0.f, 0.f);
-            rb[i] = (bIn && kb < g.K) ? B[kb * bSk + bOff] : make_float2(0.f, 0.f);
+            ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+            rb[i] = (bIn && kb <= kLast) ? B[kb * bSk + bOff] : f2v{0.f, 0.f};
         }
     };
-    auto stash = [&]() {
+    auto stash = [&](int k0) {
         float re[4], im[4];
         hv4 hi, lo;
 #pragma unroll
@@ -257,7 +270,7 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     const int bPlane = half ? 2 : 0;
     fetch(0);
     for (int k0 = 0; k0 < g.K; k0 += kTK) {
-        stash();
+        stash(k0);
         __syncthreads();
         if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
 #pragma unroll
@@ -296,7 +309,7 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + (int64_t)n0 * g.dSn;
     const int dSm = (int)g.dSm, dSn = (int)g.dSn;
     if (!RealOnly) {
-        float2* D = static_cast<float2*>(g.D) + dBase;
+        gf2o* D = (gf2o*)(static_cast<float2*>(g.D) + dBase);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int r = wave * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
@@ -305,8 +318,8 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
             for (int c = 0; c < 2; c++) {
                 const int nn = c * 32 + l32;
                 if (n0 + nn >= g.N) continue;
-                D[r * dSm + nn * dSn] = c ? make_float2(acc1[i] * os, acc1[i + 8] * os)
-                                          : make_float2(acc0[i] * os, acc0[i + 8] * os);
+                D[r * dSm + nn * dSn] = c ? f2v{acc1[i] * os, acc1[i + 8] * os}
+                                          : f2v{acc0[i] * os, acc0[i + 8] * os};
             }
         }
     } else {
@@ -320,8 +333,8 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
             }
         };
         const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
-        float* D32 = static_cast<float*>(g.D) + dBase;
-        double* D64 = static_cast<double*>(g.D) + dBase;
+        gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
+        gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
         if (g.outDtype == RF_DTYPE_F64) {
             if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
             else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);

@@ -22,6 +22,14 @@ namespace {
 
 constexpr int kTM = 64, kTN = 64, kTK = 16;   // block tile; 256 threads, 4x4 each
 
+// element offsets of column n in B and in D (two-level when nLo > 0)
+__device__ __forceinline__ int64_t col_b(const rf_gemm_desc& g, int n) {
+    return g.nLo ? (int64_t)(n / g.nLo) * g.bSnHi + (int64_t)(n % g.nLo) * g.bSn : (int64_t)n * g.bSn;
+}
+__device__ __forceinline__ int64_t col_d(const rf_gemm_desc& g, int n) {
+    return g.nLo ? (int64_t)(n / g.nLo) * g.dSnHi + (int64_t)(n % g.nLo) * g.dSn : (int64_t)n * g.dSn;
+}
+
 __device__ inline float epi32(float u, const rf_epilogue& e) {
     if (e.margi == RF_FOM_NONE) return u;
     const float y = fmaf(u, (float)e.scale, (float)e.shift);
@@ -69,7 +77,7 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
             ra[i] = (m < g.M && k < g.K) ? A[m * g.aSm + k * g.aSk] : make_float2(0.f, 0.f);
             const int kb = e / kTN, nn = e % kTN;
             const int n = n0 + nn, k2 = k0 + kb;
-            rb[i] = (n < g.N && k2 < g.K) ? B[k2 * g.bSk + n * g.bSn] : make_float2(0.f, 0.f);
+            rb[i] = (n < g.N && k2 < g.K) ? B[k2 * g.bSk + col_b(g, n)] : make_float2(0.f, 0.f);
         }
     };
     auto stash = [&]() {
@@ -130,7 +138,7 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
             for (int c = 0; c < 2; c++) {
                 const int n = n0 + c * 32 + l32;
                 if (n >= g.N) continue;
-                const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
+                const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + col_d(g, n);
                 static_cast<float2*>(g.D)[o] = c ? make_float2(acc1[i] * g.outScale, acc1[i + 1] * g.outScale)
                                                  : make_float2(acc0[i] * g.outScale, acc0[i + 1] * g.outScale);
             }
@@ -141,7 +149,7 @@ __device__ __forceinline__ void gemm_tile(const rf_gemm_desc& g, int m0, int n0,
             const int m = m0 + rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
             const int n = n0 + ct * 32 + l32;
             if (m >= g.M || n >= g.N) continue;
-            const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + n * g.dSn;
+            const int64_t o = (int64_t)batch * g.dStrideB + m * g.dSm + col_d(g, n);
             const float u = epi32(acc0[i] * g.outScale, g.epi);
             if (g.outDtype == RF_DTYPE_F64) static_cast<double*>(g.D)[o] = (double)u;
             else static_cast<float*>(g.D)[o] = u;
@@ -207,8 +215,9 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     // wait counter, so every LDS read of the K-step would wait for the
     // prefetch of the next one
     const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
-    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + (int64_t)n0 * g.bSn);
-    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk, bSn = (int)g.bSn;
+    const int64_t bCol0 = col_b(g, n0);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + bCol0);
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk;
     f32x16 acc0, acc1;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -223,7 +232,7 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     // rows / columns / k outside the operand stage zeros; the prefetch
     // loads are waited for only when staged (a clamped, branch-free form
     // with the zeroing at staging time measured 5 % slower: more VALU)
-    const int aOff = aRow * aSm, bOff = bCol * bSn;
+    const int aOff = aRow * aSm, bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
     const int kLast = g.K - 1;
     f2v ra[4], rb[4];
     auto fetch = [&](int k0) {
@@ -306,10 +315,14 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     }
     // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
     const float os = g.outScale;
-    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + (int64_t)n0 * g.dSn;
-    const int dSm = (int)g.dSm, dSn = (int)g.dSn;
+    const int64_t dCol0 = col_d(g, n0);
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + dCol0;
+    const int dSm = (int)g.dSm;
+    // this lane's column offsets (32-bit inside the tile)
+    auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
     if (!RealOnly) {
         gf2o* D = (gf2o*)(static_cast<float2*>(g.D) + dBase);
+        const int oc[2] = {dcol(l32), dcol(32 + l32)};
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int r = wave * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
@@ -318,18 +331,19 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
             for (int c = 0; c < 2; c++) {
                 const int nn = c * 32 + l32;
                 if (n0 + nn >= g.N) continue;
-                D[r * dSm + nn * dSn] = c ? f2v{acc1[i] * os, acc1[i + 8] * os}
+                D[r * dSm + oc[c]] = c ? f2v{acc1[i] * os, acc1[i + 8] * os}
                                           : f2v{acc0[i] * os, acc0[i + 8] * os};
             }
         }
     } else {
         const int nn = ct * 32 + l32;
         const bool nIn = n0 + nn < g.N;
+        const int oc = dcol(nn);
         auto store = [&](auto map, auto* D) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
-                if (m0 + r < g.M && nIn) D[r * dSm + nn * dSn] = map(acc0[i] * os);
+                if (m0 + r < g.M && nIn) D[r * dSm + oc] = map(acc0[i] * os);
             }
         };
         const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
@@ -608,13 +622,15 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     int64_t plane = n[0];                      // points already resolved
     int64_t lastStrideB = (int64_t)kNStep[1] * n[0];
     if (nDim == 3) {
-        // stage 2, batch (r, j3): W[x2][x1] = sum_{q,j2} E2^q[j2][x2] V[q][j2][x1]
+        // stage 2, batch r, columns (j3, x1):
+        // W[j3][x2][x1] = sum_{q,j2} E2^q[j2][x2] V[j3][q][j2][x1]
         float2* W = p; p += (int64_t)Nmc * kNStep[2] * n[1] * n[0];
         rf_gemm_desc& h = st->g[st->n++];
         h.A = E[1]; h.aSm = 1; h.aSk = n[1]; h.aStrideB = 0; h.aDiv = 1; h.aMod = 1;
-        h.B = V; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)K2 * n[0];
-        h.D = W; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)n[1] * n[0];
-        h.M = n[1]; h.N = n[0]; h.K = K2; h.batches = Nmc * kNStep[2]; h.realOut = 0;
+        h.B = V; h.bSk = n[0]; h.bSn = 1; h.bStrideB = (int64_t)kNStep[2] * K2 * n[0];
+        h.D = W; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)kNStep[2] * n[1] * n[0];
+        h.nLo = n[0]; h.bSnHi = (int64_t)K2 * n[0]; h.dSnHi = (int64_t)n[1] * n[0];
+        h.M = n[1]; h.N = kNStep[2] * n[0]; h.K = K2; h.batches = Nmc; h.realOut = 0;
         h.outScale = (float)std::ldexp(1.0, -m2); h.split16 = split;
         last = W;
         plane = (int64_t)n[0] * n[1];
@@ -633,8 +649,8 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];
         const int64_t lim = (int64_t)1 << 30;
-        if ((int64_t)q.K * q.aSk + 64 * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn >= lim ||
-            64 * q.dSm + 64 * q.dSn >= lim)
+        if ((int64_t)q.K * q.aSk + 64 * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn + q.bSnHi >= lim ||
+            64 * q.dSm + 64 * q.dSn + q.dSnHi >= lim)
             for (int j = 0; j < st->n; j++) st->g[j].split16 = 0;
     }
     return hipGetLastError();

@@ -200,6 +200,12 @@ struct rf_gemm_desc {
     int64_t tile0;        // grouped launches: first tile of this GEMM
     float outScale;       // D = outScale * (A B) (power of two)
     int split16;          // 1: FP16 hi/lo operands (3 products), 0: FP32 operands
+    // two-level columns (nLo > 0): column n = (n / nLo, n % nLo) sits at
+    // (n / nLo) * xSnHi + (n % nLo) * xSn in B and in D, so one GEMM spans
+    // several batch blocks along N (a short axis is then padded once, not
+    // once per block)
+    int nLo;
+    int64_t bSnHi, dSnHi;
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)

@@ -217,13 +217,20 @@ __global__ __launch_bounds__(kBlock) void k_interp(rf_interp_dev q, const TG* __
         w[j] = wj;
         idx[j] = id;
     }
+    // the 8 corner loads of a realization are unconditional (unused corners
+    // read entry 0) so they issue together; only the first nc enter the sum
 #pragma unroll
     for (int rr = 0; rr < kInterpR; rr++) {
         const int r = r0 + rr;
         if (r >= q.Nmc) break;
         const TG* g = grid + (int64_t)r * q.Ng;
+        TG v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = g[idx[j]];
         double acc = 0.0;
-        for (int j = 0; j < nc; j++) acc = acc + (double)g[idx[j]] * w[j];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < nc) acc = acc + (double)v[j] * w[j];
         out[(int64_t)r * q.nNodes + i] = (TO)acc;
     }
 }
@@ -270,11 +277,22 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
             loc[h] = E.base + gi[0] + (int64_t)gi[1] * E.stride1 + (int64_t)gi[2] * E.stride2;
             // PU product in the reference's order: lower faces of axes 1..d,
             // then upper faces (localization_RF.f90:755-851)
+            // (all six factor loads issued together; axes beyond nDim read
+            // entry 0 and multiply by an exact 1.0)
             const double* lf = b.lowF + (int64_t)E.s * 3 * b.maxSize;
             const double* uf = b.upF + (int64_t)E.s * 3 * b.maxSize;
+            double fl[3], fu[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int pos = a < b.nDim ? gi[a] - E.o[a] : 0;
+                fl[a] = lf[a * b.maxSize + pos];
+                fu[a] = uf[a * b.maxSize + pos];
+            }
             double pu = 1.0;
-            for (int a = 0; a < b.nDim; a++) pu = pu * lf[a * b.maxSize + (gi[a] - E.o[a])];
-            for (int a = 0; a < b.nDim; a++) pu = pu * uf[a * b.maxSize + (gi[a] - E.o[a])];
+#pragma unroll
+            for (int a = 0; a < 3; a++) pu = pu * (a < b.nDim ? fl[a] : 1.0);
+#pragma unroll
+            for (int a = 0; a < 3; a++) pu = pu * (a < b.nDim ? fu[a] : 1.0);
             w[h] = pu;
         }
     }
@@ -288,15 +306,16 @@ __global__ __launch_bounds__(kBlock) void k_blend(rf_blend_dev b, T* __restrict_
         double acc[kBlendR];
 #pragma unroll
         for (int j = 0; j < kBlendR; j++) acc[j] = 0.0;
+        // the chunk's loads are unconditional (realization index clamped), so
+        // they issue together instead of one branch and wait per load
 #pragma unroll
         for (int h = 0; h < kBlendMaxEntries; h++) {
             if (h < nh) {
+                T v[kBlendR];
 #pragma unroll
-                for (int j = 0; j < kBlendR; j++)
-                    if (r0 + j < b.Nmc) {
-                        const double v = (double)F[h][(int64_t)(r0 + j) * rs[h] + loc[h]];
-                        acc[j] = v * w[h] + acc[j];
-                    }
+                for (int j = 0; j < kBlendR; j++) v[j] = F[h][(int64_t)min(r0 + j, b.Nmc - 1) * rs[h] + loc[h]];
+#pragma unroll
+                for (int j = 0; j < kBlendR; j++) acc[j] = (double)v[j] * w[h] + acc[j];
             }
         }
 #pragma unroll

@@ -478,7 +478,10 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // contraction runs over nk3 instead of 4 nk3.  Without sign blocks (method 4)
 // Cv = C.
 //
-// Cv[r][j3][q][j2][h][j1] = (conj if h) ampMult amp[r][b][j][j1] exp(i phase).
+// Cv[r][j3][q][j2][h][j1] = (conj if h) ampMult amp[r][b][j][j1] exp(i phase),
+// stored K-major: element (row m = (r, j3, q, j2), k = (h, j1)) at k * rows + m,
+// so the first contraction's operand loads run along rows (whole cache lines
+// per wave) instead of 16 rows of 32 bytes each.
 // ph == NULL: real coefficients (method 4), no phase factor.
 __global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, int nk3, int64_t kNu,
                               int64_t kNr, const double* __restrict__ amp, const double* __restrict__ ph,
@@ -486,9 +489,11 @@ __global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, i
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
     const int H = fold ? 2 : 1, Q = (fold && nDim == 3) ? 2 : 1;
-    int64_t rem = e;
-    const int j1 = (int)(rem % nk1); rem /= nk1;
-    const int h = (int)(rem % H); rem /= H;
+    const int64_t rows = n / ((int64_t)H * nk1);
+    const int64_t kk = e / rows;
+    int64_t rem = e - kk * rows;
+    const int j1 = (int)(kk % nk1);
+    const int h = (int)(kk / nk1);
     const int j2 = (int)(rem % nk2); rem /= nk2;
     const int q = (int)(rem % Q); rem /= Q;
     const int j3 = nDim == 3 ? (int)(rem % nk3) : 0;
@@ -613,7 +618,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     const int64_t rows1 = nC / K1;
     float2* V = p; p += rows1 * n[0];
     rf_gemm_desc& g = st->g[st->n++];
-    g.A = Cf; g.aSm = K1; g.aSk = 1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;
+    g.A = Cf; g.aSm = 1; g.aSk = rows1; g.aStrideB = 0; g.aDiv = 1; g.aMod = 1;   // K-major
     g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;     // K (h, j1): variant h of E1
     g.D = V; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
     g.M = (int)rows1; g.N = n[0]; g.K = K1; g.batches = 1; g.realOut = 0;

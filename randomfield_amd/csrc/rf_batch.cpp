@@ -24,6 +24,7 @@ struct rf_grid_batch {
     int nDesc[3] = {0, 0, 0};
     int64_t tiles[3] = {0, 0, 0};
     int realOut[3] = {0, 0, 1};
+    int rwMax[3] = {1, 1, 1};           // largest tile-row factor of a stage's GEMMs
 };
 
 struct rf_graph {
@@ -125,6 +126,7 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         for (auto& g : stage[k]) {
             g.tile0 = t;
             t += rf_gemm_tiles(g);
+            if (g.rw == 2) B->rwMax[k] = 2;
         }
         B->tiles[k] = t;
         B->nDesc[k] = (int)stage[k].size();
@@ -152,7 +154,7 @@ int rf_grid_batch_execute(rf_grid_batch* B, void* stream) {
     rf_context* ctx = B->ctx;
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     for (int k = 0; k < 3; k++)
-        RF_HIP_TRY(ctx, rf_launch_gemm_group(B->realOut[k], B->d_desc[k], B->nDesc[k], B->tiles[k], s));
+        RF_HIP_TRY(ctx, rf_launch_gemm_group(B->realOut[k], B->d_desc[k], B->nDesc[k], B->tiles[k], B->rwMax[k], s));
     return RF_OK;
 }
 

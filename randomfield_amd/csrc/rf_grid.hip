@@ -174,8 +174,11 @@ typedef double __attribute__((address_space(1))) gdo;
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
+// RW = 2: 128-row tiles of the complex product (two 16-row groups per wave
+// sharing each B fragment read)
+template <int RW>
 struct HStage {
-    _Float16 a[6][kTM][kHP];                  // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
+    _Float16 a[6][kTM * RW][kHP];             // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
     _Float16 b[4][kTN][kHP];                  // Re hi, Re lo, Im hi, Im lo
 };
 
@@ -205,7 +208,7 @@ __device__ inline hv4 neg4(hv4 x) {
 }
 
 template <bool RealOnly>
-__device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n0, int batch, HStage* st) {
+__device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n0, int batch, HStage<1>* st) {
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
     // tile base pointers in 64-bit, offsets inside the tile in 32-bit (the
@@ -361,16 +364,150 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
     }
 }
 
+// 128-row tiles of the complex product (used where the row axis is long,
+// the first contraction): each wave owns two 16-complex-row groups and reads
+// each B fragment once for both (8 fragment reads per 12 MFMAs instead of 6
+// per 6); staging, operand split and epilogue as gemm_tile_h.
+__device__ __forceinline__ void gemm_tile_c128(const rf_gemm_desc& g, int m0, int n0, int batch, HStage<2>* st) {
+    constexpr int RW = 2;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
+    const int64_t bCol0 = col_b(g, n0);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + bCol0);
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk;
+    f32x16 acc[RW][2];
+#pragma unroll
+    for (int rr = 0; rr < RW; rr++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            acc[rr][0][i] = 0.0f;
+            acc[rr][1][i] = 0.0f;
+        }
+    // staging: A rows t/4 and t/4 + 64, k 4(t%4)..+3; B column t%64, k 4(t/64)..+3
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 63, bK = (tid >> 6) * 4;
+    bool aIn[RW];
+#pragma unroll
+    for (int rr = 0; rr < RW; rr++) aIn[rr] = m0 + aRow + 64 * rr < g.M;
+    const bool bIn = n0 + bCol < g.N;
+    const int aOff = aRow * aSm, bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
+    const int kLast = g.K - 1;
+    f2v ra[RW][4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i, kb = k0 + bK + i;
+#pragma unroll
+            for (int rr = 0; rr < RW; rr++)
+                ra[rr][i] = (aIn[rr] && ka <= kLast) ? A[aOff + 64 * rr * aSm + ka * aSk] : f2v{0.f, 0.f};
+            rb[i] = (bIn && kb <= kLast) ? B[kb * bSk + bOff] : f2v{0.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int rr = 0; rr < RW; rr++) {
+            const int row = aRow + 64 * rr;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i] = ra[rr][i].x;
+                im[i] = ra[rr][i].y;
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->a[0][row][aK]) = hi;
+            *reinterpret_cast<hv4*>(&st->a[1][row][aK]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->a[2][row][aK]) = hi;
+            *reinterpret_cast<hv4*>(&st->a[3][row][aK]) = lo;
+            *reinterpret_cast<hv4*>(&st->a[4][row][aK]) = neg4(hi);   // split(-x) = -split(x)
+            *reinterpret_cast<hv4*>(&st->a[5][row][aK]) = neg4(lo);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = rb[i].x;
+            im[i] = rb[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->b[0][bCol][bK]) = hi;
+        *reinterpret_cast<hv4*>(&st->b[1][bCol][bK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->b[2][bCol][bK]) = hi;
+        *reinterpret_cast<hv4*>(&st->b[3][bCol][bK]) = lo;
+    };
+    // operand planes as in gemm_tile_h's complex product; row group rr of
+    // this wave starts at row (2 wave + rr) 16
+    const int isIm = l32 >> 4;
+    const int aPlane = isIm ? (half ? 0 : 2) : (half ? 4 : 0);
+    const int aRowL = wave * 16 * RW + (l32 & 15);
+    const int bPlane = half ? 2 : 0;
+    const bool c1 = n0 + 32 < g.N;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + wave * 16 * RW < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (c == 1 && !c1) break;                                      // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][32 * c + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][32 * c + l32][kk]);
+#pragma unroll
+                    for (int rr = 0; rr < RW; rr++) {
+                        if (rr > 0 && m0 + (wave * RW + rr) * 16 >= g.M) break;   // uniform
+                        const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL + 16 * rr][kk]);
+                        const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL + 16 * rr][kk]);
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[rr][c], 0, 0, 0);
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[rr][c], 0, 0, 0);
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rr][c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dCol0 = col_d(g, n0);
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + dCol0;
+    const int dSm = (int)g.dSm;
+    auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
+    gf2o* D = (gf2o*)(static_cast<float2*>(g.D) + dBase);
+    const int oc[2] = {dcol(l32), dcol(32 + l32)};
+#pragma unroll
+    for (int rr = 0; rr < RW; rr++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int r = (wave * RW + rr) * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
+            if (m0 + r >= g.M) continue;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int nn = c * 32 + l32;
+                if (n0 + nn >= g.N) continue;
+                D[r * dSm + oc[c]] = f2v{acc[rr][c][i] * os, acc[rr][c][i + 8] * os};
+            }
+        }
+}
+
 struct FStage {
     float2 a[kTK][kTM + 1];
     float2 b[kTK][kTN + 1];
 };
-constexpr size_t kStageBytes = sizeof(HStage) > sizeof(FStage) ? sizeof(HStage) : sizeof(FStage);
+template <int RW>
+constexpr size_t kStageBytes = sizeof(HStage<RW>) > sizeof(FStage) ? sizeof(HStage<RW>) : sizeof(FStage);
 
-template <bool RealOnly>
+// RW = 2 only with FP16 operands (the host sets rw = 1 for the FP32 path)
+template <bool RealOnly, int RW = 1>
 __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, int batch, char* smem) {
-    if (g.split16) {
-        gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage*>(smem));
+    if constexpr (RW == 2) {
+        static_assert(!RealOnly, "128-row tiles are for the complex product");
+        gemm_tile_c128(g, m0, n0, batch, reinterpret_cast<HStage<2>*>(smem));
+    } else if (g.split16) {
+        gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage<1>*>(smem));
     } else {
         FStage* f = reinterpret_cast<FStage*>(smem);
         gemm_tile<RealOnly>(g, m0, n0, batch, f->a, f->b);
@@ -388,7 +525,8 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
 constexpr int kXcd = 8;
 
 __device__ inline void tile_coords(const rf_gemm_desc& g, int64_t local, int* mI, int* nI, int* batch) {
-    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + kTM - 1) / kTM;
+    const int tm = kTM * (g.rw == 2 ? 2 : 1);
+    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + tm - 1) / tm;
     if (ntM <= ntN) {
         *mI = (int)(local % ntM);
         const int64_t r = local / ntM;
@@ -407,21 +545,22 @@ __device__ inline int64_t xcd_tile(int64_t b, int64_t nTiles) {
     return (b % kXcd) * per + b / kXcd;
 }
 
-template <bool RealOnly>
+template <bool RealOnly, int RW>
 __global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
-    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RW>];
     const int64_t L = xcd_tile(blockIdx.x, nTiles);
     if (L >= nTiles) return;
     int mI, nI, batch;
     tile_coords(g, L, &mI, &nI, &batch);
-    run_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, smem);
+    run_tile<RealOnly, RW>(g, mI * kTM * RW, nI * kTN, batch, smem);
 }
 
 // Grouped form: the GEMMs of several fields in one launch; GEMM q owns
 // logical tiles tile0 .. tile0 + tiles(q) - 1.
-template <bool RealOnly>
+// RWMAX = 2: some GEMMs of the group use 128-row tiles (rw = 2)
+template <bool RealOnly, int RWMAX>
 __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
-    __shared__ __attribute__((aligned(16))) char smem[kStageBytes];
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RWMAX>];
     const int64_t b = xcd_tile(blockIdx.x, nTiles);
     if (b >= nTiles) return;
     int lo = 0, hi = nG - 1;
@@ -433,7 +572,8 @@ __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restr
     const rf_gemm_desc g = gs[lo];                 // uniform: scalar registers
     int mI, nI, batch;
     tile_coords(g, b - g.tile0, &mI, &nI, &batch);
-    run_tile<RealOnly>(g, mI * kTM, nI * kTN, batch, smem);
+    if (!RealOnly && RWMAX == 2 && g.rw == 2) run_tile<RealOnly, RWMAX>(g, mI * kTM * 2, nI * kTN, batch, smem);
+    else run_tile<RealOnly, 1>(g, mI * kTM, nI * kTN, batch, smem);
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -517,7 +657,8 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 int64_t rf_gemm_tiles(const rf_gemm_desc& g) {
-    return (int64_t)((g.N + kTN - 1) / kTN) * ((g.M + kTM - 1) / kTM) * g.batches;
+    const int tm = kTM * (g.rw == 2 ? 2 : 1);
+    return (int64_t)((g.N + kTN - 1) / kTN) * ((g.M + tm - 1) / tm) * g.batches;
 }
 
 // blocks of a launch over nTiles logical tiles (a multiple of the XCD count)
@@ -528,18 +669,23 @@ hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
     if (nTiles <= 0) return hipSuccess;
     const int64_t blocks = xcd_blocks(nTiles);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (g.realOut) hipLaunchKernelGGL(k_cgemm<true>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
-    else hipLaunchKernelGGL(k_cgemm<false>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    if (g.realOut) hipLaunchKernelGGL((k_cgemm<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else if (g.rw == 2) hipLaunchKernelGGL((k_cgemm<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else hipLaunchKernelGGL((k_cgemm<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
 }
 
-hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles,
+hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles, int rwMax,
                                 hipStream_t s) {
     if (nG <= 0 || nTiles <= 0) return hipSuccess;
     const int64_t blocks = xcd_blocks(nTiles);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (realOut) hipLaunchKernelGGL(k_cgemm_group<true>, dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
-    else hipLaunchKernelGGL(k_cgemm_group<false>, dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    if (realOut)
+        hipLaunchKernelGGL((k_cgemm_group<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else if (rwMax == 2)
+        hipLaunchKernelGGL((k_cgemm_group<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else
+        hipLaunchKernelGGL((k_cgemm_group<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     return hipGetLastError();
 }
 
@@ -622,6 +768,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;     // K (h, j1): variant h of E1
     g.D = V; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
     g.M = (int)rows1; g.N = n[0]; g.K = K1; g.batches = 1; g.realOut = 0;
+    g.rw = rows1 >= 128 ? 2 : 1;              // 128-row tiles over the long row axis
     g.outScale = 1.0f; g.split16 = split;
     const float2* last = V;
     int64_t plane = n[0];                      // points already resolved
@@ -654,9 +801,13 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];
         const int64_t lim = (int64_t)1 << 30;
-        if ((int64_t)q.K * q.aSk + 64 * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn + q.bSnHi >= lim ||
-            64 * q.dSm + 64 * q.dSn + q.dSnHi >= lim)
-            for (int j = 0; j < st->n; j++) st->g[j].split16 = 0;
+        const int64_t tm = 64 * (q.rw == 2 ? 2 : 1);
+        if ((int64_t)q.K * q.aSk + tm * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn + q.bSnHi >= lim ||
+            tm * q.dSm + 64 * q.dSn + q.dSnHi >= lim)
+            for (int j = 0; j < st->n; j++) {
+                st->g[j].split16 = 0;
+                st->g[j].rw = 1;                  // the FP32 tile is 64 rows
+            }
     }
     return hipGetLastError();
 }

@@ -206,6 +206,7 @@ struct rf_gemm_desc {
     // once per block)
     int nLo;
     int64_t bSnHi, dSnHi;
+    int rw;               // 2: 128-row tiles (complex product, FP16 operands); 0 or 1: 64
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)
@@ -231,7 +232,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
 hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);
 int64_t rf_gemm_tiles(const rf_gemm_desc& g);
 hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s);
-hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles,
+hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles, int rwMax,
                                 hipStream_t s);
 
 // post-processing (rf_post.hip)

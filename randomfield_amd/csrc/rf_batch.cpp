@@ -24,7 +24,7 @@ struct rf_grid_batch {
     int nDesc[3] = {0, 0, 0};
     int64_t tiles[3] = {0, 0, 0};
     int realOut[3] = {0, 0, 1};
-    int rwMax[3] = {1, 1, 1};           // largest tile-row factor of a stage's GEMMs
+    int rwMax[3] = {1, 1, 1};           // wide tile kind used by a stage's GEMMs (1: none)
 };
 
 struct rf_graph {
@@ -126,7 +126,7 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         for (auto& g : stage[k]) {
             g.tile0 = t;
             t += rf_gemm_tiles(g);
-            if (g.rw == 2) B->rwMax[k] = 2;
+            if (g.rw > B->rwMax[k]) B->rwMax[k] = g.rw;
         }
         B->tiles[k] = t;
         B->nDesc[k] = (int)stage[k].size();

@@ -174,13 +174,17 @@ typedef double __attribute__((address_space(1))) gdo;
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
-// RW = 2: 128-row tiles of the complex product (two 16-row groups per wave
-// sharing each B fragment read)
+// Tile kinds (rf_gemm_desc::rw): 1 64x64; 2 128x64 complex product (two
+// 16-row groups per wave share each B fragment read); 3 64x128 real part
+// (two 32-column tiles per wave share each A fragment read; A holds only the
+// Re and -Im planes)
 template <int RW>
 struct HStage {
-    _Float16 a[6][kTM * RW][kHP];             // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
-    _Float16 b[4][kTN][kHP];                  // Re hi, Re lo, Im hi, Im lo
+    _Float16 a[RW == 3 ? 4 : 6][kTM * (RW == 2 ? 2 : 1)][kHP];   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
+    _Float16 b[4][kTN * (RW == 3 ? 2 : 1)][kHP];                  // Re hi, Re lo, Im hi, Im lo
 };
+__host__ __device__ inline int tile_m(const rf_gemm_desc& g) { return g.rw == 2 ? 2 * kTM : kTM; }
+__host__ __device__ inline int tile_n(const rf_gemm_desc& g) { return g.rw == 3 ? 2 * kTN : kTN; }
 
 // v -> FP16 hi (round to nearest) and lo = v - hi: packed conversions for
 // hi and lo, the residuals by mixed-precision FMAs reading the f16 halves in
@@ -493,6 +497,131 @@ __device__ __forceinline__ void gemm_tile_c128(const rf_gemm_desc& g, int m0, in
         }
 }
 
+// 64x128 tiles of the real part (the last contraction, whose column axis is
+// the long (x2, x1) plane): wave w owns rows 32 (w / 2).. and the two 32-column
+// tiles from 64 (w % 2), reading each A fragment once for both (6 fragment
+// reads per 6 MFMAs instead of 4 per 3); half as many blocks, each with twice
+// the MFMAs per 16-k chunk.  Staging, operand split and epilogue as gemm_tile_h.
+__device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, int n0, int batch, HStage<3>* st) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
+    const int64_t bCol0 = col_b(g, n0);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + bCol0);
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk;
+    f32x16 acc[2];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        acc[0][i] = 0.0f;
+        acc[1][i] = 0.0f;
+    }
+    const int rt = wave >> 1, ct = wave & 1;
+    // staging: A row t/4, k 4(t%4)..+3; B column t%128, k 8(t/128)..+7
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 127, bK = (tid >> 7) * 8;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
+    const int aOff = aRow * aSm, bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
+    const int kLast = g.K - 1;
+    f2v ra[4], rb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i;
+            ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kb = k0 + bK + i;
+            rb[i] = (bIn && kb <= kLast) ? B[kb * bSk + bOff] : f2v{0.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = ra[i].x;
+            im[i] = ra[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[1][aRow][aK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = neg4(hi);   // split(-x) = -split(x)
+        *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = neg4(lo);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i] = rb[4 * h + i].x;
+                im[i] = rb[4 * h + i].y;
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[0][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[1][bCol][bK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[2][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[3][bCol][bK + 4 * h]) = lo;
+        }
+    };
+    // K' half 0 pairs Re A with Re B, half 1 -Im A with Im B
+    const int aPlane = half ? 2 : 0, bPlane = half ? 2 : 0;
+    const int aRowL = rt * 32 + l32;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + rt * 32 < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+                const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const int col = 64 * ct + 32 * c;
+                    if (n0 + col >= g.N) break;                                    // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][col + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][col + l32][kk]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dCol0 = col_d(g, n0);
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + dCol0;
+    const int dSm = (int)g.dSm;
+    auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
+    const int oc[2] = {dcol(64 * ct + l32), dcol(64 * ct + 32 + l32)};
+    const bool nIn[2] = {n0 + 64 * ct + l32 < g.N, n0 + 64 * ct + 32 + l32 < g.N};
+    auto store = [&](auto map, auto* D) {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+                if (m0 + r < g.M && nIn[c]) D[r * dSm + oc[c]] = map(acc[c][i] * os);
+            }
+    };
+    const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
+    gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
+    gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
+    if (g.outDtype == RF_DTYPE_F64) {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
+        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+    } else {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
+        else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+    }
+}
+
 struct FStage {
     float2 a[kTK][kTM + 1];
     float2 b[kTK][kTN + 1];
@@ -506,6 +635,9 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
     if constexpr (RW == 2) {
         static_assert(!RealOnly, "128-row tiles are for the complex product");
         gemm_tile_c128(g, m0, n0, batch, reinterpret_cast<HStage<2>*>(smem));
+    } else if constexpr (RW == 3) {
+        static_assert(RealOnly, "128-column tiles are for the real part");
+        gemm_tile_r128(g, m0, n0, batch, reinterpret_cast<HStage<3>*>(smem));
     } else if (g.split16) {
         gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage<1>*>(smem));
     } else {
@@ -525,8 +657,8 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
 constexpr int kXcd = 8;
 
 __device__ inline void tile_coords(const rf_gemm_desc& g, int64_t local, int* mI, int* nI, int* batch) {
-    const int tm = kTM * (g.rw == 2 ? 2 : 1);
-    const int ntN = (g.N + kTN - 1) / kTN, ntM = (g.M + tm - 1) / tm;
+    const int tm = tile_m(g), tn = tile_n(g);
+    const int ntN = (g.N + tn - 1) / tn, ntM = (g.M + tm - 1) / tm;
     if (ntM <= ntN) {
         *mI = (int)(local % ntM);
         const int64_t r = local / ntM;
@@ -552,12 +684,12 @@ __global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
     if (L >= nTiles) return;
     int mI, nI, batch;
     tile_coords(g, L, &mI, &nI, &batch);
-    run_tile<RealOnly, RW>(g, mI * kTM * RW, nI * kTN, batch, smem);
+    run_tile<RealOnly, RW>(g, mI * tile_m(g), nI * tile_n(g), batch, smem);
 }
 
 // Grouped form: the GEMMs of several fields in one launch; GEMM q owns
 // logical tiles tile0 .. tile0 + tiles(q) - 1.
-// RWMAX = 2: some GEMMs of the group use 128-row tiles (rw = 2)
+// RWMAX > 1: the GEMMs of the group with rw = RWMAX use that tile kind
 template <bool RealOnly, int RWMAX>
 __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
     __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RWMAX>];
@@ -572,7 +704,7 @@ __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restr
     const rf_gemm_desc g = gs[lo];                 // uniform: scalar registers
     int mI, nI, batch;
     tile_coords(g, b - g.tile0, &mI, &nI, &batch);
-    if (!RealOnly && RWMAX == 2 && g.rw == 2) run_tile<RealOnly, RWMAX>(g, mI * kTM * 2, nI * kTN, batch, smem);
+    if (RWMAX > 1 && g.rw == RWMAX) run_tile<RealOnly, RWMAX>(g, mI * tile_m(g), nI * tile_n(g), batch, smem);
     else run_tile<RealOnly, 1>(g, mI * kTM, nI * kTN, batch, smem);
 }
 
@@ -657,8 +789,8 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 int64_t rf_gemm_tiles(const rf_gemm_desc& g) {
-    const int tm = kTM * (g.rw == 2 ? 2 : 1);
-    return (int64_t)((g.N + kTN - 1) / kTN) * ((g.M + tm - 1) / tm) * g.batches;
+    const int tm = tile_m(g), tn = tile_n(g);
+    return (int64_t)((g.N + tn - 1) / tn) * ((g.M + tm - 1) / tm) * g.batches;
 }
 
 // blocks of a launch over nTiles logical tiles (a multiple of the XCD count)
@@ -669,7 +801,8 @@ hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
     if (nTiles <= 0) return hipSuccess;
     const int64_t blocks = xcd_blocks(nTiles);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (g.realOut) hipLaunchKernelGGL((k_cgemm<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    if (g.realOut && g.rw == 3) hipLaunchKernelGGL((k_cgemm<true, 3>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else if (g.realOut) hipLaunchKernelGGL((k_cgemm<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else if (g.rw == 2) hipLaunchKernelGGL((k_cgemm<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else hipLaunchKernelGGL((k_cgemm<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
@@ -680,7 +813,9 @@ hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, i
     if (nG <= 0 || nTiles <= 0) return hipSuccess;
     const int64_t blocks = xcd_blocks(nTiles);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (realOut)
+    if (realOut && rwMax == 3)
+        hipLaunchKernelGGL((k_cgemm_group<true, 3>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else if (realOut)
         hipLaunchKernelGGL((k_cgemm_group<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     else if (rwMax == 2)
         hipLaunchKernelGGL((k_cgemm_group<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
@@ -797,16 +932,17 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     f.M = n[ad]; f.N = (int)plane; f.K = kNStep[ad]; f.batches = Nmc; f.realOut = 1;
     f.outDtype = outDtype; f.epi = epi;
     f.outScale = (float)(1.0 / (sc * std::ldexp(1.0, -m2))); f.split16 = split;
+    f.rw = plane >= 128 ? 3 : 1;              // 128-column tiles over the long plane axis
     // the split-precision tile addresses inside a tile with 32-bit offsets
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];
         const int64_t lim = (int64_t)1 << 30;
-        const int64_t tm = 64 * (q.rw == 2 ? 2 : 1);
-        if ((int64_t)q.K * q.aSk + tm * q.aSm >= lim || (int64_t)q.K * q.bSk + 64 * q.bSn + q.bSnHi >= lim ||
-            tm * q.dSm + 64 * q.dSn + q.dSnHi >= lim)
+        const int64_t tm = q.rw == 2 ? 128 : 64, tn = q.rw == 3 ? 128 : 64;
+        if ((int64_t)q.K * q.aSk + tm * q.aSm >= lim || (int64_t)q.K * q.bSk + tn * q.bSn + q.bSnHi >= lim ||
+            tm * q.dSm + tn * q.dSn + q.dSnHi >= lim)
             for (int j = 0; j < st->n; j++) {
                 st->g[j].split16 = 0;
-                st->g[j].rw = 1;                  // the FP32 tile is 64 rows
+                st->g[j].rw = 1;                  // the FP32 tile is 64 x 64
             }
     }
     return hipGetLastError();

@@ -177,14 +177,15 @@ constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + 
 // Tile kinds (rf_gemm_desc::rw): 1 64x64; 2 128x64 complex product (two
 // 16-row groups per wave share each B fragment read); 3 64x128 real part
 // (two 32-column tiles per wave share each A fragment read; A holds only the
-// Re and -Im planes)
+// Re and -Im planes); 4 64x128 complex product (four 32-column tiles per
+// wave share each A fragment read)
 template <int RW>
 struct HStage {
     _Float16 a[RW == 3 ? 4 : 6][kTM * (RW == 2 ? 2 : 1)][kHP];   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
-    _Float16 b[4][kTN * (RW == 3 ? 2 : 1)][kHP];                  // Re hi, Re lo, Im hi, Im lo
+    _Float16 b[4][kTN * (RW >= 3 ? 2 : 1)][kHP];                  // Re hi, Re lo, Im hi, Im lo
 };
 __host__ __device__ inline int tile_m(const rf_gemm_desc& g) { return g.rw == 2 ? 2 * kTM : kTM; }
-__host__ __device__ inline int tile_n(const rf_gemm_desc& g) { return g.rw == 3 ? 2 * kTN : kTN; }
+__host__ __device__ inline int tile_n(const rf_gemm_desc& g) { return g.rw >= 3 ? 2 * kTN : kTN; }
 
 // v -> FP16 hi (round to nearest) and lo = v - hi: packed conversions for
 // hi and lo, the residuals by mixed-precision FMAs reading the f16 halves in
@@ -622,6 +623,121 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     }
 }
 
+// 64x128 tiles of the complex product (the middle contraction, whose
+// columns (j3, x1) are long): wave w owns complex rows 16 w.. and four
+// 32-column tiles, reading each A fragment once for all four (10 fragment
+// reads per 12 MFMAs instead of 6 per 6).  Staging, operand split and
+// epilogue as gemm_tile_h.
+__device__ __forceinline__ void gemm_tile_cw(const rf_gemm_desc& g, int m0, int n0, int batch, HStage<4>* st) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
+    const int64_t bCol0 = col_b(g, n0);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + bCol0);
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk;
+    f32x16 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[c][i] = 0.0f;
+    // staging: A row t/4, k 4(t%4)..+3; B column t%128, k 8(t/128)..+7
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 127, bK = (tid >> 7) * 8;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
+    const int aOff = aRow * aSm, bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
+    const int kLast = g.K - 1;
+    f2v ra[4], rb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i;
+            ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kb = k0 + bK + i;
+            rb[i] = (bIn && kb <= kLast) ? B[kb * bSk + bOff] : f2v{0.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = ra[i].x;
+            im[i] = ra[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[1][aRow][aK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = lo;
+        *reinterpret_cast<hv4*>(&st->a[4][aRow][aK]) = neg4(hi);   // split(-x) = -split(x)
+        *reinterpret_cast<hv4*>(&st->a[5][aRow][aK]) = neg4(lo);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i] = rb[4 * h + i].x;
+                im[i] = rb[4 * h + i].y;
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[0][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[1][bCol][bK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[2][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[3][bCol][bK + 4 * h]) = lo;
+        }
+    };
+    // operand planes as in gemm_tile_h's complex product
+    const int isIm = l32 >> 4;
+    const int aPlane = isIm ? (half ? 0 : 2) : (half ? 4 : 0);
+    const int aRowL = wave * 16 + (l32 & 15);
+    const int bPlane = half ? 2 : 0;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + wave * 16 < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+                const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    if (n0 + 32 * c >= g.N) break;                                 // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][32 * c + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][32 * c + l32][kk]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dCol0 = col_d(g, n0);
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + dCol0;
+    const int dSm = (int)g.dSm;
+    auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
+    gf2o* D = (gf2o*)(static_cast<float2*>(g.D) + dBase);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int nn = 32 * c + l32;
+        if (n0 + nn >= g.N) continue;
+        const int oc = dcol(nn);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int r = wave * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
+            if (m0 + r < g.M) D[r * dSm + oc] = f2v{acc[c][i] * os, acc[c][i + 8] * os};
+        }
+    }
+}
+
 struct FStage {
     float2 a[kTK][kTM + 1];
     float2 b[kTK][kTN + 1];
@@ -636,8 +752,11 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
         static_assert(!RealOnly, "128-row tiles are for the complex product");
         gemm_tile_c128(g, m0, n0, batch, reinterpret_cast<HStage<2>*>(smem));
     } else if constexpr (RW == 3) {
-        static_assert(RealOnly, "128-column tiles are for the real part");
+        static_assert(RealOnly, "kind 3 is the real part");
         gemm_tile_r128(g, m0, n0, batch, reinterpret_cast<HStage<3>*>(smem));
+    } else if constexpr (RW == 4) {
+        static_assert(!RealOnly, "kind 4 is the complex product");
+        gemm_tile_cw(g, m0, n0, batch, reinterpret_cast<HStage<4>*>(smem));
     } else if (g.split16) {
         gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage<1>*>(smem));
     } else {
@@ -804,6 +923,7 @@ hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
     if (g.realOut && g.rw == 3) hipLaunchKernelGGL((k_cgemm<true, 3>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else if (g.realOut) hipLaunchKernelGGL((k_cgemm<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else if (g.rw == 2) hipLaunchKernelGGL((k_cgemm<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else if (g.rw == 4) hipLaunchKernelGGL((k_cgemm<false, 4>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else hipLaunchKernelGGL((k_cgemm<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
 }
@@ -819,6 +939,8 @@ hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, i
         hipLaunchKernelGGL((k_cgemm_group<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     else if (rwMax == 2)
         hipLaunchKernelGGL((k_cgemm_group<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else if (rwMax == 4)
+        hipLaunchKernelGGL((k_cgemm_group<false, 4>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     else
         hipLaunchKernelGGL((k_cgemm_group<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     return hipGetLastError();
@@ -918,6 +1040,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
         h.D = W; h.dSm = n[0]; h.dSn = 1; h.dStrideB = (int64_t)kNStep[2] * n[1] * n[0];
         h.nLo = n[0]; h.bSnHi = (int64_t)K2 * n[0]; h.dSnHi = (int64_t)n[1] * n[0];
         h.M = n[1]; h.N = kNStep[2] * n[0]; h.K = K2; h.batches = Nmc; h.realOut = 0;
+        h.rw = h.N >= 128 ? 4 : 1;            // 128-column tiles over the long (j3, x1) axis
         h.outScale = (float)std::ldexp(1.0, -m2); h.split16 = split;
         last = W;
         plane = (int64_t)n[0] * n[1];
@@ -937,7 +1060,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];
         const int64_t lim = (int64_t)1 << 30;
-        const int64_t tm = q.rw == 2 ? 128 : 64, tn = q.rw == 3 ? 128 : 64;
+        const int64_t tm = q.rw == 2 ? 128 : 64, tn = q.rw >= 3 ? 128 : 64;
         if ((int64_t)q.K * q.aSk + tm * q.aSm >= lim || (int64_t)q.K * q.bSk + tn * q.bSn + q.bSnHi >= lim ||
             tm * q.dSm + tn * q.dSn + q.dSnHi >= lim)
             for (int j = 0; j < st->n; j++) {

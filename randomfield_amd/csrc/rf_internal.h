@@ -206,7 +206,8 @@ struct rf_gemm_desc {
     // once per block)
     int nLo;
     int64_t bSnHi, dSnHi;
-    int rw;               // tile kind, FP16 operands only: 2 128x64 (complex), 3 64x128 (real); 0, 1: 64x64
+    int rw;               // tile kind, FP16 operands only: 2 128x64 (complex), 3 64x128 (real),
+                          // 4 64x128 (complex); 0, 1: 64x64
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)

@@ -745,11 +745,12 @@ struct FStage {
 template <int RW>
 constexpr size_t kStageBytes = sizeof(HStage<RW>) > sizeof(FStage) ? sizeof(HStage<RW>) : sizeof(FStage);
 
-// RW = 2 only with FP16 operands (the host sets rw = 1 for the FP32 path)
+// Tile kind RW (rf_gemm_desc::rw) > 1 only with FP16 operands: the host sets
+// rw = 1 wherever it falls back to the FP32-operand tile.
 template <bool RealOnly, int RW = 1>
 __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, int batch, char* smem) {
     if constexpr (RW == 2) {
-        static_assert(!RealOnly, "128-row tiles are for the complex product");
+        static_assert(!RealOnly, "kind 2 is the complex product");
         gemm_tile_c128(g, m0, n0, batch, reinterpret_cast<HStage<2>*>(smem));
     } else if constexpr (RW == 3) {
         static_assert(RealOnly, "kind 3 is the real part");

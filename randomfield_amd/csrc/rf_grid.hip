@@ -875,33 +875,50 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // so the first contraction's operand loads run along rows (whole cache lines
 // per wave) instead of 16 rows of 32 bytes each.
 // ph == NULL: real coefficients (method 4), no phase factor.
-__global__ void k_coeffs_fold(int64_t n, int nDim, int fold, int nk1, int nk2, int nk3, int64_t kNu,
-                              int64_t kNr, const double* __restrict__ amp, const double* __restrict__ ph,
-                              double ampMult, float2* __restrict__ C) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    const int H = fold ? 2 : 1, Q = (fold && nDim == 3) ? 2 : 1;
-    const int64_t rows = n / ((int64_t)H * nk1);
-    const int64_t kk = e / rows;
-    int64_t rem = e - kk * rows;
-    const int j1 = (int)(kk % nk1);
-    const int h = (int)(kk / nk1);
-    const int j2 = (int)(rem % nk2); rem /= nk2;
-    const int q = (int)(rem % Q); rem /= Q;
-    const int j3 = nDim == 3 ? (int)(rem % nk3) : 0;
-    const int64_t r = nDim == 3 ? rem / nk3 : rem;
-    int b = 0;
-    if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
-    const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
-    const int64_t src = r * kNr + b * kNu + j;
-    const double a = amp[src] * ampMult;
-    if (!ph) {
-        C[e] = make_float2((float)a, 0.0f);
-        return;
+// One block per 64 x 64 tile of (row m, k): the coefficients are formed in
+// the source order (k fastest: the amplitude and phase reads are contiguous
+// along j1) into LDS and written out K-major (m fastest).
+constexpr int kFoldT = 64;
+__global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nDim, int fold, int nk1, int nk2,
+                                                     int nk3, int64_t kNu, int64_t kNr,
+                                                     const double* __restrict__ amp, const double* __restrict__ ph,
+                                                     double ampMult, float2* __restrict__ C) {
+    __shared__ float2 tile[kFoldT][kFoldT + 1];
+    const int64_t m0 = (int64_t)blockIdx.x * kFoldT;
+    const int k0 = blockIdx.y * kFoldT;
+    const int Q = (fold && nDim == 3) ? 2 : 1;
+    for (int e = threadIdx.x; e < kFoldT * kFoldT; e += 256) {
+        const int kk = e % kFoldT, mm = e / kFoldT;
+        const int64_t m = m0 + mm;
+        const int k = k0 + kk;
+        if (m >= rows || k >= K) continue;
+        const int j1 = k % nk1;
+        const int h = k / nk1;
+        int64_t rem = m;
+        const int j2 = (int)(rem % nk2); rem /= nk2;
+        const int q = (int)(rem % Q); rem /= Q;
+        const int j3 = nDim == 3 ? (int)(rem % nk3) : 0;
+        const int64_t r = nDim == 3 ? rem / nk3 : rem;
+        int b = 0;
+        if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
+        const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
+        const int64_t src = r * kNr + b * kNu + j;
+        const double a = amp[src] * ampMult;
+        if (!ph) {
+            tile[kk][mm] = make_float2((float)a, 0.0f);
+        } else {
+            double sn, cs;
+            sincos(ph[src], &sn, &cs);
+            tile[kk][mm] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
+        }
     }
-    double sn, cs;
-    sincos(ph[src], &sn, &cs);
-    C[e] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
+    __syncthreads();
+    for (int e = threadIdx.x; e < kFoldT * kFoldT; e += 256) {
+        const int mm = e % kFoldT, kk = e / kFoldT;
+        const int64_t m = m0 + mm;
+        const int k = k0 + kk;
+        if (m < rows && k < K) C[(int64_t)k * rows + m] = tile[kk][mm];
+    }
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -1014,9 +1031,11 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
         coef->key = ampMult * sc;
     }
     if (!fresh)
-        hipLaunchKernelGGL(k_coeffs_fold, dim3(blocks_for(nC)), dim3(256), 0, s, nC, nDim, fold, kNStep[0],
-                           nDim > 1 ? kNStep[1] : 1, nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp,
-                           kind == 1 ? nullptr : d_phase, ampMult * sc, Cf);
+        hipLaunchKernelGGL(k_coeffs_fold,
+                           dim3((unsigned)((nC / K1 + kFoldT - 1) / kFoldT), (unsigned)((K1 + kFoldT - 1) / kFoldT)),
+                           dim3(256), 0, s, nC / K1, K1, nDim, fold, kNStep[0], nDim > 1 ? kNStep[1] : 1,
+                           nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp, kind == 1 ? nullptr : d_phase,
+                           ampMult * sc, Cf);
     *st = rf_grid_stages{};
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;

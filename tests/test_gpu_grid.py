@@ -45,6 +45,11 @@ def grid_case(ctx, nstep, step, origin, corrL, Nmc, seed, dtype=np.float32, marg
     ((20, 18, 16), (0.25, 0.25, 0.25), (1.0, 1.2, 0.8)),
     ((33, 7, 40), (0.3, 0.5, 0.2), (1.0, 1.0, 1.0)),
     ((41, 29), (0.2, 0.3), (0.7, 1.3)),
+    # x1 = 129: the middle contraction's 128-column tiles straddle (j3, x1)
+    # block boundaries, the last contraction's end in a 5-column tile
+    ((129, 5, 3), (0.2, 0.4, 0.5), (1.0, 1.0, 1.0)),
+    # x2 = 140: partial 64-row tiles of the middle contraction
+    ((7, 140, 9), (0.5, 0.1, 0.4), (1.0, 1.0, 1.0)),
 ])
 def test_grid_form_matches_oracle(ctx, nstep, step, corrL):
     d = len(nstep)

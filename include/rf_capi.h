@@ -368,7 +368,7 @@ typedef struct rf_server_config {
     int32_t form;          /* method-2 form: RF_FORM_MFMA_F16 (0 = default) or _BF16 */
     int32_t maxNmc;        /* largest Nmc accepted; 0 = 1024 */
     double  xRangeNorm[3]; /* lattice extent in correlation lengths; 0 = 10 */
-    double  batchTerms;    /* batch closes above this many N*Nmc*N_k terms; 0 = 5e11 */
+    double  batchTerms;    /* batch closes above this many N*Nmc*N_k terms; 0 = 2e11 */
     int32_t reserved[8];
 } rf_server_config;
 

@@ -441,7 +441,7 @@ int rf_server_create(rf_context* ctx, const rf_server_config* cfg, rf_server** o
     if (c.maxBatch <= 0) c.maxBatch = 64;
     if (c.form == 0) c.form = RF_FORM_MFMA_F16;
     if (c.maxNmc <= 0) c.maxNmc = 1024;
-    if (!(c.batchTerms > 0.0)) c.batchTerms = 5e11;
+    if (!(c.batchTerms > 0.0)) c.batchTerms = 2e11;
     for (int d = 0; d < 3; d++)
         if (!(c.xRangeNorm[d] > 0.0)) c.xRangeNorm[d] = 10.0;
     if (c.maxInflight > 8 || (c.form != RF_FORM_MFMA_F16 && c.form != RF_FORM_MFMA_BF16)) {

@@ -19,7 +19,7 @@ from ._capi import (FOM_NONE, FORM_MFMA_F16, Request, RequestResult, ServerConfi
 class Server:
     def __init__(self, ctx, hbm_budget: int = 0, max_inflight: int = 3, max_batch: int = 64,
                  form: int = FORM_MFMA_F16, max_nmc: int = 1024, xrange_norm=(10.0, 10.0, 10.0),
-                 batch_terms: float = 5e11):
+                 batch_terms: float = 2e11):
         self.ctx = ctx
         cfg = ServerConfig()
         cfg.hbmBudget, cfg.maxInflight, cfg.maxBatch, cfg.form, cfg.maxNmc = (

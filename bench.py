@@ -229,6 +229,13 @@ def run_cfg4(args):
     _, ms_b = timed(lambda: bplan.execute(g, sptr), args.steps, 1)
     nglob = float(np.prod(groupN))
     nsub = float(sum(np.prod(s) for s in sizes))
+    # algorithmic FLOPs of the three contractions per step (8 per complex
+    # MAC, 4 per real-part MAC; sign-block fold: K = 2 nk1, 2 nk2, nk3)
+    flop_gen = 0.0
+    for pl, (o_, n_, e_) in zip(plans, geo):
+        k1, k2, k3 = [int(v) for v in pl.info.kNStep]
+        flop_gen += Nmc * (8.0 * (2 * k1) * (2 * k2 * k3) * n_[0] + 8.0 * (2 * k2) * k3 * n_[0] * n_[1]
+                           + 4.0 * k3 * n_[0] * n_[1] * n_[2])
     work = nglob * Nmc
     bytes_blend = (nsub + nglob) * Nmc * 4.0
     gbs = bytes_blend / (ms_b.mean() * 1e-3) / 1e9
@@ -260,6 +267,12 @@ def run_cfg4(args):
                  f"N_k {info.kNTotal} (subdomain 0), grid form in grouped launches + variance-preserving blend, "
                  "one captured graph per step",
                  step_kernel_ms=float(ms_s.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
+                 generation_roofline={"bound": "mfma", "unit": "TFLOP/s", "peak": 2500.0,
+                                      "achieved": flop_gen / (ms_g.mean() * 1e-3) / 1e12,
+                                      "frac": flop_gen / (ms_g.mean() * 1e-3) / 1e12 / 2500.0,
+                                      "executed_frac": 3 * flop_gen / (ms_g.mean() * 1e-3) / 1e12 / 2500.0,
+                                      "flop_per_step": flop_gen, "kernel_ms": float(ms_g.mean()),
+                                      "note": "FP16 hi/lo split: 3 MFMA products per algorithmic one"},
                  blend_roofline={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
                                  "frac": gbs / 8000.0, "kernel_ms": float(ms_b.mean()),
                                  "bytes_per_launch": bytes_blend},

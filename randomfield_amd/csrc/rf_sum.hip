@@ -106,10 +106,18 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double
 // FP64 k.x.  kT[j][r][D] in revolutions, aT/pT[j][r] as in k_sum_fp32.
 // Guard: skip the padded realizations of the chunk (worth it when requests
 // have few realizations; the plan kernel keeps the unguarded schedule).
+// Global-address-space views of the direction / phase / amplitude streams:
+// their reads are uniform across a wave, and only through such pointers (as
+// kernel arguments are, but not pointers read from a request record) does
+// the compiler issue them as batched scalar loads rather than one flat
+// vector load and wait per realization.
+typedef const double __attribute__((address_space(1))) gdbl;
+typedef const float __attribute__((address_space(1))) gflt;
+
 template <int D, int R, bool Guard>
-__device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, const double* __restrict__ kT,
-                                                   const float* __restrict__ pT,
-                                                   const float* __restrict__ aT, int Rpad, int64_t i,
+__device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, gdbl* __restrict__ kT,
+                                                   gflt* __restrict__ pT,
+                                                   gflt* __restrict__ aT, int Rpad, int64_t i,
                                                    int r0) {
     double xn[D];
 #pragma unroll
@@ -120,9 +128,9 @@ __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, const d
     const int nr = a.Nmc - r0;               // realizations of this chunk (wave-uniform)
     (void)nr;
     for (int64_t j = 0; j < a.kN; j++) {
-        const double* kj = kT + (j * Rpad + r0) * D;
-        const float* pj = pT + j * Rpad + r0;
-        const float* aj = aT + j * Rpad + r0;
+        gdbl* kj = kT + (j * Rpad + r0) * D;
+        gflt* pj = pT + j * Rpad + r0;
+        gflt* aj = aT + j * Rpad + r0;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (Guard && r >= nr) break;         // padded realizations stay zero
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const do
                                                          const float* __restrict__ aT, int Rpad) {
     const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.i1) return;
-    sum_fp32_iso_point<D, R, false>(a, kT, pT, aT, Rpad, i, blockIdx.y * R);
+    sum_fp32_iso_point<D, R, false>(a, (gdbl*)kT, (gflt*)pT, (gflt*)aT, Rpad, i, blockIdx.y * R);
 }
 
 // Grouped form: block b runs one tile = (field, block of kGroupIsoPts points,
@@ -165,8 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_group_iso(const rf_group_req* __rest
     const int64_t local = b - q.tile0;
     const int64_t i = (local / nChunk) * kGroupIsoPts + threadIdx.x;
     if (i >= q.a.N) return;
-    sum_fp32_iso_point<3, kGroupIsoR, true>(q.a, static_cast<const double*>(q.ops0), static_cast<const float*>(q.ops1),
-                                      static_cast<const float*>(q.ops2), q.Rpad, i,
+    sum_fp32_iso_point<3, kGroupIsoR, true>(q.a, (gdbl*)q.ops0, (gflt*)q.ops1, (gflt*)q.ops2, q.Rpad, i,
                                       (int)(local % nChunk) * kGroupIsoR);
 }
 

@@ -104,8 +104,9 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double
 // FP32 form of the isotropic sum with per-realization directions: the
 // argument cannot be shared across realizations, so each term forms its own
 // FP64 k.x.  kT[j][r][D] in revolutions, aT/pT[j][r] as in k_sum_fp32.
-// Guard: skip the padded realizations of the chunk (worth it when requests
-// have few realizations; the plan kernel keeps the unguarded schedule).
+// Padded realizations of a chunk (zero amplitude) are computed and not stored:
+// a per-realization exit would serialize the chunk's scalar loads and
+// transcendentals (measured slower on the cfg 5 stream).
 // Global-address-space views of the direction / phase / amplitude streams:
 // their reads are uniform across a wave, and only through such pointers (as
 // kernel arguments are, but not pointers read from a request record) does
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32(rf_sum_args a, const double
 typedef const double __attribute__((address_space(1))) gdbl;
 typedef const float __attribute__((address_space(1))) gflt;
 
-template <int D, int R, bool Guard>
+template <int D, int R>
 __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, gdbl* __restrict__ kT,
                                                    gflt* __restrict__ pT,
                                                    gflt* __restrict__ aT, int Rpad, int64_t i,
@@ -125,15 +126,12 @@ __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, gdbl* _
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0f;
-    const int nr = a.Nmc - r0;               // realizations of this chunk (wave-uniform)
-    (void)nr;
     for (int64_t j = 0; j < a.kN; j++) {
         gdbl* kj = kT + (j * Rpad + r0) * D;
         gflt* pj = pT + j * Rpad + r0;
         gflt* aj = aT + j * Rpad + r0;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            if (Guard && r >= nr) break;         // padded realizations stay zero
             double t = kj[r * D] * xn[0];
 #pragma unroll
             for (int d = 1; d < D; d++) t = fma(kj[r * D + d], xn[d], t);
@@ -154,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_fp32_iso(rf_sum_args a, const do
                                                          const float* __restrict__ aT, int Rpad) {
     const int64_t i = a.i0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.i1) return;
-    sum_fp32_iso_point<D, R, false>(a, (gdbl*)kT, (gflt*)pT, (gflt*)aT, Rpad, i, blockIdx.y * R);
+    sum_fp32_iso_point<D, R>(a, (gdbl*)kT, (gflt*)pT, (gflt*)aT, Rpad, i, blockIdx.y * R);
 }
 
 // Grouped form: block b runs one tile = (field, block of kGroupIsoPts points,
@@ -173,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_group_iso(const rf_group_req* __rest
     const int64_t local = b - q.tile0;
     const int64_t i = (local / nChunk) * kGroupIsoPts + threadIdx.x;
     if (i >= q.a.N) return;
-    sum_fp32_iso_point<3, kGroupIsoR, true>(q.a, (gdbl*)q.ops0, (gflt*)q.ops1, (gflt*)q.ops2, q.Rpad, i,
+    sum_fp32_iso_point<3, kGroupIsoR>(q.a, (gdbl*)q.ops0, (gflt*)q.ops1, (gflt*)q.ops2, q.Rpad, i,
                                       (int)(local % nChunk) * kGroupIsoR);
 }
 

@@ -23,15 +23,15 @@ from ._capi import (  # noqa: F401
     CM_COS, CM_GAUSSIAN, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
     FFT, FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
     MFMA_VARIANTS, RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
-    RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, Params, RFError, SpectrumInfo, check, flib, lib,
-    make_params,
+    RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, LayoutInfo, LayoutParams, Params, RFError, SpectrumInfo, check,
+    flib, lib, make_params,
 )
 
 __all__ = [
     "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate",
     "create_random_field", "multivariate_transformation", "reference_seed", "reference_draws",
     "blend", "BlendPlan", "GridBatch", "Graph", "stats", "corr", "interpolate", "interpolate_box", "read_unv",
-    "normalize",
+    "normalize", "localization_layout",
     "device_uniforms", "RFError", "MFMA_VARIANTS",
 ]
 
@@ -52,6 +52,25 @@ def _ptr(a):
     if hasattr(a, "data_ptr"):
         return C.c_void_p(a.data_ptr())
     return C.c_void_p(int(a))
+
+
+def _check_device(t, name: str, dtypes=("float32", "float64"), shape=None, device=None) -> None:
+    """Refuses a buffer whose address must not reach a kernel: host arrays,
+    CPU tensors, non-contiguous views, other dtypes or shapes, or a tensor on
+    another device than the context's."""
+    if isinstance(t, np.ndarray) or not hasattr(t, "data_ptr") or not hasattr(t, "is_cuda"):
+        raise ValueError(f"{name} must be a device tensor, not {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} is not in device memory")
+    dt = str(t.dtype).replace("torch.", "")
+    if dt not in dtypes:
+        raise ValueError(f"{name} has dtype {dt}; expected one of {', '.join(dtypes)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} is not contiguous")
+    if shape is not None and tuple(int(v) for v in t.shape) != tuple(int(v) for v in shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}; expected {tuple(shape)}")
+    if device is not None and t.device.index != device:
+        raise ValueError(f"{name} is on device {t.device.index}, the context on device {device}")
 
 
 def _dtype_code(dt) -> int:
@@ -105,6 +124,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
+        self.device = int(device)
         check(lib().rf_context_create(C.byref(self._h), device))
 
     @property
@@ -151,9 +171,18 @@ class FieldPlan:
         check(lib().rf_plan_info(self._h, C.byref(info)))
         return info
 
+    def _check_points(self, x, out) -> int:
+        dev = getattr(self.ctx, "device", None)
+        _check_device(x, "x", ("float64",), device=dev)
+        if x.dim() != 2 or int(x.shape[1]) != self.params.nDim:
+            raise ValueError(f"x has shape {tuple(x.shape)}; expected (N, {self.params.nDim})")
+        N = int(x.shape[0])
+        _check_device(out, "out", shape=(self.params.Nmc, N), device=dev)
+        return N
+
     def execute(self, x, out, stream=None) -> None:
         """x: device (N, nDim) float64; out: device (Nmc, N) float32/float64."""
-        N = int(x.shape[0])
+        N = self._check_points(x, out)
         check(lib().rf_plan_execute(self._h, _ptr(x), N, _ptr(out), _dtype_code(out.dtype),
                                     _ptr(stream)), self.ctx.handle)
 
@@ -164,8 +193,9 @@ class FieldPlan:
 
     def autotune(self, x, out, stream=None) -> int:
         """Times every MFMA launch shape on these points, keeps the fastest."""
+        N = self._check_points(x, out)
         chosen = C.c_int32(0)
-        check(lib().rf_plan_autotune(self._h, _ptr(x), int(x.shape[0]), _ptr(out), _dtype_code(out.dtype),
+        check(lib().rf_plan_autotune(self._h, _ptr(x), N, _ptr(out), _dtype_code(out.dtype),
                                      _ptr(stream), C.byref(chosen)), self.ctx.handle)
         return int(chosen.value)
 
@@ -174,6 +204,8 @@ class FieldPlan:
         o = np.zeros(3); o[: len(origin)] = origin
         st = np.ones(3); st[: len(step)] = step
         ns = np.ones(3, dtype=np.int32); ns[: len(nstep)] = nstep
+        _check_device(out, "out", shape=(self.params.Nmc, int(np.prod(ns, dtype=np.int64))),
+                      device=getattr(self.ctx, "device", None))
         check(lib().rf_plan_execute_grid(self._h, _ptr(o), _ptr(st), _ptr(ns), _ptr(out),
                                          _dtype_code(out.dtype), _ptr(stream)), self.ctx.handle)
 
@@ -416,3 +448,33 @@ def normalize(ctx: Context, d_field, N: int, Nmc: int, dtype_code: int, stream=N
 
 def device_uniforms(ctx: Context, seed: int, tag: int, count: int, d_out, stream=None) -> None:
     check(lib().rf_device_uniforms(ctx.handle, seed, tag, count, _ptr(d_out), _ptr(stream)), ctx.handle)
+
+
+def localization_layout(nFields, pointsPerCorrL, xMinGlob, xMaxGlob, corrL, overlap):
+    """rf_localization_layout: the reference's subdomain geometry (one
+    localization level).  Returns (LayoutInfo, origins (nSub, nDim), sizes
+    (nSub, nDim), faceMasks (nSub,), xMinSub (nSub, nDim)); subdomain s is the
+    reference's field s + 1 (last axis fastest)."""
+    d = len(nFields)
+    p = LayoutParams()
+    p.nDim = d
+
+    def fill(arr, vals, default):
+        v = list(vals) + [default] * (3 - len(vals))
+        for i in range(3):
+            arr[i] = v[i]
+    fill(p.nFields, nFields, 1)
+    fill(p.pointsPerCorrL, pointsPerCorrL, 1)
+    fill(p.xMinGlob, xMinGlob, 0.0)
+    fill(p.xMaxGlob, xMaxGlob, 0.0)
+    fill(p.corrL, corrL, 1.0)
+    fill(p.overlap, overlap, 0.0)
+    info = LayoutInfo()
+    check(lib().rf_localization_layout(C.byref(p), C.byref(info), None, None, None, None))
+    n = info.nSub
+    org = np.zeros((n, d), dtype=np.int32)
+    siz = np.zeros((n, d), dtype=np.int32)
+    msk = np.zeros(n, dtype=np.int32)
+    xms = np.zeros((n, d))
+    check(lib().rf_localization_layout(C.byref(p), C.byref(info), _ptr(org), _ptr(siz), _ptr(msk), _ptr(xms)))
+    return info, org, siz, msk, xms

@@ -1076,6 +1076,10 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     f.outDtype = outDtype; f.epi = epi;
     f.outScale = (float)(1.0 / (sc * std::ldexp(1.0, -m2))); f.split16 = split;
     f.rw = plane >= 128 ? 3 : 1;              // 128-column tiles over the long plane axis
+    // the wide tiles (kinds 2..4) exist with FP16 operands only: a K beyond
+    // the FP16 range of the first contraction runs every stage on the FP32 tile
+    if (!split)
+        for (int j = 0; j < st->n; j++) st->g[j].rw = 1;
     // the split-precision tile addresses inside a tile with 32-bit offsets
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];

@@ -12,6 +12,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _check_device
 from ._capi import (FOM_NONE, FORM_MFMA_F16, Request, RequestResult, ServerConfig, ServerStats, check,
                     lib)
 
@@ -40,7 +41,10 @@ class Server:
             x = np.ascontiguousarray(x, dtype=np.float64)
             r.xOnDevice, r.x = 0, x.ctypes.data
         else:
+            _check_device(x, "x", ("float64",), device=getattr(self.ctx, "device", None))
             r.xOnDevice, r.x = 1, x.data_ptr()
+        if x.ndim != 2 or int(x.shape[1]) != 3:
+            raise ValueError(f"x has shape {tuple(x.shape)}; expected (N, 3)")
         r.N = int(x.shape[0])
         r.corrL = (C.c_double * 3)(*[float(v) for v in corrL])
         r.fieldAvg, r.fieldVar, r.seed = float(avg), float(var), int(seed)

@@ -227,6 +227,20 @@ int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* groupN,
                     const void* const* d_fields, void* d_group,
                     int32_t dtype, void* stream);
 
+/* generateUnityPartition_Matrix (localization_RF.f90:705-857) on the host:
+ * pu (prod(xNStep), x fastest) = product of the face factors of the faces
+ * faceMask selects (bit 2a lower, 2a+1 upper face of axis a), nOvl[a] points
+ * per face (< 1: axis untapered).  REF_COMPAT: the reference's cosine taper
+ * (partitionType 1, :800-813); FIXED: its square-root exact form. */
+int rf_unity_partition(int32_t nDim, const int32_t* xNStep, const int32_t* nOvl, int32_t faceMask,
+                       int32_t mode, double* pu);
+/* rf_blend_device on host buffers (the Fortran facade's localization step,
+ * calls_RF.f90:209-369 with every field added at once): fields[s] FP64,
+ * group FP64 prod(groupN) x Nmc, overwritten.  Synchronous. */
+int rf_blend_host(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc, int32_t nSub,
+                  const int32_t* origins, const int32_t* sizes, const int32_t* nOvl,
+                  const int32_t* faceMasks, int32_t mode, const double* const* fields, double* group);
+
 /* The blend split into set-up and launch (same arguments and result as
  * rf_blend_device): create analyses the boxes and uploads the tables once
  * (synchronous); execute enqueues one kernel and allocates nothing, so it
@@ -241,6 +255,50 @@ int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t* groupN,
                          rf_blend_plan** plan);
 int rf_blend_plan_execute(rf_blend_plan* plan, void* d_group, void* stream);
 int rf_blend_plan_destroy(rf_blend_plan* plan);
+
+/* Localization geometry (one localization level; the reference's input
+ * file gives xMinGlob/xMaxGlob, corrL, pointsPerCorrL, overlap and nFields):
+ *   round_basic_inputs     topography_RF.f90:19-58
+ *   set_global_extremes    topography_RF.f90:63-144
+ *   prepare_Localization   localization_RF.f90:127-192
+ *   set_local_bounding_box topography_RF.f90:150-255 (one rank per subdomain)
+ *   overlap points         localization_RF.f90:746
+ * Host only, no GPU.  Outputs per subdomain s = 0 .. info.nSub-1 (the
+ * reference's field number s+1, last axis fastest; its seed is
+ * seedStart + s + 1, sample_RF.f90:309-312), each nullable:
+ *   origins[s*nDim+a]  first point of the subdomain in the group grid (0-based)
+ *   sizes[s*nDim+a]    points per axis (all equal in the reference)
+ *   faceMasks[s]       bit 2a / 2a+1: the lower / upper face along axis a has a
+ *                      neighbour (the faces rf_blend_device tapers in FIXED
+ *                      mode; REF_COMPAT tapers every face, as the reference)
+ *   xMinSub[s*nDim+a]  coordinate of the subdomain's first point
+ * origins/sizes/faceMasks/info.nOvl are the arguments rf_blend_device takes. */
+typedef struct rf_layout_params {
+    int32_t nDim;
+    int32_t nFields[3];         /* subdomains per axis (IPT%nFields)            */
+    int32_t pointsPerCorrL[3];
+    int32_t pad;
+    double  xMinGlob[3], xMaxGlob[3];
+    double  corrL[3];
+    double  overlap[3];         /* in correlation lengths (IPT%overlap)         */
+} rf_layout_params;
+
+typedef struct rf_layout_info {
+    int32_t nDim, nSub;
+    int32_t groupN[3];          /* group grid points per axis (xNStep_Group)    */
+    int32_t subN[3];            /* subdomain points per axis (xNStep)           */
+    int32_t nOvl[3];            /* overlap points, >= 0 (blend argument)        */
+    int32_t nOvlRaw[3];         /* the reference's formula (< 1: no taper)      */
+    double  xStep[3];
+    double  overlap[3];         /* rounded overlap, correlation lengths         */
+    double  xMinGlob[3], xMaxGlob[3];  /* rounded and grown global box          */
+    double  procExtent[3];      /* subdomain extent                             */
+    double  stepProc[3];        /* stride between subdomain starts             */
+    double  xRangeSub[3];       /* RDF%xRange of a subdomain (its lattice box)  */
+} rf_layout_info;
+
+int rf_localization_layout(const rf_layout_params* in, rf_layout_info* info, int32_t* origins,
+                           int32_t* sizes, int32_t* faceMasks, double* xMinSub);
 
 /* Batched structured-grid generation: the subdomains of a localization
  * group (one MPI rank each in the reference, localization_RF.f90:127-192,
@@ -323,6 +381,18 @@ int rf_unv_destroy(rf_unv_mesh* m);
 /* Subtract each realization's mean and divide by its RMS (FP64 sums). */
 int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N,
                         int32_t Nmc, int32_t dtype, void* stream);
+
+/* normalize_randField (randomFieldND.f90:820-926) as the reference computes
+ * it: the mean and then the root mean square of realization 1 over the box
+ * minPos..maxPos (1-based, inclusive; the points that are not repeated
+ * between ranks, sample_RF.f90:829-842) of its nDim grid gridN (x fastest),
+ * subtracted from / divided into realizations 1..Nmc of d_field (N x Nmc,
+ * N = prod(gridN)).  Synchronous.  _host: on a host FP64 buffer. */
+int rf_normalize_window_device(rf_context* ctx, void* d_field, int32_t nDim, const int32_t* gridN,
+                               const int32_t* minPos, const int32_t* maxPos, int32_t Nmc, int32_t dtype,
+                               void* stream);
+int rf_normalize_window_host(rf_context* ctx, double* field, int32_t nDim, const int32_t* gridN,
+                             const int32_t* minPos, const int32_t* maxPos, int32_t Nmc);
 
 /* Device generator: count uniforms U[0,1) of stream (seed, tag) into d_out
  * (FP64), element e = Philox4x32-10(counter=(e lo, e hi, tag, 0),

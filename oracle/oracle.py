@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         L.rfo_isotropic.argtypes = [C.POINTER(Plan), vp, C.c_int64, vp, vp, C.c_int, vp, vp, vp, C.c_int]
         L.rfo_marginal.argtypes = [C.c_int, C.c_double, C.c_double, vp, C.c_int64]
         L.rfo_overlap_npoints.argtypes = [C.c_int, vp, vp, vp, vp]
+        L.rfo_localization_layout.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.rfo_unity_partition.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, vp]
         L.rfo_blend.argtypes = [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp]
         L.rfo_normalize.argtypes = [vp, C.c_int64, C.c_int]
@@ -160,6 +161,30 @@ def marginal(margi: int, avg: float, var: float, field: np.ndarray) -> np.ndarra
     if rc != 0:
         raise ValueError("lognormal with fieldAvg <= 0")
     return f
+
+
+def localization_layout(nFields, pointsPerCorrL, xMinGlob, xMaxGlob, corrL, overlap) -> dict:
+    """The reference's localization geometry (one level): per-axis values and
+    per-subdomain origins (0-based group index) and first coordinates."""
+    d = len(nFields)
+    nsub = int(np.prod(nFields))
+    g = np.zeros(13 * d)
+    org = np.zeros(nsub * d, dtype=np.int32)
+    xms = np.zeros(nsub * d)
+    i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+    f64 = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+    n = lib().rfo_localization_layout(d, _p(i32(nFields)), _p(i32(pointsPerCorrL)), _p(f64(xMinGlob)),
+                                      _p(f64(xMaxGlob)), _p(f64(corrL)), _p(f64(overlap)), _p(g), _p(org),
+                                      _p(xms))
+    assert n == nsub
+    g = g.reshape(d, 13)
+    keys = ["xStep", "overlap", "xMinGlob", "xMaxGlob", "procExtent", "stepProc", "groupN", "subN", "nOvl"]
+    out = {k: g[:, i].copy() for i, k in enumerate(keys)}
+    for k in ("groupN", "subN", "nOvl"):
+        out[k] = out[k].astype(np.int64)
+    out["origins"] = org.reshape(nsub, d)
+    out["xMinSub"] = xms.reshape(nsub, d)
+    return out
 
 
 def overlap_npoints(overlap, corrL, xStep) -> np.ndarray:

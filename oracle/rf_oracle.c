@@ -437,6 +437,94 @@ int rfo_marginal(int margiFirst, double fieldAvg, double fieldVar,
 
 /* --------------------------------------------------- partition of unity */
 
+/* --------------------------------------------------- localization geometry */
+
+/* areEqual (math_RF.f90:397-413), TOLERANCE = 1e-9 (constants_RF.f90:8) */
+static int are_equal(double a, double b) { return fabs(a - b) < 0.000000001; }
+
+/* roundToMultiple (math_RF.f90:419-450) */
+static double round_to_multiple(double nmb, double divisor, int up)
+{
+    if (divisor != 0.0) {
+        nmb = nmb / fabs(divisor);
+        if (are_equal(nmb, (double)lround(nmb))) nmb = (double)lround(nmb);
+        else nmb = up ? ceil(nmb) : floor(nmb);
+        nmb = nmb * fabs(divisor);
+    }
+    return nmb;
+}
+
+int rfo_localization_layout(int nDim, const int32_t* nFields, const int32_t* pointsPerCorrL,
+                            const double* xMinGlobIn, const double* xMaxGlobIn, const double* corrL,
+                            const double* overlapIn, double* g, int32_t* origins, double* xMinSub)
+{
+    double xStep[3], overlap[3], xMin[3], xMax[3], procExtent[3], stepProc[3];
+    int32_t groupN[3], subN[3];
+    for (int i = 0; i < nDim; i++) {
+        const double P = (double)nFields[i];
+        /* round_basic_inputs, topography_RF.f90:34-53 */
+        xStep[i] = corrL[i] / (double)pointsPerCorrL[i];
+        overlap[i] = overlapIn[i];
+        if (nFields[i] == 1) overlap[i] = 0.0;
+        overlap[i] = (double)lround(overlap[i] * corrL[i] / xStep[i]) * xStep[i] / corrL[i];
+        if (nFields[i] != 1 && overlap[i] < 2.0 * xStep[i] / corrL[i]) overlap[i] = 2.0 * xStep[i] / corrL[i];
+        /* set_global_extremes, topography_RF.f90:85-140 */
+        xMin[i] = round_to_multiple(xMinGlobIn[i], xStep[i], 0);
+        xMax[i] = round_to_multiple(xMaxGlobIn[i], xStep[i], 1);
+        double ovlp = overlap[i] * corrL[i];
+        double ovlpTot = ovlp * (P - 1.0) + 2.0 * (ovlp - xStep[i]);
+        if (nFields[i] == 1) { ovlp = 0.0; ovlpTot = 0.0; }
+        if (ovlpTot <= 0.0) ovlpTot = 0.0;
+        double novlpTot = (xMax[i] - xMin[i]) - ovlpTot;
+        if (novlpTot < 0.0) novlpTot = 0.0;
+        novlpTot = round_to_multiple(novlpTot, xStep[i] * P, 1);
+        const double novlp = novlpTot / P;
+        xMax[i] = xMin[i] + (ovlpTot + novlpTot);
+        double deltaProc = novlpTot / P + 2.0 * ovlp - 2.0 * xStep[i];
+        if (nFields[i] == 1) deltaProc = novlpTot / P;
+        stepProc[i] = novlp + ovlp;
+        procExtent[i] = deltaProc;
+        /* set_local_bounding_box, one rank: [start, start + procExtent];
+         * find_xNStep = 1 + nint(range/xStep) (math_RF.f90:156-192) */
+        subN[i] = 1 + (int32_t)lround(procExtent[i] / xStep[i]);
+    }
+    int nSub = 1;
+    for (int i = 0; i < nDim; i++) nSub *= nFields[i];
+    /* prepare_Localization, localization_RF.f90:150-176: starts on the
+     * stride grid, setGrid inverse (last axis fastest); the group box */
+    double gMin[3], gMax[3];
+    for (int i = 0; i < nDim; i++) { gMin[i] = 1e308; gMax[i] = -1e308; }
+    for (int s = 0; s < nSub; s++) {
+        int rem = s, c[3] = {0, 0, 0};
+        for (int i = nDim - 1; i >= 0; i--) { c[i] = rem % nFields[i]; rem /= nFields[i]; }
+        for (int i = 0; i < nDim; i++) {
+            const double start = xMin[i] + stepProc[i] * (double)c[i];
+            if (xMinSub) xMinSub[s * nDim + i] = start;
+            if (start < gMin[i]) gMin[i] = start;
+            if (start + procExtent[i] > gMax[i]) gMax[i] = start + procExtent[i];
+        }
+    }
+    for (int i = 0; i < nDim; i++) groupN[i] = 1 + (int32_t)lround((gMax[i] - gMin[i]) / xStep[i]);
+    for (int s = 0; s < nSub && origins; s++) {
+        int rem = s, c[3] = {0, 0, 0};
+        for (int i = nDim - 1; i >= 0; i--) { c[i] = rem % nFields[i]; rem /= nFields[i]; }
+        for (int i = 0; i < nDim; i++) {
+            const double start = xMin[i] + stepProc[i] * (double)c[i];
+            /* origin_Group-relative index: find_xNStep(xMin_Group, start) - 1 */
+            origins[s * nDim + i] = (int32_t)lround((start - gMin[i]) / xStep[i]);
+        }
+    }
+    int32_t nOvl[3];
+    rfo_overlap_npoints(nDim, overlap, corrL, xStep, nOvl);
+    for (int i = 0; i < nDim; i++) {
+        double* r = g + 13 * i;
+        r[0] = xStep[i]; r[1] = overlap[i]; r[2] = xMin[i]; r[3] = xMax[i];
+        r[4] = procExtent[i]; r[5] = stepProc[i]; r[6] = groupN[i]; r[7] = subN[i]; r[8] = nOvl[i];
+        r[9] = r[10] = r[11] = r[12] = 0.0;
+    }
+    return nSub;
+}
+
 /* localization_RF.f90:746 */
 void rfo_overlap_npoints(int nDim, const double* overlap, const double* corrL,
                          const double* xStep, int32_t* nOvl)

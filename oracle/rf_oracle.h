@@ -109,6 +109,17 @@ int rfo_marginal(int margiFirst, double fieldAvg, double fieldVar,
  * the reference pattern s/(n+2), FIXED the exact partition s/(n+1) with
  * square-root (variance-preserving) weights.  pu has prod(xNStep) entries,
  * x fastest. */
+/* Localization geometry of one localization level, in the reference's own
+ * floating-point steps (topography_RF.f90:19-144, localization_RF.f90:127-192,
+ * topography_RF.f90:150-255 with one rank per subdomain).  g[] receives per
+ * axis a (stride 13): xStep, overlap, xMinGlob, xMaxGlob, procExtent,
+ * stepProc, groupN, subN, nOvl (localization_RF.f90:746), and 4 unused; per
+ * subdomain s (field s+1, last axis fastest): origins (0-based group index),
+ * xMinSub.  Returns the number of subdomains. */
+int rfo_localization_layout(int nDim, const int32_t* nFields, const int32_t* pointsPerCorrL,
+                            const double* xMinGlob, const double* xMaxGlob, const double* corrL,
+                            const double* overlap, double* g, int32_t* origins, double* xMinSub);
+
 void rfo_overlap_npoints(int nDim, const double* overlap, const double* corrL,
                          const double* xStep, int32_t* nOvl);
 void rfo_unity_partition(int nDim, const int32_t* xNStep, const int32_t* nOvl,

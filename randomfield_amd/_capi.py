@@ -35,6 +35,8 @@ EXPORTS = (
     "rf_blend_plan_create", "rf_blend_plan_execute", "rf_blend_plan_destroy",
     "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy",
     "rf_graph_capture_begin", "rf_graph_capture_end", "rf_graph_launch", "rf_graph_destroy",
+    "rf_localization_layout", "rf_unity_partition", "rf_blend_host", "rf_normalize_window_device",
+    "rf_normalize_window_host",
 )
 
 
@@ -55,6 +57,23 @@ class SpectrumInfo(C.Structure):
         ("nPhaseUniform", C.c_int64), ("kNStep", C.c_int32 * 3), ("nSignBlocks", C.c_int32),
         ("nVarIso", C.c_int32), ("pad", C.c_int32), ("kMax", C.c_double * 3),
         ("kDelta", C.c_double * 3), ("ampMult", C.c_double), ("isoStep", C.c_double),
+    ]
+
+
+class LayoutParams(C.Structure):
+    _fields_ = [
+        ("nDim", C.c_int32), ("nFields", C.c_int32 * 3), ("pointsPerCorrL", C.c_int32 * 3), ("pad", C.c_int32),
+        ("xMinGlob", C.c_double * 3), ("xMaxGlob", C.c_double * 3), ("corrL", C.c_double * 3),
+        ("overlap", C.c_double * 3),
+    ]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [
+        ("nDim", C.c_int32), ("nSub", C.c_int32), ("groupN", C.c_int32 * 3), ("subN", C.c_int32 * 3),
+        ("nOvl", C.c_int32 * 3), ("nOvlRaw", C.c_int32 * 3), ("xStep", C.c_double * 3),
+        ("overlap", C.c_double * 3), ("xMinGlob", C.c_double * 3), ("xMaxGlob", C.c_double * 3),
+        ("procExtent", C.c_double * 3), ("stepProc", C.c_double * 3), ("xRangeSub", C.c_double * 3),
     ]
 
 
@@ -163,6 +182,11 @@ def lib() -> C.CDLL:
     L.rf_graph_capture_end.argtypes = [vp, vp, C.POINTER(vp)]
     L.rf_graph_launch.argtypes = [vp, vp]
     L.rf_graph_destroy.argtypes = [vp]
+    L.rf_localization_layout.argtypes = [C.POINTER(LayoutParams), C.POINTER(LayoutInfo), vp, vp, vp, vp]
+    L.rf_unity_partition.argtypes = [i32, vp, vp, i32, i32, vp]
+    L.rf_blend_host.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp]
+    L.rf_normalize_window_device.argtypes = [vp, vp, i32, vp, vp, vp, i32, i32, vp]
+    L.rf_normalize_window_host.argtypes = [vp, vp, i32, vp, vp, vp, i32]
     L.rf_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
     L.rf_server_destroy.argtypes = [vp]
     L.rf_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]

@@ -3,7 +3,8 @@
     librf_hip.so          HIP kernels + C ABI (hipcc, gfx950 code objects)
     librandomfield_gpu.so Fortran facade over the C ABI (amdflang)
     librf_h5.so           HDF5 / XDMF writer of structured fields (host, g++)
-    rf_peaks              measured MFMA / transcendental ceilings (bench.py --peaks)
+    rf_peaks              measured MFMA / transcendental ceilings (bench.py --peaks;
+                          source rf_peaks.hip beside bench.py)
     tests/fortran/bin/*   Fortran drivers of the facade used by the test suite
     oracle/liboracle_rf.so  the CPU oracle (test infrastructure, gcc)
 
@@ -46,14 +47,20 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
         raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
 
 
-def build_hip(jobs: int = 8) -> Path:
+def build_hip(jobs: int = 8, incremental: bool = False) -> Path:
+    """librf_hip.so; incremental=True recompiles only the sources newer than
+    their object (or older than any header), build_all() always compiles all."""
     OBJ.mkdir(parents=True, exist_ok=True)
     srcs = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+    headers = list(CSRC.glob("*.h")) + list((ROOT / "include").glob("*.h"))
+    newest_header = max((h.stat().st_mtime for h in headers), default=0.0)
     objs = []
     cmds = []
     for s in srcs:
         o = OBJ / (s.name + ".o")
         objs.append(o)
+        if incremental and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, newest_header):
+            continue
         lang = ["-x", "hip"] if s.suffix == ".hip" else []
         cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(s), "-o", str(o)])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -109,7 +116,7 @@ def build_peaks() -> Path:
     (bench.py reports them beside the datasheet peaks)."""
     out = LIB / "rf_peaks"
     tmp = LIB / "rf_peaks.tmp"
-    _run([HIPCC, *HIP_FLAGS, str(CSRC / "probe" / "rf_peaks.hip"), "-o", str(tmp)])
+    _run([HIPCC, *HIP_FLAGS, str(ROOT / "rf_peaks.hip"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

@@ -362,3 +362,91 @@ extern "C" int rf_blend_device(rf_context* ctx, int32_t nDim, const int32_t* gro
     if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("blend: ") + hipGetErrorString(e));
     return RF_OK;
 }
+
+// generateUnityPartition_Matrix (localization_RF.f90:705-857) as a host
+// table: pu[p] = product over tapered faces of the face factor, multiplied
+// face by face in the reference's order (lower faces of every axis, then
+// upper faces, :755-777).
+extern "C" int rf_unity_partition(int32_t nDim, const int32_t* xNStep, const int32_t* nOvl, int32_t faceMask,
+                                  int32_t mode, double* pu) {
+    if (nDim < 1 || nDim > 3 || !xNStep || !nOvl || !pu)
+        return rf_set_error(nullptr, RF_E_ARG, "rf_unity_partition: bad arguments");
+    if (mode != RF_REF_COMPAT && mode != RF_FIXED) return rf_set_error(nullptr, RF_E_ARG, "unknown mode");
+    int n[3] = {1, 1, 1};
+    int64_t tot = 1;
+    for (int a = 0; a < nDim; a++) {
+        if (xNStep[a] < 1) return rf_set_error(nullptr, RF_E_ARG, "rf_unity_partition: xNStep must be >= 1");
+        for (int f = 0; f < 2; f++)
+            if (((faceMask >> (2 * a + f)) & 1) && nOvl[a] > xNStep[a])
+                return rf_set_error(nullptr, RF_E_ARG, "rf_unity_partition: overlap wider than the box");
+        n[a] = xNStep[a];
+        tot *= n[a];
+    }
+    std::vector<double> low[3], up[3];
+    for (int a = 0; a < 3; a++) {
+        low[a].assign((size_t)n[a], 1.0);
+        up[a].assign((size_t)n[a], 1.0);
+        if (a < nDim)
+            axis_factors(n[a], nOvl[a], (faceMask >> (2 * a)) & 1, (faceMask >> (2 * a + 1)) & 1, mode,
+                         low[a].data(), up[a].data());
+    }
+    for (int64_t p = 0; p < tot; p++) {
+        const int i0 = (int)(p % n[0]), i1 = (int)((p / n[0]) % n[1]), i2 = (int)(p / ((int64_t)n[0] * n[1]));
+        double w = 1.0;
+        w = w * low[0][i0];
+        w = w * low[1][i1];
+        w = w * low[2][i2];
+        w = w * up[0][i0];
+        w = w * up[1][i1];
+        w = w * up[2][i2];
+        pu[p] = w;
+    }
+    return RF_OK;
+}
+
+// The blend of rf_blend_device on host buffers: fields[s] (sizes[s] product
+// x Nmc, FP64) uploaded, blended on the device, group (prod(groupN) x Nmc)
+// downloaded.  Synchronous.
+extern "C" int rf_blend_host(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc, int32_t nSub,
+                             const int32_t* origins, const int32_t* sizes, const int32_t* nOvl,
+                             const int32_t* faceMasks, int32_t mode, const double* const* fields, double* group) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!fields || !group || !sizes || !groupN || nSub < 1 || nDim < 1 || nDim > 3 || Nmc < 1)
+        return rf_set_error(ctx, RF_E_ARG, "NULL blend argument");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int64_t gTot = 1;
+    for (int a = 0; a < nDim; a++) gTot *= groupN[a];
+    std::vector<int64_t> off((size_t)nSub + 1, 0);
+    for (int s = 0; s < nSub; s++) {
+        int64_t t = 1;
+        for (int a = 0; a < nDim; a++) t *= sizes[s * nDim + a];
+        if (t < 1 || !fields[s]) return rf_set_error(ctx, RF_E_ARG, "bad subdomain field");
+        off[(size_t)s + 1] = off[(size_t)s] + t * Nmc;
+    }
+    const size_t fieldBytes = (size_t)off[(size_t)nSub] * sizeof(double);
+    const size_t groupBytes = (size_t)gTot * Nmc * sizeof(double);
+    char* d = nullptr;
+    if (hipMalloc((void**)&d, fieldBytes + groupBytes) != hipSuccess)
+        return rf_set_error(ctx, RF_E_NOMEM, "blend buffers allocation failed");
+    double* d_fields = reinterpret_cast<double*>(d);
+    double* d_group = reinterpret_cast<double*>(d + fieldBytes);
+    std::vector<const void*> ptrs((size_t)nSub);
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < nSub && e == hipSuccess; s++) {
+        ptrs[(size_t)s] = d_fields + off[(size_t)s];
+        e = hipMemcpyAsync(d_fields + off[(size_t)s], fields[s],
+                           (size_t)(off[(size_t)s + 1] - off[(size_t)s]) * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream);
+    }
+    int rc = e == hipSuccess ? rf_blend_device(ctx, nDim, groupN, Nmc, nSub, origins, sizes, nOvl, faceMasks, mode,
+                                               ptrs.data(), d_group, RF_DTYPE_F64, nullptr)
+                             : rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    if (rc == RF_OK) {
+        e = hipMemcpyAsync(group, d_group, groupBytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    return rc;
+}

@@ -75,5 +75,12 @@ hipError_t rf_launch_corr(const void* u, int dtype, int64_t N, int64_t strideA, 
                           double std, double* d_part, int nPart, double* d_R, hipStream_t s);
 hipError_t rf_launch_moments(const void* d, int64_t N, int Nmc, int dtype, double* d_part,
                              int nPart, double* d_sums, hipStream_t s);
+// A box window lo .. lo + ext - 1 of an n[0] x n[1] x n[2] grid (x fastest).
+struct rf_window {
+    int64_t n[3], lo[3], ext[3];
+    int64_t count;                // ext[0] * ext[1] * ext[2]
+};
+hipError_t rf_launch_window_moments(const void* d, int dtype, const rf_window& w, double* d_part, int nPart,
+                                    double* d_sums, hipStream_t s);
 hipError_t rf_launch_affine(void* d, int64_t N, int Nmc, int dtype, const double* d_shift,
                             const double* d_mul, hipStream_t s);

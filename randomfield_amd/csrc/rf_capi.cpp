@@ -835,6 +835,84 @@ int rf_normalize_device(rf_context* ctx, void* d_field, int64_t N, int32_t Nmc, 
     return RF_OK;
 }
 
+int rf_normalize_window_device(rf_context* ctx, void* d_field, int32_t nDim, const int32_t* gridN,
+                               const int32_t* minPos, const int32_t* maxPos, int32_t Nmc, int32_t dtype,
+                               void* stream) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!d_field || !gridN || !minPos || !maxPos || nDim < 1 || nDim > 3 || Nmc < 1)
+        return rf_set_error(ctx, RF_E_ARG, "bad field or window");
+    if (dtype != RF_DTYPE_F64 && dtype != RF_DTYPE_F32) return rf_set_error(ctx, RF_E_ARG, "unknown dtype");
+    rf_window w{};
+    int64_t N = 1;
+    w.count = 1;
+    for (int a = 0; a < 3; a++) {
+        w.n[a] = a < nDim ? gridN[a] : 1;
+        w.lo[a] = a < nDim ? minPos[a] - 1 : 0;
+        w.ext[a] = a < nDim ? (int64_t)maxPos[a] - minPos[a] + 1 : 1;
+        if (w.n[a] < 1 || w.lo[a] < 0 || w.ext[a] < 1 || w.lo[a] + w.ext[a] > w.n[a])
+            return rf_set_error(ctx, RF_E_ARG, "window outside the grid (1 <= minPos <= maxPos <= gridN)");
+        N *= w.n[a];
+        w.count *= w.ext[a];
+    }
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    int nPart = rf_moments_parts(w.count);
+    const size_t bytes = sizeof(double) * ((size_t)nPart * 2 + 2 + 2 * (size_t)Nmc);
+    int rc = ensure_scratch(ctx, bytes);
+    if (rc) return rc;
+    double* part = static_cast<double*>(ctx->d_scratch);
+    double* dsum = part + (size_t)nPart * 2;
+    double* dv = dsum + 2;                       // shift[Nmc], mul[Nmc]
+    std::vector<double> hv((size_t)Nmc * 2);
+    double hs[2];
+    // randomFieldND.f90:856-875: window mean of realization 1, subtracted
+    // from every realization
+    RF_HIP_TRY(ctx, rf_launch_window_moments(d_field, dtype, w, part, nPart, dsum, s));
+    RF_HIP_TRY(ctx, hipMemcpyAsync(hs, dsum, sizeof(hs), hipMemcpyDeviceToHost, s));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double avg = hs[0] / (double)w.count;
+    for (int r = 0; r < Nmc; r++) { hv[r] = avg; hv[Nmc + r] = 1.0; }
+    RF_HIP_TRY(ctx, hipMemcpyAsync(dv, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    RF_HIP_TRY(ctx, rf_launch_affine(d_field, N, Nmc, dtype, dv, dv + Nmc, s));
+    // :891-908: window root mean square of realization 1, every realization
+    // divided by it
+    RF_HIP_TRY(ctx, rf_launch_window_moments(d_field, dtype, w, part, nPart, dsum, s));
+    RF_HIP_TRY(ctx, hipMemcpyAsync(hs, dsum, sizeof(hs), hipMemcpyDeviceToHost, s));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double sd = std::sqrt(hs[1] / (double)w.count);
+    for (int r = 0; r < Nmc; r++) { hv[r] = 0.0; hv[Nmc + r] = 1.0 / sd; }
+    RF_HIP_TRY(ctx, hipMemcpyAsync(dv, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    RF_HIP_TRY(ctx, rf_launch_affine(d_field, N, Nmc, dtype, dv, dv + Nmc, s));
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));     // the coefficients are context scratch
+    return RF_OK;
+}
+
+int rf_normalize_window_host(rf_context* ctx, double* field, int32_t nDim, const int32_t* gridN,
+                             const int32_t* minPos, const int32_t* maxPos, int32_t Nmc) {
+    if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
+    if (!field || !gridN || nDim < 1 || nDim > 3 || Nmc < 1) return rf_set_error(ctx, RF_E_ARG, "bad field");
+    int64_t N = 1;
+    for (int a = 0; a < nDim; a++) {
+        if (gridN[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
+        N *= gridN[a];
+    }
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)N * Nmc * sizeof(double);
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "field allocation failed");
+    hipError_t e = hipMemcpyAsync(d, field, bytes, hipMemcpyHostToDevice, ctx->stream);
+    int rc = e == hipSuccess ? rf_normalize_window_device(ctx, d, nDim, gridN, minPos, maxPos, Nmc, RF_DTYPE_F64,
+                                                          nullptr)
+                             : rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    if (rc == RF_OK) {
+        e = hipMemcpyAsync(field, d, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
 int rf_device_uniforms(rf_context* ctx, uint64_t seed, uint32_t tag, int64_t count, double* d_out,
                        void* stream) {
     if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");

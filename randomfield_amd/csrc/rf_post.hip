@@ -87,6 +87,42 @@ __global__ __launch_bounds__(kBlock) void k_moments_partial(const T* __restrict_
     }
 }
 
+// Partial sums of u and u^2 over the box lo..hi (0-based, inclusive) of
+// realization 0 of a grid field (x fastest); grid (nPart).  The summed box is
+// the window normalize_randField reads through its RF_2D / RF_3D views of
+// column 1 (randomFieldND.f90:857-862).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_window_moments_partial(const T* __restrict__ d, rf_window w,
+                                                                   double* __restrict__ part) {
+    __shared__ double s1[kBlock], s2[kBlock];
+    double a = 0.0, b = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < w.count; q += stride) {
+        const int64_t i0 = q % w.ext[0];
+        const int64_t r = q / w.ext[0];
+        const int64_t i1 = r % w.ext[1];
+        const int64_t i2 = r / w.ext[1];
+        const int64_t off = (w.lo[0] + i0) + w.n[0] * ((w.lo[1] + i1) + w.n[1] * (w.lo[2] + i2));
+        const double v = (double)d[off];
+        a += v;
+        b = fma(v, v, b);
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = kBlock / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            s1[threadIdx.x] += s1[threadIdx.x + k];
+            s2[threadIdx.x] += s2[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[(int64_t)blockIdx.x * 2 + 0] = s1[0];
+        part[(int64_t)blockIdx.x * 2 + 1] = s2[0];
+    }
+}
+
 // Fixed-order final reduction of the partials: sums[r] = (sum u, sum u^2).
 __global__ void k_moments_final(const double* __restrict__ part, int nPart, int Nmc,
                                 double* __restrict__ sums) {
@@ -528,6 +564,18 @@ hipError_t rf_launch_moments(const void* d, int64_t N, int Nmc, int dtype, doubl
                            static_cast<const float*>(d), N, d_part);
     hipLaunchKernelGGL(k_moments_final, dim3((unsigned)((Nmc + 63) / 64)), dim3(64), 0, s, d_part,
                        nPart, Nmc, d_sums);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_window_moments(const void* d, int dtype, const rf_window& w, double* d_part, int nPart,
+                                    double* d_sums, hipStream_t s) {
+    if (dtype == RF_DTYPE_F64)
+        hipLaunchKernelGGL(k_window_moments_partial<double>, dim3((unsigned)nPart), dim3(kBlock), 0, s,
+                           static_cast<const double*>(d), w, d_part);
+    else
+        hipLaunchKernelGGL(k_window_moments_partial<float>, dim3((unsigned)nPart), dim3(kBlock), 0, s,
+                           static_cast<const float*>(d), w, d_part);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, s, d_part, nPart, 1, d_sums);
     return hipGetLastError();
 }
 

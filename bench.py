@@ -178,9 +178,12 @@ def run_cfg2(args):
 def run_cfg4(args):
     """cfg 4: 512x512x256 grid (spacing 0.25), 4x4x2 uneven subdomains with
     7-point overlaps, Shinozuka per subdomain on its own lattice (seed 100 +
-    field index), Nmc 16, FP32.  A step is the generation of all 32
-    subdomains (three grouped launches) and the variance-preserving blend,
-    replayed as one captured graph."""
+    field index), Nmc 16, FP32.  A step draws fresh realizations for all 32
+    subdomains (new phases from each field's device stream folded into the
+    coefficients), generates them (three grouped launches) and blends them
+    with the variance-preserving weights, replayed as one captured graph whose
+    draw counter advances on every replay.  The re-evaluation of frozen
+    realizations (no redraw) is reported beside it."""
     import numpy as np
     import torch
 
@@ -206,8 +209,13 @@ def run_cfg4(args):
     batch = R.GridBatch(ctx, plans, [x[0] for x in geo], [step] * len(geo), [x[1] for x in geo], fields,
                         R.DTYPE_F32)
     bplan = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, fields, R.DTYPE_F32)
-    graph = R.Graph(ctx, sptr)
-    with graph:
+    fresh = R.Graph(ctx, sptr)
+    with fresh:
+        batch.redraw(R.DRAW_NEXT, sptr)
+        batch.execute(sptr)
+        bplan.execute(g, sptr)
+    replay = R.Graph(ctx, sptr)
+    with replay:
         batch.execute(sptr)
         bplan.execute(g, sptr)
 
@@ -224,7 +232,10 @@ def run_cfg4(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps, np.array([a.elapsed_time(b) for a, b in ev])
 
-    wall_s, ms_s = timed(lambda: graph.launch(sptr), args.steps, args.warmup)
+    kinds = batch.autotune(sptr)                 # tile kinds per contraction (autotuner)
+    wall_f, ms_f = timed(lambda: fresh.launch(sptr), args.steps, args.warmup)
+    wall_r, ms_r = timed(lambda: replay.launch(sptr), args.steps, args.warmup)
+    _, ms_d = timed(lambda: batch.redraw(R.DRAW_NEXT, sptr), args.steps, 1)
     _, ms_g = timed(lambda: batch.execute(sptr), args.steps, 1)
     _, ms_b = timed(lambda: bplan.execute(g, sptr), args.steps, 1)
     nglob = float(np.prod(groupN))
@@ -232,14 +243,19 @@ def run_cfg4(args):
     # algorithmic FLOPs of the three contractions per step (8 per complex
     # MAC, 4 per real-part MAC; sign-block fold: K = 2 nk1, 2 nk2, nk3)
     flop_gen = 0.0
+    ncoef = 0.0
     for pl, (o_, n_, e_) in zip(plans, geo):
         k1, k2, k3 = [int(v) for v in pl.info.kNStep]
+        ncoef += Nmc * pl.info.kNTotal
         flop_gen += Nmc * (8.0 * (2 * k1) * (2 * k2 * k3) * n_[0] + 8.0 * (2 * k2) * k3 * n_[0] * n_[1]
                            + 4.0 * k3 * n_[0] * n_[1] * n_[2])
     work = nglob * Nmc
     bytes_blend = (nsub + nglob) * Nmc * 4.0
     gbs = bytes_blend / (ms_b.mean() * 1e-3) / 1e9
-    # accuracy: 24 points of subdomain 0 against the FP64 oracle
+    # accuracy: draw 0 (the plans' own realizations), 24 points of subdomain 0
+    batch.redraw(0, sptr)
+    batch.execute(sptr)
+    torch.cuda.synchronize()
     info = plans[0].info
     du = torch.empty(Nmc * info.kNTotal, dtype=torch.float64, device="cuda")
     R.device_uniforms(ctx, 101, 2, Nmc * info.kNTotal, du)
@@ -255,18 +271,61 @@ def run_cfg4(args):
     ref = O.shinozuka(po, xs[sel], k, sk, phU, threads=_threads())
     cpu_s = time.perf_counter() - t
     got = fields[0][:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
-    graph.close()
+    fresh.close()
+    replay.close()
+    # the blend fused into the last contraction: points one subdomain covers
+    # go straight to the group grid, the blend forms the overlaps only
+    batch.fuse_blend(bplan, g)
+    fused_fresh = R.Graph(ctx, sptr)
+    with fused_fresh:
+        batch.redraw(R.DRAW_NEXT, sptr)
+        batch.execute(sptr)
+        bplan.execute(g, sptr)
+    fused_replay = R.Graph(ctx, sptr)
+    with fused_replay:
+        batch.execute(sptr)
+        bplan.execute(g, sptr)
+    wall_ff, ms_ff = timed(lambda: fused_fresh.launch(sptr), args.steps, args.warmup)
+    wall_fr, ms_fr = timed(lambda: fused_replay.launch(sptr), args.steps, args.warmup)
+    _, ms_fg = timed(lambda: batch.execute(sptr), args.steps, 1)
+    _, ms_fb = timed(lambda: bplan.execute(g, sptr), args.steps, 1)
+    fused_fresh.close()
+    fused_replay.close()
+    batch.fuse_blend(bplan, None)
     bplan.close()
     batch.close()
     for pl in plans:
         pl.close()
     ctx.close()
-    return _line("points x realizations / s (cfg4 localization: generation + blend)", work / (wall_s * 1e-3),
-                 args.steps, args.warmup, wall_s, "f32",
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_cfg4_step.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text())
+        traffic["source"] = "profiles/traffic_cfg4_step.json (rocprofv3 --pmc of bench.py --config 4)"
+    best_fused = wall_ff <= wall_f
+    step_wall = wall_ff if best_fused else wall_f
+    return _line("points x realizations / s (cfg4 localization: fresh realizations, generation + blend)",
+                 work / (step_wall * 1e-3), args.steps, args.warmup, step_wall, "f32",
                  "cfg4: 512x512x256 grid, 4x4x2 uneven subdomains, 7-point overlap, Nmc 16, "
-                 f"N_k {info.kNTotal} (subdomain 0), grid form in grouped launches + variance-preserving blend, "
-                 "one captured graph per step",
-                 step_kernel_ms=float(ms_s.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
+                 f"N_k {info.kNTotal} (subdomain 0); per step: new phases for every field (device stream, "
+                 "folded into the coefficients), grid form in grouped launches, variance-preserving blend"
+                 + (" fused into the last contraction" if best_fused else "") + "; one captured graph per step",
+                 step_kernel_ms=float((ms_ff if best_fused else ms_f).mean()),
+                 tile_kinds=kinds,
+                 separate_blend={"ms_per_step": wall_f, "kernel_ms": float(ms_f.mean()),
+                                 "replay_ms_per_step": wall_r, "generation_ms": float(ms_g.mean()),
+                                 "blend_ms": float(ms_b.mean())},
+                 fused_blend={"ms_per_step": wall_ff, "kernel_ms": float(ms_ff.mean()),
+                              "replay_ms_per_step": wall_fr, "generation_ms": float(ms_fg.mean()),
+                              "overlap_blend_ms": float(ms_fb.mean())},
+                 frozen_replay={"ms_per_step": wall_fr if best_fused else wall_r,
+                                "kernel_ms": float((ms_fr if best_fused else ms_r).mean()),
+                                "value": work / ((wall_fr if best_fused else wall_r) * 1e-3),
+                                "note": "the same realizations re-evaluated (no redraw)"},
+                 redraw_ms=float(ms_d.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
+                 redraw={"coefficients_per_step": ncoef, "kernel_ms": float(ms_d.mean()),
+                         "note": "Philox phases + FP64 sincos + fold, 16 B per coefficient (amplitude read, "
+                                 "FP32 complex write)"},
                  generation_roofline={"bound": "mfma", "unit": "TFLOP/s", "peak": 2500.0,
                                       "achieved": flop_gen / (ms_g.mean() * 1e-3) / 1e12,
                                       "frac": flop_gen / (ms_g.mean() * 1e-3) / 1e12 / 2500.0,
@@ -276,7 +335,9 @@ def run_cfg4(args):
                  blend_roofline={"bound": "hbm", "achieved_GBps": gbs, "peak_GBps": 8000.0,
                                  "frac": gbs / 8000.0, "kernel_ms": float(ms_b.mean()),
                                  "bytes_per_launch": bytes_blend},
-                 accuracy={**_err(got, ref), "sample": "24 points of subdomain 0 x 16 realizations"},
+                 output_bytes_per_step=nglob * Nmc * 4.0,
+                 traffic=traffic,
+                 accuracy={**_err(got, ref), "sample": "24 points of subdomain 0 x 16 realizations (draw 0)"},
                  cpu_baseline={"value": 24 * Nmc / cpu_s, "unit": "points*realizations/s",
                                "cores": _threads(), "kind": "port", "seconds": cpu_s,
                                "sample": f"24 points x 16 realizations x {info.kNTotal} wavevectors"})
@@ -465,6 +526,10 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000, help="cfg 5: requests in the stream")
     ap.add_argument("--budget-gib", type=int, default=64, help="cfg 5: server memory budget")
     ap.add_argument("--peaks", action="store_true", help="only print the measured unit ceilings (JSON)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: skip the cfg1 / cfg4 / cfg5 records of the default line")
+    ap.add_argument("--cpu-points-1core", type=int, default=400,
+                    help="points of the single-core CPU baseline sample")
     ap.add_argument("--config", type=int, default=3,
                     help="benchmark configuration 1..6 (3, the headline workload, by default; "
                          "6 is method 4, the FFT method)")
@@ -603,11 +668,21 @@ def main() -> int:
                "kind": "port", "seconds": cs,
                "sample": f"first {args.cpu_points} of the 2,000,000 points, all 256 realizations and "
                          f"4096 wavevectors (FP64 oracle, reference operation order)"}
+        # one core: the reference runs serially per MPI rank
+        n1 = args.cpu_points_1core
+        t3 = time.perf_counter()
+        O.shinozuka(po, xs[:n1], kt, skt, phU, threads=1)
+        c1 = time.perf_counter() - t3
+        cpu["single_core"] = {"value": n1 * N_MC / c1, "cores": 1, "seconds": c1,
+                              "sample": f"first {n1} points, all 256 realizations and 4096 wavevectors"}
 
     traffic = None
+    traffic_src = None
     tf = ROOT / "profiles" / "traffic_cfg3_mfma_f16.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        traffic_src = ("profiles/traffic_cfg3_mfma_f16.json: an earlier rocprofv3 --pmc run of this "
+                       "kernel (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch), not measured in this run")
 
     # the f16 MFMA ceiling measured on this device with operands that change
     # every instruction (the datasheet peak assumes a clock the chip does not
@@ -637,6 +712,7 @@ def main() -> int:
                    "points": N_POINTS, "realizations": N_MC, "wavevectors": N_K},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_F16_DENSE / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_F16_DENSE, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel_ms": float(kms.mean()),
                      "flop_per_launch": flop_alg, "executed_mfma_flop_per_launch": 3 * flop_alg,
                      "variant": chosen, "variants_kernel_ms": variants_ms,
@@ -646,11 +722,28 @@ def main() -> int:
         "accuracy": acc,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
     plan.close()
     if dplan is not None:
         dplan.close()
     ctx.close()
+    del dx, out, out_d
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        # the other workloads, each on its own context, in the same run
+        extras = {}
+        for name, fn in (("cfg1", run_cfg1), ("cfg4", run_cfg4), ("cfg5", run_cfg5)):
+            t = time.perf_counter()
+            try:
+                extras[name] = fn(args)
+            except Exception as e:                       # recorded, the headline still prints
+                extras[name] = {"error": f"{type(e).__name__}: {e}"}
+            extras[name]["bench_seconds"] = time.perf_counter() - t
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+            print(f"[bench] {name} done in {extras[name]['bench_seconds']:.1f} s", file=sys.stderr, flush=True)
+        line["other_configs"] = extras
+    print(json.dumps(line), flush=True)
     return 0
 
 

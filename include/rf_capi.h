@@ -314,6 +314,36 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
                          const int32_t* nsteps, void* const* d_outs,
                          int32_t outDtype, rf_grid_batch** batch);
 int rf_grid_batch_execute(rf_grid_batch* batch, void* stream);
+/* Fresh realizations for the next executes: every field's coefficients are
+ * re-formed from new phases 2 pi U, U from the device stream of its plan
+ * (rf_params.seed, tag 2), realizations d*Nmc .. d*Nmc + Nmc - 1 of that
+ * stream, where d = drawIndex (>= 0; 0 gives the plan's own realizations when
+ * it was created without caller draws) or, for RF_DRAW_NEXT, a counter the
+ * batch keeps on the device (1 after create), used and then advanced -- so a
+ * graph captured with RF_DRAW_NEXT draws new realizations on every replay.
+ * Each realization is generated as in the reference (a new random_number
+ * draw per createRandomField call, randomFieldND.f90:276-279).  Allocation-
+ * free, capturable; method 4 batches: RF_E_UNSUPPORTED. */
+#define RF_DRAW_NEXT (-1)
+/* Times each contraction stage of the batch with the wide tile kinds the
+ * set-up chose and with 64 x 64 tiles on every GEMM of the stage, keeps the
+ * faster (chosen[3]: the widest kind now used per stage, 1 = 64 x 64) and
+ * leaves one execute's result in the outputs.  The field values do not depend
+ * on the tile kind (the same K order per element).  Synchronous; no
+ * counterpart in the reference. */
+int rf_grid_batch_autotune(rf_grid_batch* batch, void* stream, int32_t* chosen);
+/* The blend fused into the batch's last contraction (FIXED mode, a
+ * tensor-product layout whose fields are the batch's outputs; same element
+ * type): points that one field alone covers -- weight 1 -- are stored by the
+ * last contraction straight into d_group, the others into the field as
+ * before, and rf_blend_plan_execute then forms only the points several
+ * fields cover.  The group and the fields are the same as without fusion,
+ * bit for bit, with the fields' interior neither written to nor read back
+ * from the field buffers.  execute of the blend plan must then be given
+ * d_group; d_group NULL undoes the fusion.  Synchronous set-up; the batch's
+ * tables are replaced. */
+int rf_grid_batch_fuse_blend(rf_grid_batch* batch, rf_blend_plan* blend, void* d_group);
+int rf_grid_batch_redraw(rf_grid_batch* batch, int64_t drawIndex, void* stream);
 int rf_grid_batch_destroy(rf_grid_batch* batch);
 
 /* Graph capture of the launches a thread enqueues on an explicit stream

@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from ._capi import (  # noqa: F401
-    CM_COS, CM_GAUSSIAN, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
+    CM_COS, CM_GAUSSIAN, DRAW_NEXT, DTYPE_F32, DTYPE_F64, FIXED, FOM_GAUSSIAN, FOM_LOGNORMAL, FOM_NONE,
     FFT, FORM_FP32, FORM_FP64, FORM_MFMA_BF16, FORM_MFMA_F16, ISOTROPIC, KSAMP_BOX, KSAMP_DENSITY,
     MFMA_VARIANTS, RANDOMIZATION, REF_COMPAT, RF_E_ARG, RF_E_HIP, RF_E_LOGNORMAL, RF_E_NODEVICE, RF_E_NOMEM,
     RF_E_UNSUPPORTED, RF_OK, SHINOZUKA, EXPORTS, LayoutInfo, LayoutParams, Params, RFError, SpectrumInfo, check,
@@ -331,6 +331,24 @@ class GridBatch:
 
     def execute(self, stream=None) -> None:
         check(lib().rf_grid_batch_execute(self._h, _ptr(stream)), self.ctx.handle)
+
+    def autotune(self, stream=None):
+        """Times each stage with its wide tile kinds and with 64 x 64 tiles,
+        keeps the faster; returns the widest kind used per stage."""
+        chosen = (C.c_int32 * 3)()
+        check(lib().rf_grid_batch_autotune(self._h, _ptr(stream), chosen), self.ctx.handle)
+        return list(chosen)
+
+    def fuse_blend(self, blend_plan, d_group) -> None:
+        """Single-covered points go from the last contraction straight into
+        d_group; blend_plan.execute(d_group) then forms the overlaps.
+        d_group None undoes the fusion."""
+        check(lib().rf_grid_batch_fuse_blend(self._h, blend_plan._h, _ptr(d_group)), self.ctx.handle)
+
+    def redraw(self, draw_index: int = -1, stream=None) -> None:
+        """New phases for every field: realizations draw_index*Nmc.. of each
+        plan's device stream, or (-1) the next set from the batch's counter."""
+        check(lib().rf_grid_batch_redraw(self._h, int(draw_index), _ptr(stream)), self.ctx.handle)
 
     def close(self) -> None:
         if self._h:

@@ -21,6 +21,7 @@ FORM_FP64, FORM_FP32, FORM_MFMA_F16, FORM_MFMA_BF16 = 0, 1, 2, 3
 KSAMP_BOX, KSAMP_DENSITY = 0, 1
 DTYPE_F64, DTYPE_F32 = 0, 1
 MFMA_VARIANTS = 9
+DRAW_NEXT = -1
 
 # every entry point the header declares
 EXPORTS = (
@@ -33,7 +34,8 @@ EXPORTS = (
     "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_read",
     "rf_server_release", "rf_server_get_stats",
     "rf_blend_plan_create", "rf_blend_plan_execute", "rf_blend_plan_destroy",
-    "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy",
+    "rf_grid_batch_create", "rf_grid_batch_execute", "rf_grid_batch_destroy", "rf_grid_batch_redraw",
+    "rf_grid_batch_autotune", "rf_grid_batch_fuse_blend",
     "rf_graph_capture_begin", "rf_graph_capture_end", "rf_graph_launch", "rf_graph_destroy",
     "rf_localization_layout", "rf_unity_partition", "rf_blend_host", "rf_normalize_window_device",
     "rf_normalize_window_host",
@@ -130,6 +132,18 @@ _lib = None
 _flib = None
 
 
+def _sig(L, name, argtypes, restype=C.c_int):
+    """Declares a foreign function's signature.  A symbol missing from the
+    library (a binding newer than the built .so) is left undeclared, so the
+    rest of the library stays usable and calling it raises AttributeError."""
+    f = getattr(L, name, None)
+    if f is None:
+        return
+    if argtypes is not None:
+        f.argtypes = argtypes
+    f.restype = restype
+
+
 def lib() -> C.CDLL:
     """librf_hip.so; raises when it is not built (no CPU fallback)."""
     global _lib
@@ -140,60 +154,63 @@ def lib() -> C.CDLL:
         raise RuntimeError(f"{path} is missing: run __graft_entry__.build() (the GPU path has no fallback)")
     L = C.CDLL(str(path))
     vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
-    L.rf_context_create.argtypes = [C.POINTER(vp), C.c_int]
-    L.rf_context_destroy.argtypes = [vp]
-    L.rf_last_error.argtypes = [vp]
-    L.rf_last_error.restype = C.c_char_p
-    L.rf_context_sync.argtypes = [vp]
-    L.rf_plan_spectrum.argtypes = [C.POINTER(Params), C.POINTER(SpectrumInfo)]
-    L.rf_spectrum_tables.argtypes = [C.POINTER(Params), vp, vp, vp]
-    L.rf_plan_create.argtypes = [vp, C.POINTER(Params), vp, vp, vp, C.POINTER(vp)]
-    L.rf_plan_destroy.argtypes = [vp]
-    L.rf_plan_info.argtypes = [vp, C.POINTER(SpectrumInfo)]
-    L.rf_plan_execute.argtypes = [vp, vp, i64, vp, i32, vp]
-    L.rf_plan_set_variant.argtypes = [vp, i32]
-    L.rf_plan_autotune.argtypes = [vp, vp, i64, vp, i32, vp, vp]
-    L.rf_plan_execute_grid.argtypes = [vp, vp, vp, vp, vp, i32, vp]
-    L.rf_generate.argtypes = [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32]
-    L.rf_marginal_device.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp]
-    L.rf_marginal_host.argtypes = [vp, i32, C.c_double, C.c_double, vp, i64]
-    L.rf_blend_device.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]
-    L.rf_stats_device.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
-    L.rf_normalize_device.argtypes = [vp, vp, i64, i32, i32, vp]
-    L.rf_corr_device.argtypes = [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp]
-    L.rf_interpolate_device.argtypes = [vp, i32, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
-    L.rf_generate_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i32]
-    L.rf_interpolate_box_device.argtypes = [vp, i32, vp, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp]
-    L.rf_unv_read.argtypes = [C.c_char_p, i32, vp]
-    L.rf_unv_sizes.argtypes = [vp, vp, vp, vp, vp]
-    L.rf_unv_copy.argtypes = [vp, vp, vp, vp]
-    L.rf_unv_destroy.argtypes = [vp]
-    L.rf_device_uniforms.argtypes = [vp, C.c_uint64, C.c_uint32, i64, vp, vp]
-    L.rf_uniform_host.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
-    L.rf_uniform_host.restype = C.c_double
-    L.rf_philox4x32_10_host.argtypes = [vp, vp, vp]
-    L.rf_blend_plan_create.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, i32, C.POINTER(vp)]
-    L.rf_blend_plan_execute.argtypes = [vp, vp, vp]
-    L.rf_blend_plan_destroy.argtypes = [vp]
-    L.rf_grid_batch_create.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, C.POINTER(vp)]
-    L.rf_grid_batch_execute.argtypes = [vp, vp]
-    L.rf_grid_batch_destroy.argtypes = [vp]
-    L.rf_graph_capture_begin.argtypes = [vp, vp]
-    L.rf_graph_capture_end.argtypes = [vp, vp, C.POINTER(vp)]
-    L.rf_graph_launch.argtypes = [vp, vp]
-    L.rf_graph_destroy.argtypes = [vp]
-    L.rf_localization_layout.argtypes = [C.POINTER(LayoutParams), C.POINTER(LayoutInfo), vp, vp, vp, vp]
-    L.rf_unity_partition.argtypes = [i32, vp, vp, i32, i32, vp]
-    L.rf_blend_host.argtypes = [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp]
-    L.rf_normalize_window_device.argtypes = [vp, vp, i32, vp, vp, vp, i32, i32, vp]
-    L.rf_normalize_window_host.argtypes = [vp, vp, i32, vp, vp, vp, i32]
-    L.rf_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
-    L.rf_server_destroy.argtypes = [vp]
-    L.rf_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]
-    L.rf_server_wait.argtypes = [vp, i64, C.c_double, C.POINTER(RequestResult)]
-    L.rf_server_read.argtypes = [vp, i64, vp]
-    L.rf_server_release.argtypes = [vp, i64]
-    L.rf_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    _sig(L, "rf_context_create", [C.POINTER(vp), C.c_int])
+    _sig(L, "rf_context_destroy", [vp])
+    _sig(L, "rf_last_error", [vp])
+    _sig(L, "rf_last_error", None, C.c_char_p)
+    _sig(L, "rf_context_sync", [vp])
+    _sig(L, "rf_plan_spectrum", [C.POINTER(Params), C.POINTER(SpectrumInfo)])
+    _sig(L, "rf_spectrum_tables", [C.POINTER(Params), vp, vp, vp])
+    _sig(L, "rf_plan_create", [vp, C.POINTER(Params), vp, vp, vp, C.POINTER(vp)])
+    _sig(L, "rf_plan_destroy", [vp])
+    _sig(L, "rf_plan_info", [vp, C.POINTER(SpectrumInfo)])
+    _sig(L, "rf_plan_execute", [vp, vp, i64, vp, i32, vp])
+    _sig(L, "rf_plan_set_variant", [vp, i32])
+    _sig(L, "rf_plan_autotune", [vp, vp, i64, vp, i32, vp, vp])
+    _sig(L, "rf_plan_execute_grid", [vp, vp, vp, vp, vp, i32, vp])
+    _sig(L, "rf_generate", [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32])
+    _sig(L, "rf_marginal_device", [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp])
+    _sig(L, "rf_marginal_host", [vp, i32, C.c_double, C.c_double, vp, i64])
+    _sig(L, "rf_blend_device", [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp, i32, vp])
+    _sig(L, "rf_stats_device", [vp, vp, i64, i32, i32, vp, vp, vp])
+    _sig(L, "rf_normalize_device", [vp, vp, i64, i32, i32, vp])
+    _sig(L, "rf_corr_device", [vp, vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, vp, vp])
+    _sig(L, "rf_interpolate_device", [vp, i32, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp])
+    _sig(L, "rf_generate_grid", [vp, vp, vp, vp, vp, vp, vp, vp, i32])
+    _sig(L, "rf_interpolate_box_device", [vp, i32, vp, vp, vp, vp, i32, vp, i32, vp, i64, vp, i32, vp, vp])
+    _sig(L, "rf_unv_read", [C.c_char_p, i32, vp])
+    _sig(L, "rf_unv_sizes", [vp, vp, vp, vp, vp])
+    _sig(L, "rf_unv_copy", [vp, vp, vp, vp])
+    _sig(L, "rf_unv_destroy", [vp])
+    _sig(L, "rf_device_uniforms", [vp, C.c_uint64, C.c_uint32, i64, vp, vp])
+    _sig(L, "rf_uniform_host", [C.c_uint64, C.c_uint32, C.c_uint64])
+    _sig(L, "rf_uniform_host", None, C.c_double)
+    _sig(L, "rf_philox4x32_10_host", [vp, vp, vp])
+    _sig(L, "rf_blend_plan_create", [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, i32, C.POINTER(vp)])
+    _sig(L, "rf_blend_plan_execute", [vp, vp, vp])
+    _sig(L, "rf_blend_plan_destroy", [vp])
+    _sig(L, "rf_grid_batch_create", [vp, i32, vp, vp, vp, vp, vp, i32, C.POINTER(vp)])
+    _sig(L, "rf_grid_batch_execute", [vp, vp])
+    _sig(L, "rf_grid_batch_destroy", [vp])
+    _sig(L, "rf_grid_batch_redraw", [vp, i64, vp])
+    _sig(L, "rf_grid_batch_autotune", [vp, vp, vp])
+    _sig(L, "rf_grid_batch_fuse_blend", [vp, vp, vp])
+    _sig(L, "rf_graph_capture_begin", [vp, vp])
+    _sig(L, "rf_graph_capture_end", [vp, vp, C.POINTER(vp)])
+    _sig(L, "rf_graph_launch", [vp, vp])
+    _sig(L, "rf_graph_destroy", [vp])
+    _sig(L, "rf_localization_layout", [C.POINTER(LayoutParams), C.POINTER(LayoutInfo), vp, vp, vp, vp])
+    _sig(L, "rf_unity_partition", [i32, vp, vp, i32, i32, vp])
+    _sig(L, "rf_blend_host", [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp])
+    _sig(L, "rf_normalize_window_device", [vp, vp, i32, vp, vp, vp, i32, i32, vp])
+    _sig(L, "rf_normalize_window_host", [vp, vp, i32, vp, vp, vp, i32])
+    _sig(L, "rf_server_create", [vp, C.POINTER(ServerConfig), C.POINTER(vp)])
+    _sig(L, "rf_server_destroy", [vp])
+    _sig(L, "rf_server_submit", [vp, C.POINTER(Request), C.POINTER(i64)])
+    _sig(L, "rf_server_wait", [vp, i64, C.c_double, C.POINTER(RequestResult)])
+    _sig(L, "rf_server_read", [vp, i64, vp])
+    _sig(L, "rf_server_release", [vp, i64])
+    _sig(L, "rf_server_get_stats", [vp, C.POINTER(ServerStats)])
     _lib = L
     return L
 

@@ -40,6 +40,11 @@ HIP_FLAGS = [
 ]
 
 
+# sources compiled as several translation units (one per value of a macro)
+SPLIT = {"rf_mfma.hip": 7}
+SPLIT_MACRO = {"rf_mfma.hip": "RF_MFMA_PART"}
+
+
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
     proc = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -57,12 +62,17 @@ def build_hip(jobs: int = 8, incremental: bool = False) -> Path:
     objs = []
     cmds = []
     for s in srcs:
-        o = OBJ / (s.name + ".o")
-        objs.append(o)
-        if incremental and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, newest_header):
-            continue
-        lang = ["-x", "hip"] if s.suffix == ".hip" else []
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(s), "-o", str(o)])
+        for part in range(SPLIT.get(s.name, 0) or 1):
+            o = OBJ / (s.name + (f".{part}" if s.name in SPLIT else "") + ".o")
+            objs.append(o)
+            if incremental and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, newest_header):
+                continue
+            lang = ["-x", "hip"] if s.suffix == ".hip" else []
+            defs = [f"-D{SPLIT_MACRO[s.name]}={part}"] if s.name in SPLIT else []
+            cmds.append([HIPCC, *HIP_FLAGS, *defs, *lang, "-c", str(s), "-o", str(o)])
+    for stale in OBJ.glob("*.o"):
+        if stale not in objs:
+            stale.unlink()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     out = LIB / "librf_hip.so"

@@ -15,6 +15,7 @@
 #include <new>
 #include <vector>
 
+#include "rf_blend.h"
 #include "rf_internal.h"
 
 struct rf_grid_batch {
@@ -25,6 +26,20 @@ struct rf_grid_batch {
     int64_t tiles[3] = {0, 0, 0};
     int realOut[3] = {0, 0, 1};
     int rwMax[3] = {1, 1, 1};           // wide tile kind used by a stage's GEMMs (1: none)
+    // fresh realizations: every field's coefficient fold, its seed and Nmc,
+    // and the device draw counter (realizations counter * Nmc .. of each
+    // field's device stream)
+    std::vector<rf_fold_desc> folds;
+    std::vector<uint64_t> seeds;
+    std::vector<int> nmc;
+    bool redrawable = true;
+    int64_t* d_draw = nullptr;
+    // host copies of the stage descriptors and the tile kinds the set-up
+    // chose (the autotuner switches a stage between those and 64 x 64)
+    std::vector<rf_gemm_desc> stage[3];
+    std::vector<int> kind0[3];
+    int outDtype = RF_DTYPE_F32;
+    char* d_fuse = nullptr;             // fused blend: row / column tables of the last stage
 };
 
 struct rf_graph {
@@ -35,6 +50,27 @@ struct rf_graph {
 
 namespace {
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// tile offsets, tile counts and widest kind of every stage, uploaded
+hipError_t upload_stages(rf_grid_batch* B, hipStream_t s) {
+    std::vector<rf_gemm_desc> all;
+    for (int k = 0; k < 3; k++) {
+        int64_t t = 0;
+        B->rwMax[k] = 1;
+        for (auto& g : B->stage[k]) {
+            g.tile0 = t;
+            t += rf_gemm_tiles(g);
+            if (g.rw > B->rwMax[k]) B->rwMax[k] = g.rw;
+        }
+        B->tiles[k] = t;
+        all.insert(all.end(), B->stage[k].begin(), B->stage[k].end());
+    }
+    for (int k = 0; k < 3; k++)
+        if (B->tiles[k] > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyAsync(B->d_desc[0], all.data(), all.size() * sizeof(rf_gemm_desc), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
 }  // namespace
 
 extern "C" {
@@ -77,10 +113,13 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
     }
     const size_t descOff = total;
     total += align256(sizeof(rf_gemm_desc) * 3 * (size_t)n);
+    const size_t drawOff = total;
+    total += 256;
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<rf_grid_batch> B(new (std::nothrow) rf_grid_batch());
     if (!B) return rf_set_error(ctx, RF_E_NOMEM, "batch allocation failed");
     B->ctx = ctx;
+    B->outDtype = outDtype;
     if (hipMalloc((void**)&B->d_mem, total) != hipSuccess)
         return rf_set_error(ctx, RF_E_NOMEM, "batch scratch allocation failed");
     const hipStream_t s = ctx->stream;
@@ -114,32 +153,30 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
             (void)hipFree(B->d_mem);
             return rf_set_error(ctx, RF_E_HIP, std::string("batch tables: ") + hipGetErrorString(e));
         }
+        B->folds.push_back(gs.fold);
+        B->seeds.push_back(p.seed);
+        B->nmc.push_back(p.Nmc);
+        if (fft) B->redrawable = false;
         // stage slots: 0 first contraction, 1 middle (3-D), 2 last (real)
         stage[0].push_back(gs.g[0]);
         if (gs.n == 3) stage[1].push_back(gs.g[1]);
         stage[2].push_back(gs.g[gs.n - 1]);
     }
-    std::vector<rf_gemm_desc> all;
     size_t dOff = 0;
     for (int k = 0; k < 3; k++) {
-        int64_t t = 0;
-        for (auto& g : stage[k]) {
-            g.tile0 = t;
-            t += rf_gemm_tiles(g);
-            if (g.rw > B->rwMax[k]) B->rwMax[k] = g.rw;
-        }
-        B->tiles[k] = t;
+        B->stage[k] = stage[k];
+        for (const auto& g : stage[k]) B->kind0[k].push_back(g.rw);
         B->nDesc[k] = (int)stage[k].size();
         B->d_desc[k] = reinterpret_cast<rf_gemm_desc*>(B->d_mem + descOff) + dOff;
-        all.insert(all.end(), stage[k].begin(), stage[k].end());
         dOff += stage[k].size();
-        if (t > 0x7fffffffLL) {
-            (void)hipFree(B->d_mem);
-            return rf_set_error(ctx, RF_E_ARG, "batch too large for one grouped launch");
-        }
     }
-    hipError_t e = hipMemcpyAsync(B->d_mem + descOff, all.data(), all.size() * sizeof(rf_gemm_desc),
-                                  hipMemcpyHostToDevice, s);
+    hipError_t e = upload_stages(B.get(), s);
+    if (e == hipErrorInvalidValue) {
+        (void)hipFree(B->d_mem);
+        return rf_set_error(ctx, RF_E_ARG, "batch too large for one grouped launch");
+    }
+    B->d_draw = reinterpret_cast<int64_t*>(B->d_mem + drawOff);
+    if (e == hipSuccess) e = rf_launch_draw_counter(B->d_draw, 1, s);   // draw 0: the plans' own realizations
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         (void)hipFree(B->d_mem);
@@ -158,10 +195,162 @@ int rf_grid_batch_execute(rf_grid_batch* B, void* stream) {
     return RF_OK;
 }
 
+int rf_grid_batch_autotune(rf_grid_batch* B, void* stream, int32_t* chosen) {
+    if (!B) return rf_set_error(nullptr, RF_E_ARG, "batch is NULL");
+    rf_context* ctx = B->ctx;
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    int rc = rf_grid_batch_execute(B, stream);          // every stage's inputs in place
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RF_HIP_TRY(ctx, hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(he));
+    }
+    auto time_stage = [&](int k, float* ms) -> hipError_t {
+        hipError_t e = rf_launch_gemm_group(B->realOut[k], B->d_desc[k], B->nDesc[k], B->tiles[k], B->rwMax[k], s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int i = 0; i < 3 && e == hipSuccess; i++)
+            e = rf_launch_gemm_group(B->realOut[k], B->d_desc[k], B->nDesc[k], B->tiles[k], B->rwMax[k], s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+        return e;
+    };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        bool wide = false;
+        for (int r : B->kind0[k]) wide |= r > 1;
+        if (!wide) continue;
+        // candidates: the set-up's wide kinds, and 64 x 64 tiles everywhere
+        float best = 0.0f;
+        int bestOpt = 0;
+        for (int opt = 0; opt < 2 && e == hipSuccess; opt++) {
+            for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = opt == 0 ? B->kind0[k][i] : 1;
+            e = upload_stages(B, s);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = time_stage(k, &ms);
+            if (e == hipSuccess && (opt == 0 || ms < best)) {
+                best = ms;
+                bestOpt = opt;
+            }
+        }
+        for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = bestOpt == 0 ? B->kind0[k][i] : 1;
+    }
+    if (e == hipSuccess) e = upload_stages(B, s);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("batch autotune: ") + hipGetErrorString(e));
+    if (chosen)
+        for (int k = 0; k < 3; k++) chosen[k] = B->rwMax[k];
+    return RF_OK;
+}
+
+int rf_grid_batch_redraw(rf_grid_batch* B, int64_t drawIndex, void* stream) {
+    if (!B) return rf_set_error(nullptr, RF_E_ARG, "batch is NULL");
+    rf_context* ctx = B->ctx;
+    if (!B->redrawable)
+        return rf_set_error(ctx, RF_E_UNSUPPORTED, "rf_grid_batch_redraw: method 4 fields have no phases to redraw");
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (drawIndex >= 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, drawIndex, s));
+    for (size_t i = 0; i < B->folds.size(); i++)
+        RF_HIP_TRY(ctx, rf_grid_refold_draw(B->folds[i], B->nmc[i], B->seeds[i], B->d_draw, s));
+    if (drawIndex < 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, -1, s));
+    return RF_OK;
+}
+
+int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) {
+    if (!B || !P) return rf_set_error(B ? B->ctx : nullptr, RF_E_ARG, "batch or blend plan is NULL");
+    rf_context* ctx = B->ctx;
+    if (P->ctx != ctx) return rf_set_error(ctx, RF_E_ARG, "blend plan of another context");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = ctx->stream;
+    RF_HIP_TRY(ctx, hipStreamSynchronize(s));          // no launch may read the tables being replaced
+    std::vector<rf_gemm_desc>& last = B->stage[2];
+    if (!d_group) {                                    // back to the separate blend
+        for (auto& g : last) { g.grp = nullptr; g.grpRow = nullptr; g.grpCol = nullptr; g.grpStrideB = 0; }
+        P->sep.skipSingle = 0;
+        P->fusedGroup = nullptr;
+        RF_HIP_TRY(ctx, upload_stages(B, s));
+        if (B->d_fuse) (void)hipFree(B->d_fuse);
+        B->d_fuse = nullptr;
+        return RF_OK;
+    }
+    if (!P->separable || P->mode != RF_FIXED)
+        return rf_set_error(ctx, RF_E_UNSUPPORTED, "the fused blend needs a FIXED-mode tensor-product layout");
+    if (P->dtype != B->outDtype) return rf_set_error(ctx, RF_E_ARG, "blend and batch element types differ");
+    const int d = P->nDim;
+    int64_t gTot = 1;
+    for (int a = 0; a < d; a++) gTot *= P->groupN[a];
+    // per field: its subdomain in the blend, row table (last axis) and
+    // column table (the plane of the other axes); -1 where another field
+    // also covers the index along that axis
+    std::vector<int64_t> rows, cols;
+    std::vector<size_t> rowOff, colOff;
+    for (const auto& g : last) {
+        if (!g.split16) return rf_set_error(ctx, RF_E_UNSUPPORTED, "fused blend: FP32-operand last stage");
+        int s0 = -1;
+        for (size_t k = 0; k < P->fields.size(); k++)
+            if (P->fields[k] == g.D) { s0 = (int)k; break; }
+        if (s0 < 0) return rf_set_error(ctx, RF_E_ARG, "a batch output is not a field of the blend plan");
+        const int32_t* o = &P->origins[(size_t)s0 * d];
+        const int32_t* n = &P->sizes[(size_t)s0 * d];
+        int64_t plane = 1;
+        for (int a = 0; a + 1 < d; a++) plane *= n[a];
+        if (g.M != n[d - 1] || g.N != plane) return rf_set_error(ctx, RF_E_ARG, "field and blend box sizes differ");
+        auto single = [&](int a, int gi) { return P->cover[a][(size_t)gi * 2 + 1] < 0; };
+        int64_t planeG = 1;
+        for (int a = 0; a + 1 < d; a++) planeG *= P->groupN[a];
+        rowOff.push_back(rows.size());
+        for (int z = 0; z < n[d - 1]; z++)
+            rows.push_back(single(d - 1, o[d - 1] + z) ? (int64_t)(o[d - 1] + z) * planeG : -1);
+        colOff.push_back(cols.size());
+        for (int64_t p = 0; p < plane; p++) {
+            const int x = (int)(p % n[0]);
+            const int y = d == 3 ? (int)(p / n[0]) : 0;
+            bool in = single(0, o[0] + x);
+            if (d == 3) in = in && single(1, o[1] + y);
+            cols.push_back(in ? (int64_t)(o[0] + x) + (d == 3 ? (int64_t)P->groupN[0] * (o[1] + y) : 0) : -1);
+        }
+    }
+    char* d_tab = nullptr;
+    const size_t bytes = (rows.size() + cols.size()) * sizeof(int64_t);
+    if (hipMalloc((void**)&d_tab, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "fused blend tables");
+    hipError_t e = hipMemcpyAsync(d_tab, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_tab + rows.size() * sizeof(int64_t), cols.data(), cols.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        (void)hipFree(d_tab);
+        return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    }
+    const int64_t* dRows = reinterpret_cast<const int64_t*>(d_tab);
+    const int64_t* dCols = dRows + rows.size();
+    for (size_t i = 0; i < last.size(); i++) {
+        last[i].grp = d_group;
+        last[i].grpStrideB = gTot;
+        last[i].grpRow = dRows + rowOff[i];
+        last[i].grpCol = dCols + colOff[i];
+    }
+    e = upload_stages(B, s);
+    if (e != hipSuccess) {
+        (void)hipFree(d_tab);
+        return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+    }
+    if (B->d_fuse) (void)hipFree(B->d_fuse);
+    B->d_fuse = d_tab;
+    P->sep.skipSingle = 1;
+    P->fusedGroup = d_group;
+    return RF_OK;
+}
+
 int rf_grid_batch_destroy(rf_grid_batch* B) {
     if (!B) return RF_OK;
     (void)hipSetDevice(B->ctx->device);
     (void)hipFree(B->d_mem);
+    if (B->d_fuse) (void)hipFree(B->d_fuse);
     delete B;
     return RF_OK;
 }

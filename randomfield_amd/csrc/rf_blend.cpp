@@ -53,15 +53,6 @@ void axis_factors(int N, int n, bool lower, bool upper, int mode, double* low, d
 
 }  // namespace
 
-struct rf_blend_plan {
-    rf_context* ctx = nullptr;
-    char* d_blob = nullptr;
-    rf_blend_dev b{};
-    bool separable = false;
-    rf_blend_sep sep{};
-    int dtype = RF_DTYPE_F64;
-};
-
 extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t* groupN, int32_t Nmc,
                                     int32_t nSub, const int32_t* origins, const int32_t* sizes,
                                     const int32_t* nOvl, const int32_t* faceMasks, int32_t mode,
@@ -282,6 +273,15 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
     if (!P) return rf_set_error(ctx, RF_E_NOMEM, "blend plan allocation failed");
     P->ctx = ctx;
     P->dtype = dtype;
+    P->nDim = nDim;
+    P->mode = mode;
+    for (int a = 0; a < nDim; a++) {
+        P->groupN[a] = groupN[a];
+        P->cover[a] = cover[a];
+    }
+    P->origins.assign(origins, origins + (size_t)nSub * nDim);
+    P->sizes.assign(sizes, sizes + (size_t)nSub * nDim);
+    P->fields.assign(d_fields, d_fields + nSub);
     RF_HIP_TRY(ctx, hipMalloc((void**)&P->d_blob, blob.size()));
     const hipError_t e = hipMemcpy(P->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -330,6 +330,8 @@ extern "C" int rf_blend_plan_execute(rf_blend_plan* P, void* d_group, void* stre
     RF_HIP_TRY(ctx, hipGetDevice(&dev));
     if (dev != ctx->device) RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (P->fusedGroup && d_group != P->fusedGroup)
+        return rf_set_error(ctx, RF_E_ARG, "this blend is fused into a grid batch writing another group buffer");
     if (P->separable) RF_HIP_TRY(ctx, rf_launch_blend_sep(P->sep, d_group, P->dtype, s));
     else RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
     return RF_OK;

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 struct rf_blend_entry {
     const void* field;            // subdomain field, [Nmc][subTot]
@@ -62,8 +63,28 @@ struct rf_blend_sep {
     int64_t groupTot;
     const rf_blend_axis* ax[3];   // [groupN[a]] (one unit entry for a >= nDim)
     const rf_blend_sub* sub;      // [i0 + nInt0 * (i1 + nInt1 * i2)]
+    int skipSingle;               // 1: points with one contribution are left
+                                  // alone (the fused last contraction wrote them)
 };
 hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s);
+
+struct rf_context;
+// A blend prepared once (rf_blend_plan_create): device tables plus the host
+// description a grid batch needs to fuse it into its last contraction.
+struct rf_blend_plan {
+    rf_context* ctx = nullptr;
+    char* d_blob = nullptr;
+    rf_blend_dev b{};
+    bool separable = false;
+    rf_blend_sep sep{};
+    int dtype = 0;
+    int nDim = 0, mode = 0;
+    int groupN[3] = {1, 1, 1};
+    std::vector<int32_t> origins, sizes;      // nSub x nDim
+    std::vector<const void*> fields;          // nSub
+    std::vector<int32_t> cover[3];            // [g][2]: intervals covering group index g
+    void* fusedGroup = nullptr;               // the group a fused batch writes
+};
 
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s);
 int rf_moments_parts(int64_t N);

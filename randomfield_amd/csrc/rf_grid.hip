@@ -11,12 +11,17 @@
 //
 // three small batched complex GEMMs instead of N * N_k cosines
 // (O(n N_k + n^2 N_k^(2/3) + n^3 N_k^(1/3)) per realization and block).
-// One tiled FP32 MFMA kernel does every stage: D[m][n] = sum_k A[m][k] B[k][n]
-// with per-stage strides; A may come in per-block variants (conjugated
-// tables of the negative-sign blocks).  The phase tables are formed in FP64
-// and rounded to FP32; products and accumulation are FP32 over
-// <= 4*N_k^(1/3) terms.
+// Every stage is one tiled complex GEMM on the matrix cores,
+// D[m][n] = sum_k A[m][k] B[k][n] with per-stage strides: operands split
+// into FP16 hi + lo halves (power-of-two scaled into range), three
+// v_mfma_f32_32x32x16_f16 products with FP32 accumulation, on 64x64 tiles or
+// the wide kinds along each stage's long axis; an FP32-operand MFMA tile
+// (v_mfma_f32_32x32x2_f32) serves shapes whose K would leave the FP16 range.
+// The negative sign blocks are folded into the first contraction
+// (conjugated tables and coefficients).  The phase tables are formed in FP64
+// and rounded to FP32; accumulation is FP32 over <= 4*N_k^(1/3) terms.
 #include "rf_internal.h"
+#include "rf_philox.h"
 
 namespace {
 
@@ -347,24 +352,33 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         const int nn = ct * 32 + l32;
         const bool nIn = n0 + nn < g.N;
         const int oc = dcol(nn);
-        auto store = [&](auto map, auto* D) {
+        // fused blend: this lane's column offset in the group grid, or -1
+        const int64_t gc = (g.grp && nIn) ? g.grpCol[n0 + nn] : -1;
+        auto store = [&](auto map, auto* D, auto* G) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
-                if (m0 + r < g.M && nIn) D[r * dSm + oc] = map(acc0[i] * os);
+                if (m0 + r < g.M && nIn) {
+                    const auto v = map(acc0[i] * os);
+                    const int64_t gr = gc >= 0 ? g.grpRow[m0 + r] : -1;
+                    if (gr >= 0) G[gr + gc] = v;
+                    else D[r * dSm + oc] = v;
+                }
             }
         };
         const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
         gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
         gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
+        gfo* G32 = (gfo*)(static_cast<float*>(g.grp) + (int64_t)batch * g.grpStrideB);
+        gdo* G64 = (gdo*)(static_cast<double*>(g.grp) + (int64_t)batch * g.grpStrideB);
         if (g.outDtype == RF_DTYPE_F64) {
-            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
-            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
-            else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64);
+            else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64);
         } else {
-            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
-            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
-            else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32);
+            else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32);
         }
     }
 }
@@ -600,26 +614,37 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
     const int oc[2] = {dcol(64 * ct + l32), dcol(64 * ct + 32 + l32)};
     const bool nIn[2] = {n0 + 64 * ct + l32 < g.N, n0 + 64 * ct + 32 + l32 < g.N};
-    auto store = [&](auto map, auto* D) {
+    // fused blend: the lane's column offsets in the group grid, or -1
+    const int64_t gc[2] = {(g.grp && nIn[0]) ? g.grpCol[n0 + 64 * ct + l32] : -1,
+                           (g.grp && nIn[1]) ? g.grpCol[n0 + 64 * ct + 32 + l32] : -1};
+    auto store = [&](auto map, auto* D, auto* G) {
 #pragma unroll
-        for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 16; i++) {
+            const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+            if (m0 + r >= g.M) continue;
+            const int64_t gr = (gc[0] >= 0 || gc[1] >= 0) ? g.grpRow[m0 + r] : -1;
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
-                if (m0 + r < g.M && nIn[c]) D[r * dSm + oc[c]] = map(acc[c][i] * os);
+            for (int c = 0; c < 2; c++) {
+                if (!nIn[c]) continue;
+                const auto v = map(acc[c][i] * os);
+                if (gr >= 0 && gc[c] >= 0) G[gr + gc[c]] = v;
+                else D[r * dSm + oc[c]] = v;
             }
+        }
     };
     const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
     gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
     gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
+    gfo* G32 = (gfo*)(static_cast<float*>(g.grp) + (int64_t)batch * g.grpStrideB);
+    gdo* G64 = (gdo*)(static_cast<double*>(g.grp) + (int64_t)batch * g.grpStrideB);
     if (g.outDtype == RF_DTYPE_F64) {
-        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
-        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
-        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64);
+        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64);
     } else {
-        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
-        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
-        else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32);
+        else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32);
     }
 }
 
@@ -878,11 +903,18 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // One block per 64 x 64 tile of (row m, k): the coefficients are formed in
 // the source order (k fastest: the amplitude and phase reads are contiguous
 // along j1) into LDS and written out K-major (m fastest).
+// Draw: the phase is not read but formed from the device stream, 2 pi U with
+// U = element (*draw * Nmc + r) * kNr + src of (seed, tag 2) -- the value
+// k_phase would write from those uniforms.
 constexpr int kFoldT = 64;
+template <bool Draw>
 __global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nDim, int fold, int nk1, int nk2,
                                                      int nk3, int64_t kNu, int64_t kNr,
                                                      const double* __restrict__ amp, const double* __restrict__ ph,
-                                                     double ampMult, float2* __restrict__ C) {
+                                                     double ampMult, float2* __restrict__ C, uint64_t seed,
+                                                     const int64_t* __restrict__ draw, int Nmc) {
+#pragma clang fp contract(off)
+    const int64_t drawBase = Draw ? (*draw) * (int64_t)Nmc * kNr : 0;
     __shared__ float2 tile[kFoldT][kFoldT + 1];
     const int64_t m0 = (int64_t)blockIdx.x * kFoldT;
     const int k0 = blockIdx.y * kFoldT;
@@ -904,11 +936,12 @@ __global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nD
         const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
         const int64_t src = r * kNr + b * kNu + j;
         const double a = amp[src] * ampMult;
-        if (!ph) {
+        if (!Draw && !ph) {
             tile[kk][mm] = make_float2((float)a, 0.0f);
         } else {
             double sn, cs;
-            sincos(ph[src], &sn, &cs);
+            const double phase = Draw ? 2.0 * RF_PI_REF * rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : ph[src];
+            sincos(phase, &sn, &cs);
             tile[kk][mm] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
         }
     }
@@ -1030,13 +1063,18 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
         fresh = coef->key == ampMult * sc;
         coef->key = ampMult * sc;
     }
-    if (!fresh)
-        hipLaunchKernelGGL(k_coeffs_fold,
-                           dim3((unsigned)((nC / K1 + kFoldT - 1) / kFoldT), (unsigned)((K1 + kFoldT - 1) / kFoldT)),
-                           dim3(256), 0, s, nC / K1, K1, nDim, fold, kNStep[0], nDim > 1 ? kNStep[1] : 1,
-                           nDim > 2 ? kNStep[2] : 1, kNu, (int64_t)nb * kNu, d_amp, kind == 1 ? nullptr : d_phase,
-                           ampMult * sc, Cf);
     *st = rf_grid_stages{};
+    rf_fold_desc& fd = st->fold;
+    fd.rows = nC / K1; fd.K = K1; fd.nDim = nDim; fd.fold = fold;
+    fd.nk1 = kNStep[0]; fd.nk2 = nDim > 1 ? kNStep[1] : 1; fd.nk3 = nDim > 2 ? kNStep[2] : 1;
+    fd.kNu = kNu; fd.kNr = (int64_t)nb * kNu;
+    fd.amp = d_amp; fd.ph = kind == 1 ? nullptr : d_phase;
+    fd.ampMult = ampMult * sc; fd.C = Cf;
+    if (!fresh)
+        hipLaunchKernelGGL(k_coeffs_fold<false>,
+                           dim3((unsigned)((fd.rows + kFoldT - 1) / kFoldT), (unsigned)((K1 + kFoldT - 1) / kFoldT)),
+                           dim3(256), 0, s, fd.rows, fd.K, nDim, fold, fd.nk1, fd.nk2, fd.nk3, fd.kNu, fd.kNr,
+                           fd.amp, fd.ph, fd.ampMult, Cf, (uint64_t)0, (const int64_t*)nullptr, Nmc);
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;
     float2* V = p; p += rows1 * n[0];
@@ -1092,6 +1130,27 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
                 st->g[j].rw = 1;                  // the FP32 tile is 64 x 64
             }
     }
+    return hipGetLastError();
+}
+
+hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw,
+                               hipStream_t s) {
+    if (!f.ph || !d_draw) return hipErrorInvalidValue;      // method 4 has no phases to redraw
+    hipLaunchKernelGGL(k_coeffs_fold<true>,
+                       dim3((unsigned)((f.rows + kFoldT - 1) / kFoldT), (unsigned)((f.K + kFoldT - 1) / kFoldT)),
+                       dim3(256), 0, s, f.rows, f.K, f.nDim, f.fold, f.nk1, f.nk2, f.nk3, f.kNu, f.kNr, f.amp,
+                       f.ph, f.ampMult, f.C, seed, d_draw, Nmc);
+    return hipGetLastError();
+}
+
+namespace {
+__global__ void k_draw_counter(int64_t* d_draw, int64_t value) {
+    if (threadIdx.x == 0) *d_draw = value >= 0 ? value : *d_draw + 1;
+}
+}  // namespace
+
+hipError_t rf_launch_draw_counter(int64_t* d_draw, int64_t value, hipStream_t s) {
+    hipLaunchKernelGGL(k_draw_counter, dim3(1), dim3(64), 0, s, d_draw, value);
     return hipGetLastError();
 }
 

@@ -208,11 +208,37 @@ struct rf_gemm_desc {
     int64_t bSnHi, dSnHi;
     int rw;               // tile kind, FP16 operands only: 2 128x64 (complex), 3 64x128 (real),
                           // 4 64x128 (complex); 0, 1: 64x64
+    // blend fused into a real last stage (grp != NULL): an output point whose
+    // row and column both lie inside this field only (grpRow[m] >= 0 and
+    // grpCol[n] >= 0) is stored to grp + batch * grpStrideB + grpRow[m] +
+    // grpCol[n] -- the group grid, its blended value -- instead of to D
+    void* grp;
+    int64_t grpStrideB;
+    const int64_t* grpRow;
+    const int64_t* grpCol;
+};
+// The coefficient fold of one grid execution: Cf[k][m] for rows m (r, j3, q,
+// j2) and K index k (h, j1) from amp / phase [r][b][j] (rf_grid.hip).
+struct rf_fold_desc {
+    int64_t rows; int K, nDim, fold, nk1, nk2, nk3;
+    int64_t kNu, kNr;
+    const double* amp; const double* ph;   // ph NULL: real coefficients (method 4)
+    double ampMult;
+    float2* C;
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)
     int n;
+    rf_fold_desc fold;    // the coefficient fold this execution formed (or reused)
 };
+// Re-forms the folded coefficients of a Shinozuka grid execution from new
+// phases 2 pi U, U element (*d_draw * Nmc + r) * kNr + b kNu + j of the
+// device stream (seed, tag 2): realizations *d_draw * Nmc .. of the stream
+// the plan's own phases begin (draw index 0).  Reads the draw index on the
+// device, so a captured graph draws what the counter holds at replay.
+hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw, hipStream_t s);
+// *d_draw = value (value >= 0) or *d_draw += 1 (value < 0), one thread
+hipError_t rf_launch_draw_counter(int64_t* d_draw, int64_t value, hipStream_t s);
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
 // kind 0: Shinozuka lattice, tables exp(i k_j x); kind 1: method 4, tables of
 // the DCT-III (FFTW REDFT01) of each axis, w_j cos(pi j (2i+1) / (2n))

@@ -446,16 +446,52 @@ hipError_t launch_variant(int variant, int mt, const rf_sum_args& a, int64_t nSt
     }
 }
 
-template <typename V>
-hipError_t launch_form(int variant, const rf_sum_args& a, int64_t kNpad, int Npad, const void* B,
+}  // namespace
+
+// The six (element type, dimension) instantiations of the point-wise kernel
+// are separate translation units of this file (RF_MFMA_PART 1..6; part 0 holds
+// everything else), so the build compiles them in parallel.
+#ifndef RF_MFMA_PART
+#error "rf_mfma.hip is compiled once per RF_MFMA_PART (0..6)"
+#endif
+#define RF_MFMA_LAUNCH(NAME)                                                                      \
+    hipError_t NAME(int variant, int mt, const rf_sum_args& a, int64_t nSteps, int mtTotal,       \
+                    const void* B, float outScale, hipStream_t s)
+RF_MFMA_LAUNCH(rf_mfma_launch_f16_1);
+RF_MFMA_LAUNCH(rf_mfma_launch_f16_2);
+RF_MFMA_LAUNCH(rf_mfma_launch_f16_3);
+RF_MFMA_LAUNCH(rf_mfma_launch_bf16_1);
+RF_MFMA_LAUNCH(rf_mfma_launch_bf16_2);
+RF_MFMA_LAUNCH(rf_mfma_launch_bf16_3);
+#define RF_MFMA_DEFINE(PART, NAME, V, D) \
+    RF_MFMA_LAUNCH(NAME) { return launch_variant<V, D>(variant, mt, a, nSteps, mtTotal, B, outScale, s); }
+#if RF_MFMA_PART == 1
+RF_MFMA_DEFINE(1, rf_mfma_launch_f16_1, h8, 1)
+#elif RF_MFMA_PART == 2
+RF_MFMA_DEFINE(2, rf_mfma_launch_f16_2, h8, 2)
+#elif RF_MFMA_PART == 3
+RF_MFMA_DEFINE(3, rf_mfma_launch_f16_3, h8, 3)
+#elif RF_MFMA_PART == 4
+RF_MFMA_DEFINE(4, rf_mfma_launch_bf16_1, b8, 1)
+#elif RF_MFMA_PART == 5
+RF_MFMA_DEFINE(5, rf_mfma_launch_bf16_2, b8, 2)
+#elif RF_MFMA_PART == 6
+RF_MFMA_DEFINE(6, rf_mfma_launch_bf16_3, b8, 3)
+#endif
+
+#if RF_MFMA_PART == 0
+namespace {
+
+hipError_t launch_form(bool bf16, int variant, const rf_sum_args& a, int64_t kNpad, int Npad, const void* B,
                        double outScale, hipStream_t s) {
     const int mtTotal = Npad / 32;
     const int mt = mtTotal >= kMaxMT ? kMaxMT : mtTotal;
     const int64_t nSteps = kNpad / kStepK;
+    const float sc = (float)outScale;
     switch (a.nDim) {
-        case 1: return launch_variant<V, 1>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
-        case 2: return launch_variant<V, 2>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
-        default: return launch_variant<V, 3>(variant, mt, a, nSteps, mtTotal, B, (float)outScale, s);
+        case 1: return (bf16 ? rf_mfma_launch_bf16_1 : rf_mfma_launch_f16_1)(variant, mt, a, nSteps, mtTotal, B, sc, s);
+        case 2: return (bf16 ? rf_mfma_launch_bf16_2 : rf_mfma_launch_f16_2)(variant, mt, a, nSteps, mtTotal, B, sc, s);
+        default: return (bf16 ? rf_mfma_launch_bf16_3 : rf_mfma_launch_f16_3)(variant, mt, a, nSteps, mtTotal, B, sc, s);
     }
 }
 
@@ -513,6 +549,6 @@ hipError_t rf_launch_sum_mfma(int form, int variant, const rf_sum_args& a, int64
                               const void* d_B, double outScale, hipStream_t s) {
     if (a.i1 <= a.i0) return hipSuccess;
     if (kNpad % kKPad != 0 || variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
-    if (form == RF_FORM_MFMA_BF16) return launch_form<b8>(variant, a, kNpad, Npad, d_B, outScale, s);
-    return launch_form<h8>(variant, a, kNpad, Npad, d_B, outScale, s);
+    return launch_form(form == RF_FORM_MFMA_BF16, variant, a, kNpad, Npad, d_B, outScale, s);
 }
+#endif  // RF_MFMA_PART == 0

@@ -436,6 +436,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
         else { F1 = F; st1 = S.subTot; w1 = w; }
         n++;
     }
+    if (b.skipSingle && n == 1) return;            // written by the fused last contraction
     if (!__any(n > 2)) {
         if (!__any(n > 1)) {
             constexpr int RB = 16;

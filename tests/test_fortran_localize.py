@@ -104,8 +104,8 @@ def test_localization_pipeline(tmp_path, mode):
     xs = d["xStep"]
     # every field: the oracle on the field's grid with the draws of seed
     # seedStart + field number (the Fortran random_number stream)
+    # (x, y, z) local indices, x fastest
     idx = np.stack(np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")[::-1], -1).reshape(-1, 3)
-    idx = idx[:, ::-1]       # x fastest
     po = O.make_plan(3, O.SHINOZUKA, omode, list(d["procExtent"]))
     k = O.kpoints(po)
     sk = O.skvec(po, k)

@@ -267,3 +267,156 @@ def test_repeated_grid_calls_reuse_coefficients(ctx, method):
         ctx.sync()
         fresh.close()
         assert np.array_equal(o.cpu().numpy(), got)
+
+
+def test_grid_batch_redraw(ctx):
+    """Fresh realizations in a batch: redraw(0) gives the plans' own fields
+    bit for bit; redraw(2) gives realizations 2 Nmc .. 3 Nmc - 1 of each
+    plan's device stream (checked against the oracle on those draws); a graph
+    captured with the draw counter draws the next set on every replay, equal
+    to the eager redraw of that index."""
+    import torch
+    specs = SPECS[:2] + [((12, 10, 9), (0.3, 0.3, 0.3), (1.0, 0.0, -1.0), (1.0, 1.0, 1.0), 4, 21)]
+    plans, outs, geo = _grid_fields(ctx, specs, torch.float32)
+    b = R.GridBatch(ctx, plans, [g[0] for g in geo], [g[1] for g in geo], [g[2] for g in geo], outs, R.DTYPE_F32)
+    b.execute()
+    ctx.sync()
+    own = [o.cpu().numpy().copy() for o in outs]
+    b.redraw(0)
+    b.execute()
+    ctx.sync()
+    for i, o in enumerate(outs):
+        assert np.array_equal(o.cpu().numpy(), own[i]), i
+    eager = {}
+    for d in (1, 2, 3):
+        b.redraw(d)
+        b.execute()
+        ctx.sync()
+        eager[d] = [o.cpu().numpy().copy() for o in outs]
+    assert not np.array_equal(eager[2][0], own[0])
+    # draw 2 of field 2 against the oracle on the same stream elements
+    (nstep, step, origin, corrL, Nmc, seed) = specs[2]
+    ext = [(nstep[a] - 1) * step[a] for a in range(3)]
+    kN = plans[2].info.kNTotal
+    du = torch.empty(3 * Nmc * kN, dtype=torch.float64, device="cuda")
+    R.device_uniforms(ctx, seed, 2, 3 * Nmc * kN, du)
+    ctx.sync()
+    phU = du.cpu().numpy()[2 * Nmc * kN:].reshape(Nmc, kN)
+    po = O.make_plan(3, O.SHINOZUKA, O.FIXED, [ext[a] / corrL[a] for a in range(3)])
+    k = O.kpoints(po)
+    x = W.grid_points(nstep, step, origin)
+    ref = O.shinozuka(po, x / np.array(corrL), k, O.skvec(po, k), phU)
+    ref = O.marginal(1, 0.5, 2.0, ref)          # the specs' gaussian marginal (avg 0.5, var 2)
+    e = rel_rms(eager[2][2].astype(np.float64), ref)
+    print(f"[parity] batch redraw 2: rel_rms={e:.3e}", flush=True)
+    assert e <= 1e-5
+    # graph: counter-driven draws
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    b.redraw(1, sp)                      # sets the counter to 1 (and draws set 1)
+    torch.cuda.synchronize()
+    graph = R.Graph(ctx, sp)
+    with graph:
+        b.redraw(R.DRAW_NEXT, sp)
+        b.execute(sp)
+    # capturing launched nothing: the counter is still 1
+    for rep, d in enumerate((1, 2, 3)):
+        for o in outs:
+            o.fill_(float(rep + 5))
+        torch.cuda.synchronize()
+        graph.launch(sp)
+        torch.cuda.synchronize()
+        for i, o in enumerate(outs):
+            assert np.array_equal(o.cpu().numpy(), eager[d][i]), (rep, i)
+    graph.close()
+    b.close()
+    for pl in plans:
+        pl.close()
+
+
+def test_grid_batch_autotune_and_fused_blend(ctx):
+    """The tile autotuner and the blend fused into the last contraction leave
+    every group value and every overlap value of the fields unchanged, bit for
+    bit; a fused graph with fresh draws matches the eager fused result; the
+    blend plan refuses another group buffer while fused; unfusing restores
+    the separate blend."""
+    import torch
+    groupN, parts, nOvl, step, Nmc = (40, 36, 20), (3, 2, 2), 5, (0.25, 0.25, 0.25), 3
+    origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
+    specs = []
+    for s in range(len(sizes)):
+        n = tuple(int(v) for v in sizes[s])
+        specs.append((n, step, tuple(origins[s][a] * step[a] for a in range(3)), (1.0, 1.0, 1.0), Nmc, 200 + s))
+    plans, outs, geo = _grid_fields(ctx, specs, torch.float32)
+    batch = R.GridBatch(ctx, plans, [x[0] for x in geo], [x[1] for x in geo], [x[2] for x in geo], outs, R.DTYPE_F32)
+    bp = R.BlendPlan(ctx, groupN, Nmc, origins, sizes, [nOvl] * 3, masks, R.FIXED, outs, R.DTYPE_F32)
+    g = torch.empty((Nmc, int(np.prod(groupN))), dtype=torch.float32, device="cuda")
+    batch.execute()
+    bp.execute(g)
+    ctx.sync()
+    ref_g = g.cpu().numpy().copy()
+    ref_f = [o.cpu().numpy().copy() for o in outs]
+    chosen = batch.autotune()
+    print(f"[grid] autotune chose tile kinds {chosen}", flush=True)
+    assert len(chosen) == 3 and all(c in (1, 2, 3, 4) for c in chosen)
+    g.fill_(-7.0)
+    batch.execute()
+    bp.execute(g)
+    ctx.sync()
+    assert np.array_equal(g.cpu().numpy(), ref_g)
+    # fused: interior points never reach the field buffers
+    g2 = torch.full_like(g, 1234.5)
+    for o in outs:
+        o.fill_(9876.5)
+    batch.fuse_blend(bp, g2)
+    batch.execute()
+    bp.execute(g2)
+    ctx.sync()
+    got = g2.cpu().numpy()
+    print(f"[grid] fused blend: max |diff| {np.max(np.abs(got - ref_g)):.3e}", flush=True)
+    assert np.array_equal(got, ref_g)
+    n_interior = 0
+    for i, o in enumerate(outs):
+        f = o.cpu().numpy()
+        written = f != np.float32(9876.5)
+        assert np.array_equal(f[written], ref_f[i][written])
+        n_interior += int((~written).sum())
+    assert n_interior > 0
+    with pytest.raises(R.RFError):
+        bp.execute(g)
+    # fused graph with fresh draws against eager fused redraws
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    eager = {}
+    for d in (1, 2):
+        batch.redraw(d, sp)
+        batch.execute(sp)
+        bp.execute(g2, sp)
+        torch.cuda.synchronize()
+        eager[d] = g2.cpu().numpy().copy()
+    batch.redraw(1, sp)
+    torch.cuda.synchronize()
+    graph = R.Graph(ctx, sp)
+    with graph:
+        batch.redraw(R.DRAW_NEXT, sp)
+        batch.execute(sp)
+        bp.execute(g2, sp)
+    for d in (1, 2):
+        g2.fill_(float(d))
+        torch.cuda.synchronize()
+        graph.launch(sp)
+        torch.cuda.synchronize()
+        assert np.array_equal(g2.cpu().numpy(), eager[d]), d
+    graph.close()
+    # unfused again: the separate blend on any group buffer
+    batch.fuse_blend(bp, None)
+    batch.redraw(0)
+    batch.execute()
+    g.fill_(3.0)
+    bp.execute(g)
+    ctx.sync()
+    assert np.array_equal(g.cpu().numpy(), ref_g)
+    bp.close()
+    batch.close()
+    for pl in plans:
+        pl.close()

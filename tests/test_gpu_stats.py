@@ -33,8 +33,8 @@ def test_corr_device_matches_oracle(ctx, dtype, nstep):
 def test_generated_field_moments_and_correlation(ctx):
     """Shinozuka fields (FIXED mode, grid form) on 128 x 128 x 64 points at
     spacing 0.1 L against the statistics of the discrete spectral sum they
-    realise (the lattice, not the continuum: its left Riemann sums put the
-    variance at ampMult^2/2 sum S, about 1.3 here): mean within 4 standard
+    realise (the lattice, not the continuum: the variance is ampMult^2/2 sum S
+    with the FIXED trapezoid weights on the k = 0 planes, close to 1): mean within 4 standard
     errors of the box average, variance within 8 %, lag correlations of every
     axis within 0.06 of sum S cos(k tau) / sum S (times (n - m)/n, the bias
     of the reference's circular estimator), and the reference's corrL

@@ -424,6 +424,20 @@ int rf_normalize_window_device(rf_context* ctx, void* d_field, int32_t nDim, con
 int rf_normalize_window_host(rf_context* ctx, double* field, int32_t nDim, const int32_t* gridN,
                              const int32_t* minPos, const int32_t* maxPos, int32_t Nmc);
 
+/* rf_normalize_device without allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that must not call hipMalloc).  Same arguments and
+ * result as rf_normalize_device (same kernels and sum partition), but the
+ * reduction partials, sums and coefficients live in a static device array of
+ * the library (about 4.2 MB per device, held until the process ends), the
+ * coefficients are formed on the device, and the call is stream-ordered and
+ * asynchronous.  Nmc <= RF_NOALLOC_MAX_NMC, else RF_E_ARG.  Every call on a
+ * device shares that device's array: calls that overlap on one device (other
+ * streams or host threads) race. */
+#define RF_NOALLOC_MAX_NMC 1024
+int rf_normalize_device_noalloc(rf_context* ctx, void* d_field, int64_t N, int32_t Nmc,
+                                int32_t dtype, void* stream);
+
 /* Device generator: count uniforms U[0,1) of stream (seed, tag) into d_out
  * (FP64), element e = Philox4x32-10(counter=(e lo, e hi, tag, 0),
  * key=(seed lo, seed hi)), 53-bit mantissa from words 0..1. */

@@ -38,7 +38,7 @@ EXPORTS = (
     "rf_grid_batch_autotune", "rf_grid_batch_fuse_blend",
     "rf_graph_capture_begin", "rf_graph_capture_end", "rf_graph_launch", "rf_graph_destroy",
     "rf_localization_layout", "rf_unity_partition", "rf_blend_host", "rf_normalize_window_device",
-    "rf_normalize_window_host",
+    "rf_normalize_window_host", "rf_normalize_device_noalloc",
 )
 
 
@@ -204,6 +204,7 @@ def lib() -> C.CDLL:
     _sig(L, "rf_blend_host", [vp, i32, vp, i32, i32, vp, vp, vp, vp, i32, vp, vp])
     _sig(L, "rf_normalize_window_device", [vp, vp, i32, vp, vp, vp, i32, i32, vp])
     _sig(L, "rf_normalize_window_host", [vp, vp, i32, vp, vp, vp, i32])
+    _sig(L, "rf_normalize_device_noalloc", [vp, vp, i64, i32, i32, vp])
     _sig(L, "rf_server_create", [vp, C.POINTER(ServerConfig), C.POINTER(vp)])
     _sig(L, "rf_server_destroy", [vp])
     _sig(L, "rf_server_submit", [vp, C.POINTER(Request), C.POINTER(i64)])

@@ -99,6 +99,9 @@ int rf_context_create(rf_context** out, int device) {
         return rf_set_error(nullptr, RF_E_NODEVICE,
                             std::string("librf_hip.so is built for gfx950 only, device is ") + prop.gcnArchName);
     if (hipSetDevice(device) != hipSuccess) return rf_set_error(nullptr, RF_E_HIP, "hipSetDevice failed");
+    const hipError_t se = rf_noalloc_setup(device);
+    if (se != hipSuccess)
+        return rf_set_error(nullptr, RF_E_HIP, std::string("static scratch lookup failed: ") + hipGetErrorString(se));
     auto* c = new (std::nothrow) rf_context();
     if (!c) return rf_set_error(nullptr, RF_E_NOMEM, "context allocation failed");
     c->device = device;

@@ -268,6 +268,9 @@ hipError_t rf_launch_marginal(int margi, double scale, double shift, void* d, in
 
 // utilities
 int rf_set_error(rf_context* ctx, int code, const std::string& msg);
+// device address of the allocation-free entry's static scratch, looked up
+// once per device with that device current (rf_noalloc.hip)
+hipError_t rf_noalloc_setup(int device);
 #define RF_HIP_TRY(ctx, expr)                                                        \
     do {                                                                             \
         hipError_t _e = (expr);                                                      \

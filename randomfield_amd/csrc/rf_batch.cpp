@@ -39,7 +39,6 @@ struct rf_grid_batch {
     std::vector<rf_gemm_desc> stage[3];
     std::vector<int> kind0[3];
     int outDtype = RF_DTYPE_F32;
-    char* d_fuse = nullptr;             // fused blend: row / column tables of the last stage
 };
 
 struct rf_graph {
@@ -146,7 +145,8 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         }
         rf_grid_stages gs;
         const hipError_t e = rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, o, st,
-                                             ns, corrL, P->d_amp, P->d_phase, ampMult, ampMult * P->ampMax, P->epi,
+                                             ns, corrL, P->d_amp, P->nCalc == p.Nmc && !fft, P->d_uni, ampMult,
+                                             ampMult * P->ampMax, P->epi,
                                              d_outs[i], outDtype,
                                              reinterpret_cast<float2*>(B->d_mem + off[(size_t)i]), s, &gs);
         if (e != hipSuccess) {
@@ -269,27 +269,42 @@ int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) 
     const hipStream_t s = ctx->stream;
     RF_HIP_TRY(ctx, hipStreamSynchronize(s));          // no launch may read the tables being replaced
     std::vector<rf_gemm_desc>& last = B->stage[2];
+    auto drop_boxes = [&]() {
+        if (P->d_boxes) (void)hipFree(P->d_boxes);
+        P->d_boxes = nullptr;
+        P->nBoxes = 0;
+        P->boxPoints = 0;
+    };
     if (!d_group) {                                    // back to the separate blend
-        for (auto& g : last) { g.grp = nullptr; g.grpRow = nullptr; g.grpCol = nullptr; g.grpStrideB = 0; }
+        for (auto& g : last) g.grp = nullptr;
         P->sep.skipSingle = 0;
         P->fusedGroup = nullptr;
+        drop_boxes();
         RF_HIP_TRY(ctx, upload_stages(B, s));
-        if (B->d_fuse) (void)hipFree(B->d_fuse);
-        B->d_fuse = nullptr;
         return RF_OK;
     }
     if (!P->separable || P->mode != RF_FIXED)
         return rf_set_error(ctx, RF_E_UNSUPPORTED, "the fused blend needs a FIXED-mode tensor-product layout");
     if (P->dtype != B->outDtype) return rf_set_error(ctx, RF_E_ARG, "blend and batch element types differ");
     const int d = P->nDim;
+    if (d < 2) return rf_set_error(ctx, RF_E_UNSUPPORTED, "the fused blend needs a 2-D or 3-D layout");
     int64_t gTot = 1;
     for (int a = 0; a < d; a++) gTot *= P->groupN[a];
-    // per field: its subdomain in the blend, row table (last axis) and
-    // column table (the plane of the other axes); -1 where another field
-    // also covers the index along that axis
-    std::vector<int64_t> rows, cols;
-    std::vector<size_t> rowOff, colOff;
-    for (const auto& g : last) {
+    auto single = [&](int a, int gi) { return P->cover[a][(size_t)gi * 2 + 1] < 0; };
+    // the local index range along axis a of subdomain box [o, o + n) that no
+    // other subdomain covers (one run: the overlaps sit at the box's ends)
+    auto run = [&](int a, int o, int n, int* lo, int* hi) -> bool {
+        int l = 0, h = n;
+        while (l < n && !single(a, o + l)) l++;
+        while (h > l && !single(a, o + h - 1)) h--;
+        for (int i = l; i < h; i++)
+            if (!single(a, o + i)) return false;
+        *lo = l;
+        *hi = h;
+        return true;
+    };
+    std::vector<rf_gemm_desc> fused = last;
+    for (auto& g : fused) {
         if (!g.split16) return rf_set_error(ctx, RF_E_UNSUPPORTED, "fused blend: FP32-operand last stage");
         int s0 = -1;
         for (size_t k = 0; k < P->fields.size(); k++)
@@ -297,50 +312,64 @@ int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) 
         if (s0 < 0) return rf_set_error(ctx, RF_E_ARG, "a batch output is not a field of the blend plan");
         const int32_t* o = &P->origins[(size_t)s0 * d];
         const int32_t* n = &P->sizes[(size_t)s0 * d];
-        int64_t plane = 1;
-        for (int a = 0; a + 1 < d; a++) plane *= n[a];
+        const int64_t plane = d == 3 ? (int64_t)n[0] * n[1] : n[0];
         if (g.M != n[d - 1] || g.N != plane) return rf_set_error(ctx, RF_E_ARG, "field and blend box sizes differ");
-        auto single = [&](int a, int gi) { return P->cover[a][(size_t)gi * 2 + 1] < 0; };
-        int64_t planeG = 1;
-        for (int a = 0; a + 1 < d; a++) planeG *= P->groupN[a];
-        rowOff.push_back(rows.size());
-        for (int z = 0; z < n[d - 1]; z++)
-            rows.push_back(single(d - 1, o[d - 1] + z) ? (int64_t)(o[d - 1] + z) * planeG : -1);
-        colOff.push_back(cols.size());
-        for (int64_t p = 0; p < plane; p++) {
-            const int x = (int)(p % n[0]);
-            const int y = d == 3 ? (int)(p / n[0]) : 0;
-            bool in = single(0, o[0] + x);
-            if (d == 3) in = in && single(1, o[1] + y);
-            cols.push_back(in ? (int64_t)(o[0] + x) + (d == 3 ? (int64_t)P->groupN[0] * (o[1] + y) : 0) : -1);
+        int xl, xh, yl = 0, yh = 1, zl, zh;
+        bool ok = run(0, o[0], n[0], &xl, &xh) && run(d - 1, o[d - 1], n[d - 1], &zl, &zh);
+        if (d == 3) ok = ok && run(1, o[1], n[1], &yl, &yh);
+        if (!ok) return rf_set_error(ctx, RF_E_UNSUPPORTED, "fused blend: a subdomain's own points are not one box");
+        const int64_t planeG = d == 3 ? (int64_t)P->groupN[0] * P->groupN[1] : P->groupN[0];
+        g.grp = d_group;
+        g.grpStrideB = gTot;
+        g.grpNx = n[0];
+        g.grpXlo = xl; g.grpXhi = xh; g.grpYlo = yl; g.grpYhi = yh; g.grpZlo = zl; g.grpZhi = zh;
+        g.grpG0 = P->groupN[0];
+        g.grpCol0 = o[0] + (d == 3 ? (int64_t)P->groupN[0] * o[1] : 0);
+        g.grpRow0 = (int64_t)o[d - 1] * planeG;
+        g.grpPlaneG = planeG;
+    }
+    // the overlap bands: per axis, each run of group indices two intervals cover
+    std::vector<rf_blend_box> boxes;
+    int64_t first = 0;
+    for (int a = 0; a < d; a++) {
+        for (int gi = 0; gi < P->groupN[a];) {
+            if (single(a, gi)) { gi++; continue; }
+            int ge = gi;
+            while (ge < P->groupN[a] && !single(a, ge)) ge++;
+            rf_blend_box bx{};
+            for (int c = 0; c < 3; c++) {
+                bx.lo[c] = 0;
+                bx.n[c] = c < d ? P->groupN[c] : 1;
+            }
+            bx.lo[a] = gi;
+            bx.n[a] = ge - gi;
+            bx.first = first;
+            first += (int64_t)bx.n[0] * bx.n[1] * bx.n[2];
+            boxes.push_back(bx);
+            gi = ge;
         }
     }
-    char* d_tab = nullptr;
-    const size_t bytes = (rows.size() + cols.size()) * sizeof(int64_t);
-    if (hipMalloc((void**)&d_tab, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "fused blend tables");
-    hipError_t e = hipMemcpyAsync(d_tab, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_tab + rows.size() * sizeof(int64_t), cols.data(), cols.size() * sizeof(int64_t),
-                           hipMemcpyHostToDevice, s);
+    rf_blend_box* d_boxes = nullptr;
+    if (!boxes.empty()) {
+        if (hipMalloc((void**)&d_boxes, boxes.size() * sizeof(rf_blend_box)) != hipSuccess)
+            return rf_set_error(ctx, RF_E_NOMEM, "fused blend boxes");
+        const hipError_t e = hipMemcpy(d_boxes, boxes.data(), boxes.size() * sizeof(rf_blend_box),
+                                       hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d_boxes);
+            return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
+        }
+    }
+    last = fused;
+    const hipError_t e = upload_stages(B, s);
     if (e != hipSuccess) {
-        (void)hipFree(d_tab);
+        if (d_boxes) (void)hipFree(d_boxes);
         return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
     }
-    const int64_t* dRows = reinterpret_cast<const int64_t*>(d_tab);
-    const int64_t* dCols = dRows + rows.size();
-    for (size_t i = 0; i < last.size(); i++) {
-        last[i].grp = d_group;
-        last[i].grpStrideB = gTot;
-        last[i].grpRow = dRows + rowOff[i];
-        last[i].grpCol = dCols + colOff[i];
-    }
-    e = upload_stages(B, s);
-    if (e != hipSuccess) {
-        (void)hipFree(d_tab);
-        return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
-    }
-    if (B->d_fuse) (void)hipFree(B->d_fuse);
-    B->d_fuse = d_tab;
+    drop_boxes();
+    P->d_boxes = d_boxes;
+    P->nBoxes = (int)boxes.size();
+    P->boxPoints = first;
     P->sep.skipSingle = 1;
     P->fusedGroup = d_group;
     return RF_OK;
@@ -350,7 +379,6 @@ int rf_grid_batch_destroy(rf_grid_batch* B) {
     if (!B) return RF_OK;
     (void)hipSetDevice(B->ctx->device);
     (void)hipFree(B->d_mem);
-    if (B->d_fuse) (void)hipFree(B->d_fuse);
     delete B;
     return RF_OK;
 }

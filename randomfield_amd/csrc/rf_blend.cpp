@@ -332,7 +332,9 @@ extern "C" int rf_blend_plan_execute(rf_blend_plan* P, void* d_group, void* stre
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (P->fusedGroup && d_group != P->fusedGroup)
         return rf_set_error(ctx, RF_E_ARG, "this blend is fused into a grid batch writing another group buffer");
-    if (P->separable) RF_HIP_TRY(ctx, rf_launch_blend_sep(P->sep, d_group, P->dtype, s));
+    if (P->fusedGroup) RF_HIP_TRY(ctx, rf_launch_blend_boxes(P->sep, P->d_boxes, P->nBoxes, P->boxPoints, d_group,
+                                                             P->dtype, s));
+    else if (P->separable) RF_HIP_TRY(ctx, rf_launch_blend_sep(P->sep, d_group, P->dtype, s));
     else RF_HIP_TRY(ctx, rf_launch_blend_dev(P->b, d_group, P->dtype, s));
     return RF_OK;
 }
@@ -341,6 +343,7 @@ extern "C" int rf_blend_plan_destroy(rf_blend_plan* P) {
     if (!P) return RF_OK;
     (void)hipSetDevice(P->ctx->device);
     (void)hipFree(P->d_blob);          // the caller has no launch of this plan pending
+    if (P->d_boxes) (void)hipFree(P->d_boxes);
     delete P;
     return RF_OK;
 }

@@ -68,6 +68,15 @@ struct rf_blend_sep {
 };
 hipError_t rf_launch_blend_sep(const rf_blend_sep& b, void* d_group, int dtype, hipStream_t s);
 
+// A box of group indices lo .. lo + n - 1 per axis; first = points of the
+// boxes before it (the overlap bands the fused blend forms).
+struct rf_blend_box {
+    int lo[3], n[3];
+    int64_t first;
+};
+hipError_t rf_launch_blend_boxes(const rf_blend_sep& b, const rf_blend_box* d_boxes, int nBoxes, int64_t total,
+                                 void* d_group, int dtype, hipStream_t s);
+
 struct rf_context;
 // A blend prepared once (rf_blend_plan_create): device tables plus the host
 // description a grid batch needs to fuse it into its last contraction.
@@ -84,6 +93,9 @@ struct rf_blend_plan {
     std::vector<const void*> fields;          // nSub
     std::vector<int32_t> cover[3];            // [g][2]: intervals covering group index g
     void* fusedGroup = nullptr;               // the group a fused batch writes
+    rf_blend_box* d_boxes = nullptr;          // fused: the overlap bands
+    int nBoxes = 0;
+    int64_t boxPoints = 0;
 };
 
 hipError_t rf_launch_blend_dev(const rf_blend_dev& b, void* d_group, int dtype, hipStream_t s);

@@ -521,7 +521,8 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     }
     rf_grid_coef coef{P->d_gridCoef, P->gridCoefKey};
     RF_HIP_TRY(ctx, rf_grid_prepare(fft ? 1 : 0, p.nDim, p.Nmc, in.kNStep, in.kDelta, in.nSignBlocks, origin, step,
-                                    nstep, corrL, P->d_amp, P->d_phase, ampMult, ampMult * P->ampMax, P->epi, d_out,
+                                    nstep, corrL, P->d_amp, P->nCalc == p.Nmc && !fft, P->d_uni, ampMult,
+                                    ampMult * P->ampMax, P->epi, d_out,
                                     outDtype,
                                     static_cast<float2*>(P->d_gridScratch), s, &st, &coef));
     P->gridCoefKey = coef.key;

@@ -179,6 +179,17 @@ typedef double __attribute__((address_space(1))) gdo;
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
+// Fused blend: the group-grid column offset of plane column n, or -1 where
+// another field also covers it; the group row offset of row m, or -1.
+__device__ __forceinline__ int64_t grp_col(const rf_gemm_desc& g, int n) {
+    const int x = n % g.grpNx, y = n / g.grpNx;
+    return (x >= g.grpXlo && x < g.grpXhi && y >= g.grpYlo && y < g.grpYhi)
+               ? g.grpCol0 + x + g.grpG0 * (int64_t)y : (int64_t)-1;
+}
+__device__ __forceinline__ int64_t grp_row(const rf_gemm_desc& g, int m) {
+    return (m >= g.grpZlo && m < g.grpZhi) ? g.grpRow0 + g.grpPlaneG * (int64_t)m : (int64_t)-1;
+}
+
 // Tile kinds (rf_gemm_desc::rw): 1 64x64; 2 128x64 complex product (two
 // 16-row groups per wave share each B fragment read); 3 64x128 real part
 // (two 32-column tiles per wave share each A fragment read; A holds only the
@@ -353,14 +364,14 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         const bool nIn = n0 + nn < g.N;
         const int oc = dcol(nn);
         // fused blend: this lane's column offset in the group grid, or -1
-        const int64_t gc = (g.grp && nIn) ? g.grpCol[n0 + nn] : -1;
+        const int64_t gc = (g.grp && nIn) ? grp_col(g, n0 + nn) : -1;
         auto store = [&](auto map, auto* D, auto* G) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
                 if (m0 + r < g.M && nIn) {
                     const auto v = map(acc0[i] * os);
-                    const int64_t gr = gc >= 0 ? g.grpRow[m0 + r] : -1;
+                    const int64_t gr = gc >= 0 ? grp_row(g, m0 + r) : -1;
                     if (gr >= 0) G[gr + gc] = v;
                     else D[r * dSm + oc] = v;
                 }
@@ -615,14 +626,14 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     const int oc[2] = {dcol(64 * ct + l32), dcol(64 * ct + 32 + l32)};
     const bool nIn[2] = {n0 + 64 * ct + l32 < g.N, n0 + 64 * ct + 32 + l32 < g.N};
     // fused blend: the lane's column offsets in the group grid, or -1
-    const int64_t gc[2] = {(g.grp && nIn[0]) ? g.grpCol[n0 + 64 * ct + l32] : -1,
-                           (g.grp && nIn[1]) ? g.grpCol[n0 + 64 * ct + 32 + l32] : -1};
+    const int64_t gc[2] = {(g.grp && nIn[0]) ? grp_col(g, n0 + 64 * ct + l32) : -1,
+                           (g.grp && nIn[1]) ? grp_col(g, n0 + 64 * ct + 32 + l32) : -1};
     auto store = [&](auto map, auto* D, auto* G) {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
             if (m0 + r >= g.M) continue;
-            const int64_t gr = (gc[0] >= 0 || gc[1] >= 0) ? g.grpRow[m0 + r] : -1;
+            const int64_t gr = (gc[0] >= 0 || gc[1] >= 0) ? grp_row(g, m0 + r) : -1;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 if (!nIn[c]) continue;
@@ -903,15 +914,31 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // One block per 64 x 64 tile of (row m, k): the coefficients are formed in
 // the source order (k fastest: the amplitude and phase reads are contiguous
 // along j1) into LDS and written out K-major (m fastest).
-// Draw: the phase is not read but formed from the device stream, 2 pi U with
-// U = element (*draw * Nmc + r) * kNr + src of (seed, tag 2) -- the value
-// k_phase would write from those uniforms.
+// The phase is 2 pi U with U the phase uniform [r][src] (randomFieldND.f90:
+// 278-279), or, Draw, element (*draw * Nmc + r) * kNr + src of the device
+// stream (seed, tag 2) -- the uniform the plan's generator would have drawn.
+// ampStrideR 0: every realization has the amplitudes of row 0.
 constexpr int kFoldT = 64;
+
+// cos and sin of 2 pi U for U in [0, 1): FP32 sincospi of the FP32 part Uh
+// and the first-order term of the remainder U - Uh (|U - Uh| <= 2^-25), a few
+// FP32 ulps from the FP64 values -- the coefficients are FP32.
+__device__ __forceinline__ void cis_rev(double U, float* c, float* s) {
+    const float uh = (float)U;
+    const float ul = (float)(U - (double)uh);
+    float sh, ch;
+    sincospif(2.0f * uh, &sh, &ch);
+    const float d = 6.28318530717958648f * ul;
+    *s = fmaf(d, ch, sh);
+    *c = fmaf(-d, sh, ch);
+}
+
 template <bool Draw>
 __global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nDim, int fold, int nk1, int nk2,
                                                      int nk3, int64_t kNu, int64_t kNr,
-                                                     const double* __restrict__ amp, const double* __restrict__ ph,
-                                                     double ampMult, float2* __restrict__ C, uint64_t seed,
+                                                     const double* __restrict__ amp, int64_t ampStrideR,
+                                                     const double* __restrict__ uni, double ampMult,
+                                                     float2* __restrict__ C, uint64_t seed,
                                                      const int64_t* __restrict__ draw, int Nmc) {
 #pragma clang fp contract(off)
     const int64_t drawBase = Draw ? (*draw) * (int64_t)Nmc * kNr : 0;
@@ -935,14 +962,13 @@ __global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nD
         if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
         const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
         const int64_t src = r * kNr + b * kNu + j;
-        const double a = amp[src] * ampMult;
-        if (!Draw && !ph) {
+        const double a = amp[r * ampStrideR + b * kNu + j] * ampMult;
+        if (!Draw && !uni) {
             tile[kk][mm] = make_float2((float)a, 0.0f);
         } else {
-            double sn, cs;
-            const double phase = Draw ? 2.0 * RF_PI_REF * rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : ph[src];
-            sincos(phase, &sn, &cs);
-            tile[kk][mm] = make_float2((float)(a * cs), (float)(h ? -(a * sn) : a * sn));
+            float cs, sn;
+            cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : uni[src], &cs, &sn);
+            tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
         }
     }
     __syncthreads();
@@ -1017,7 +1043,7 @@ size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, c
 // x_d x (b, j_d) times (b, j_d) x plane, real part, epilogue, into d_out.
 hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
-                           const double* corrL, const double* d_amp, const double* d_phase,
+                           const double* corrL, const double* d_amp, int ampShared, const double* d_unif,
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st, rf_grid_coef* coef) {
     // power-of-two scales for the FP16 operands: coefficients into [1/2, 1),
@@ -1068,13 +1094,14 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     fd.rows = nC / K1; fd.K = K1; fd.nDim = nDim; fd.fold = fold;
     fd.nk1 = kNStep[0]; fd.nk2 = nDim > 1 ? kNStep[1] : 1; fd.nk3 = nDim > 2 ? kNStep[2] : 1;
     fd.kNu = kNu; fd.kNr = (int64_t)nb * kNu;
-    fd.amp = d_amp; fd.ph = kind == 1 ? nullptr : d_phase;
+    fd.amp = d_amp; fd.ampStrideR = ampShared ? 0 : fd.kNr;
+    fd.uni = kind == 1 ? nullptr : d_unif;
     fd.ampMult = ampMult * sc; fd.C = Cf;
     if (!fresh)
         hipLaunchKernelGGL(k_coeffs_fold<false>,
                            dim3((unsigned)((fd.rows + kFoldT - 1) / kFoldT), (unsigned)((K1 + kFoldT - 1) / kFoldT)),
                            dim3(256), 0, s, fd.rows, fd.K, nDim, fold, fd.nk1, fd.nk2, fd.nk3, fd.kNu, fd.kNr,
-                           fd.amp, fd.ph, fd.ampMult, Cf, (uint64_t)0, (const int64_t*)nullptr, Nmc);
+                           fd.amp, fd.ampStrideR, fd.uni, fd.ampMult, Cf, (uint64_t)0, (const int64_t*)nullptr, Nmc);
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;
     float2* V = p; p += rows1 * n[0];
@@ -1135,11 +1162,11 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
 
 hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw,
                                hipStream_t s) {
-    if (!f.ph || !d_draw) return hipErrorInvalidValue;      // method 4 has no phases to redraw
+    if (!f.uni || !d_draw) return hipErrorInvalidValue;     // method 4 has no phases to redraw
     hipLaunchKernelGGL(k_coeffs_fold<true>,
                        dim3((unsigned)((f.rows + kFoldT - 1) / kFoldT), (unsigned)((f.K + kFoldT - 1) / kFoldT)),
                        dim3(256), 0, s, f.rows, f.K, f.nDim, f.fold, f.nk1, f.nk2, f.nk3, f.kNu, f.kNr, f.amp,
-                       f.ph, f.ampMult, f.C, seed, d_draw, Nmc);
+                       f.ampStrideR, f.uni, f.ampMult, f.C, seed, d_draw, Nmc);
     return hipGetLastError();
 }
 

@@ -208,21 +208,23 @@ struct rf_gemm_desc {
     int64_t bSnHi, dSnHi;
     int rw;               // tile kind, FP16 operands only: 2 128x64 (complex), 3 64x128 (real),
                           // 4 64x128 (complex); 0, 1: 64x64
-    // blend fused into a real last stage (grp != NULL): an output point whose
-    // row and column both lie inside this field only (grpRow[m] >= 0 and
-    // grpCol[n] >= 0) is stored to grp + batch * grpStrideB + grpRow[m] +
-    // grpCol[n] -- the group grid, its blended value -- instead of to D
+    // blend fused into a real last stage (grp != NULL): output row m (the
+    // last axis) and column n = x + grpNx y (the plane) whose indices all lie
+    // in the part of the field no other field covers (grpZlo <= m < grpZhi,
+    // grpXlo <= x < grpXhi, grpYlo <= y < grpYhi) -- weight 1, the blended
+    // value -- is stored to the group grid at grp + batch * grpStrideB +
+    // grpRow0 + m grpPlaneG + grpCol0 + x + grpG0 y instead of to D
     void* grp;
-    int64_t grpStrideB;
-    const int64_t* grpRow;
-    const int64_t* grpCol;
+    int64_t grpStrideB, grpRow0, grpPlaneG, grpCol0, grpG0;
+    int grpNx, grpXlo, grpXhi, grpYlo, grpYhi, grpZlo, grpZhi;
 };
 // The coefficient fold of one grid execution: Cf[k][m] for rows m (r, j3, q,
 // j2) and K index k (h, j1) from amp / phase [r][b][j] (rf_grid.hip).
 struct rf_fold_desc {
     int64_t rows; int K, nDim, fold, nk1, nk2, nk3;
     int64_t kNu, kNr;
-    const double* amp; const double* ph;   // ph NULL: real coefficients (method 4)
+    const double* amp; int64_t ampStrideR;  // amplitudes [r * ampStrideR + b kNu + j]
+    const double* uni;                      // phase uniforms [r][kNr]; NULL: real coefficients (method 4)
     double ampMult;
     float2* C;
 };
@@ -234,7 +236,7 @@ struct rf_grid_stages {
 // Re-forms the folded coefficients of a Shinozuka grid execution from new
 // phases 2 pi U, U element (*d_draw * Nmc + r) * kNr + b kNu + j of the
 // device stream (seed, tag 2): realizations *d_draw * Nmc .. of the stream
-// the plan's own phases begin (draw index 0).  Reads the draw index on the
+// whose first Nmc are the plan's own device draws (draw index 0).  Reads the draw index on the
 // device, so a captured graph draws what the counter holds at replay.
 hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw, hipStream_t s);
 // *d_draw = value (value >= 0) or *d_draw += 1 (value < 0), one thread
@@ -253,7 +255,7 @@ struct rf_grid_coef {
 // be NULL (coefficients formed in scratch on every call).
 hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, const double* kDelta, int nb,
                            const double* origin, const double* step, const int32_t* n,
-                           const double* corrL, const double* d_amp, const double* d_phase,
+                           const double* corrL, const double* d_amp, int ampShared, const double* d_unif,
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st, rf_grid_coef* coef = nullptr);
 hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);

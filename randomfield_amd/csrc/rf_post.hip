@@ -383,16 +383,11 @@ __device__ __forceinline__ void blend_store(T* p, double u) {
     __builtin_nontemporal_store((T)u, p);   // written once, read by the next kernel
 }
 
-// 7 waves per SIMD (at most 72 VGPRs, no spills; unconstrained the
-// allocation is 81 VGPRs, 5 waves)
+// One group point (gi0, gi1, gi2) of the separable blend, every realization.
 template <typename T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
+__device__ __forceinline__ void blend_sep_point(const rf_blend_sep& b, int gi0, int gi1, int gi2,
+                                                T* __restrict__ group) {
 #pragma clang fp contract(off)
-    // block (x segment, gi1, gi2): the axis-1 and axis-2 entries are
-    // block-uniform (scalar loads), one axis-0 entry per thread
-    const int gi0 = blockIdx.x * kBlock + threadIdx.x;
-    const int gi1 = blockIdx.y, gi2 = blockIdx.z;
-    if (gi0 >= b.groupN[0]) return;
     const int64_t g = gi0 + (int64_t)b.groupN[0] * (gi1 + (int64_t)b.groupN[1] * gi2);
     const rf_blend_axis A0 = b.ax[0][gi0];
     const rf_blend_axis A1 = b.ax[1][gi1];
@@ -509,6 +504,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
                 blend_store(group + (int64_t)(r0 + j) * b.groupTot + g, u);
             }
     }
+}
+
+// 7 waves per SIMD (at most 72 VGPRs, no spills; unconstrained the
+// allocation is 81 VGPRs, 5 waves)
+// block (x segment, gi1, gi2): the axis-1 and axis-2 entries are
+// block-uniform (scalar loads), one axis-0 entry per thread
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_blend_sep(rf_blend_sep b, T* __restrict__ group) {
+    const int gi0 = blockIdx.x * kBlock + threadIdx.x;
+    if (gi0 >= b.groupN[0]) return;
+    blend_sep_point<T>(b, gi0, blockIdx.y, blockIdx.z, group);
+}
+
+// The overlap bands only (the fused blend: the last contraction stored the
+// points one subdomain covers): boxes of group indices, box k covering
+// points first .. first + n0 n1 n2 - 1 of the launch, x fastest.
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_blend_boxes(
+        rf_blend_sep b, const rf_blend_box* __restrict__ boxes, int nBoxes, int64_t total, T* __restrict__ group) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    int k = 0;
+    while (k + 1 < nBoxes && boxes[k + 1].first <= e) k++;
+    const rf_blend_box& x = boxes[k];
+    const int64_t q = e - x.first;
+    const int i0 = (int)(q % x.n[0]);
+    const int64_t r = q / x.n[0];
+    blend_sep_point<T>(b, x.lo[0] + i0, x.lo[1] + (int)(r % x.n[1]), x.lo[2] + (int)(r / x.n[1]), group);
 }
 
 }  // namespace
@@ -654,6 +677,19 @@ hipError_t rf_launch_interp(int nDim, const int32_t* nstep, const int32_t* lo, c
         hipLaunchKernelGGL((k_interp<float, double>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<double*>(out), d_flag);
     else
         hipLaunchKernelGGL((k_interp<float, float>), grid2, dim3(kBlock), 0, s, q, static_cast<const float*>(grid), x, static_cast<float*>(out), d_flag);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_blend_boxes(const rf_blend_sep& b, const rf_blend_box* d_boxes, int nBoxes, int64_t total,
+                                 void* d_group, int dtype, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((total + kBlock - 1) / kBlock);
+    if (dtype == RF_DTYPE_F64)
+        hipLaunchKernelGGL(k_blend_boxes<double>, dim3(grid), dim3(kBlock), 0, s, b, d_boxes, nBoxes, total,
+                           static_cast<double*>(d_group));
+    else
+        hipLaunchKernelGGL(k_blend_boxes<float>, dim3(grid), dim3(kBlock), 0, s, b, d_boxes, nBoxes, total,
+                           static_cast<float*>(d_group));
     return hipGetLastError();
 }
 

@@ -186,6 +186,18 @@ int rf_plan_autotune(rf_field_plan* plan, const double* d_x, int64_t N,
 int rf_plan_execute_grid(rf_field_plan* plan, const double origin[3],
                          const double step[3], const int32_t nstep[3],
                          void* d_out, int32_t outDtype, void* stream);
+/* Fresh realizations for a plan (createRandomField draws new phases on each
+ * call, randomFieldND.f90:276-279 and :683-689 for method 4): the phase
+ * uniforms become realizations d*Nmc .. d*Nmc + Nmc - 1 of the device
+ * stream (rf_params.seed, tag 2), d = drawIndex >= 0 (0 repeats the draws of
+ * a plan created without caller draws; a plan created with them switches to
+ * the stream), and everything built from them is re-formed: phases (methods
+ * 2/3), directions, amplitudes and phases (method 1), DCT coefficients
+ * (method 4), the form's operand tables, the grid form's folded coefficients.
+ * Method 3's wavevector draws (tag 1) stay.  Enqueued on stream (NULL: the
+ * context's); allocates nothing; later executes on another stream must be
+ * ordered after it by the caller.  No separate entry in the reference. */
+int rf_plan_redraw(rf_field_plan* plan, int64_t drawIndex, void* stream);
 /* Host-buffer form of rf_plan_execute_grid (the Fortran facade's structured
  * path; method 4 runs here, as gen_Std_Gauss_FFT does on the mesh's grid,
  * randomFieldND.f90:520-763): plan, grid of nstep points from origin at step

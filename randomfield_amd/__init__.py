@@ -209,6 +209,13 @@ class FieldPlan:
         check(lib().rf_plan_execute_grid(self._h, _ptr(o), _ptr(st), _ptr(ns), _ptr(out),
                                          _dtype_code(out.dtype), _ptr(stream)), self.ctx.handle)
 
+    def redraw(self, draw: int, stream=None) -> None:
+        """Fresh realizations draw*Nmc .. draw*Nmc + Nmc - 1 of the plan's
+        device phase stream (draw >= 0; 0 repeats a plan made without caller
+        draws); phases, directions, DCT coefficients and operand tables are
+        re-formed on the device."""
+        check(lib().rf_plan_redraw(self._h, int(draw), _ptr(stream)), self.ctx.handle)
+
     def close(self) -> None:
         if self._h:
             lib().rf_plan_destroy(self._h)

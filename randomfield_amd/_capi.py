@@ -27,7 +27,7 @@ DRAW_NEXT = -1
 EXPORTS = (
     "rf_context_create", "rf_context_destroy", "rf_last_error", "rf_version", "rf_context_sync",
     "rf_plan_spectrum", "rf_spectrum_tables", "rf_plan_create", "rf_plan_destroy", "rf_plan_info",
-    "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_generate", "rf_generate_grid", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
+    "rf_plan_execute", "rf_plan_set_variant", "rf_plan_autotune", "rf_plan_execute_grid", "rf_plan_redraw", "rf_generate", "rf_generate_grid", "rf_marginal_device", "rf_marginal_host", "rf_blend_device",
     "rf_stats_device", "rf_corr_device", "rf_interpolate_device", "rf_interpolate_box_device", "rf_unv_read", "rf_unv_sizes", "rf_unv_copy",
     "rf_unv_destroy", "rf_normalize_device", "rf_device_uniforms", "rf_uniform_host",
     "rf_philox4x32_10_host",
@@ -168,6 +168,7 @@ def lib() -> C.CDLL:
     _sig(L, "rf_plan_set_variant", [vp, i32])
     _sig(L, "rf_plan_autotune", [vp, vp, i64, vp, i32, vp, vp])
     _sig(L, "rf_plan_execute_grid", [vp, vp, vp, vp, vp, i32, vp])
+    _sig(L, "rf_plan_redraw", [vp, i64, vp])
     _sig(L, "rf_generate", [vp, C.POINTER(Params), vp, i64, vp, vp, vp, vp, i32])
     _sig(L, "rf_marginal_device", [vp, i32, C.c_double, C.c_double, vp, i64, i32, vp])
     _sig(L, "rf_marginal_host", [vp, i32, C.c_double, C.c_double, vp, i64])

@@ -34,6 +34,8 @@ struct rf_grid_batch {
     std::vector<int> nmc;
     bool redrawable = true;
     int64_t* d_draw = nullptr;
+    rf_fold_item* d_items = nullptr;    // the folds as one grouped launch
+    int64_t foldBlocks = 0;
     // host copies of the stage descriptors and the tile kinds the set-up
     // chose (the autotuner switches a stage between those and 64 x 64)
     std::vector<rf_gemm_desc> stage[3];
@@ -114,6 +116,8 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
     total += align256(sizeof(rf_gemm_desc) * 3 * (size_t)n);
     const size_t drawOff = total;
     total += 256;
+    const size_t itemOff = total;
+    total += align256(sizeof(rf_fold_item) * (size_t)n);
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<rf_grid_batch> B(new (std::nothrow) rf_grid_batch());
     if (!B) return rf_set_error(ctx, RF_E_NOMEM, "batch allocation failed");
@@ -177,6 +181,17 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
     }
     B->d_draw = reinterpret_cast<int64_t*>(B->d_mem + drawOff);
     if (e == hipSuccess) e = rf_launch_draw_counter(B->d_draw, 1, s);   // draw 0: the plans' own realizations
+    std::vector<rf_fold_item> items((size_t)n);
+    if (B->redrawable) {
+        for (int i = 0; i < n; i++) {
+            rf_fold_item_init(&items[(size_t)i], B->folds[(size_t)i], B->nmc[(size_t)i], B->seeds[(size_t)i],
+                              B->foldBlocks);
+            B->foldBlocks += rf_fold_item_blocks(B->folds[(size_t)i]);
+        }
+        B->d_items = reinterpret_cast<rf_fold_item*>(B->d_mem + itemOff);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(B->d_items, items.data(), items.size() * sizeof(rf_fold_item), hipMemcpyHostToDevice, s);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         (void)hipFree(B->d_mem);
@@ -255,8 +270,7 @@ int rf_grid_batch_redraw(rf_grid_batch* B, int64_t drawIndex, void* stream) {
         return rf_set_error(ctx, RF_E_UNSUPPORTED, "rf_grid_batch_redraw: method 4 fields have no phases to redraw");
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (drawIndex >= 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, drawIndex, s));
-    for (size_t i = 0; i < B->folds.size(); i++)
-        RF_HIP_TRY(ctx, rf_grid_refold_draw(B->folds[i], B->nmc[i], B->seeds[i], B->d_draw, s));
+    RF_HIP_TRY(ctx, rf_launch_refold_draw(B->d_items, (int)B->folds.size(), B->foldBlocks, B->d_draw, s));
     if (drawIndex < 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, -1, s));
     return RF_OK;
 }

@@ -160,7 +160,7 @@ int rf_plan_destroy(rf_field_plan* P) {
         (void)hipSetDevice(P->ctx->device);
         (void)hipStreamSynchronize(P->ctx->stream);
     }
-    dfree(P->d_k); dfree(P->d_kRev); dfree(P->d_kIso); dfree(P->d_kIsoT);
+    dfree(P->d_k); dfree(P->d_kRad); dfree(P->d_kRev); dfree(P->d_kIso); dfree(P->d_kIsoT);
     dfree(P->d_phase); dfree(P->d_amp); dfree(P->d_uni); dfree(P->d_phT);
     dfree(P->d_aT); dfree(P->d_B); dfree(P->d_Sk); dfree(P->d_calc); dfree(P->d_gridScratch);
     if (P->d_gridCoef) (void)hipFree(P->d_gridCoef);
@@ -245,16 +245,14 @@ int rf_plan_create(rf_context* ctx, const rf_params* p, const double* kUniform,
     }
     if (p->method == RF_ISOTROPIC) {
         RF_HIP_TRY(ctx, dmalloc(&P->d_kIso, (size_t)(kN * Nmc * d)));
-        RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)kN));
+        RF_HIP_TRY(ctx, dmalloc(&P->d_kRad, (size_t)kN));
         RF_HIP_TRY(ctx, dmalloc(&P->d_Sk, (size_t)kN));
-        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->spec.k.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
+        RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_kRad, P->spec.k.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
         RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_Sk, P->spec.Sk.data(), (size_t)kN * sizeof(double), hipMemcpyHostToDevice, s));
-        RF_HIP_TRY(ctx, rf_launch_iso_prep(d, p->mode, p->isoShared, kN, Nmc, P->d_k, P->d_Sk, P->d_uni,
+        RF_HIP_TRY(ctx, rf_launch_iso_prep(d, p->mode, p->isoShared, kN, Nmc, P->d_kRad, P->d_Sk, P->d_uni,
                                            P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
         if (p->isoShared) {
             // realization 0's direction table is the shared wavevector table
-            RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // iso_prep read the radii
-            dfree(P->d_k);
             RF_HIP_TRY(ctx, dmalloc(&P->d_k, (size_t)(kN * d)));
             RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->d_kIso, (size_t)(kN * d) * sizeof(double), hipMemcpyDeviceToDevice, s));
         }
@@ -406,6 +404,42 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
         if (rc) return rc;
         RF_HIP_TRY(ctx, hipStreamSynchronize(s));   // amp is reused next slab
     }
+    return RF_OK;
+}
+
+int rf_plan_redraw(rf_field_plan* P, int64_t drawIndex, void* stream) {
+    if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
+    rf_context* ctx = P->ctx;
+    if (drawIndex < 0) return rf_set_error(ctx, RF_E_ARG, "drawIndex must be >= 0");
+    const rf_params& p = P->p;
+    const int64_t kN = P->spec.info.kNTotal;
+    const int64_t nPU = P->spec.info.nPhaseUniform * p.Nmc;
+    if (nPU > 0 && (uint64_t)drawIndex > (UINT64_MAX - (uint64_t)nPU) / (uint64_t)nPU)
+        return rf_set_error(ctx, RF_E_ARG, "drawIndex too large for the 64-bit draw counter");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    // realizations d*Nmc .. d*Nmc + Nmc - 1 of the phase stream: element
+    // (d*Nmc + r) * nPhaseUniform + j, the element plan creation draws for d = 0
+    RF_HIP_TRY(ctx, rf_launch_uniforms_at(p.seed, 2u, (uint64_t)drawIndex * (uint64_t)nPU, nPU, P->d_uni, s));
+    if (p.method == RF_FFT) {
+        RF_HIP_TRY(ctx, rf_launch_fft_coeffs(kN, p.Nmc, P->d_Sk, P->d_uni, P->d_calc, P->d_amp, P->d_phase, s));
+    } else if (p.method == RF_ISOTROPIC) {
+        // new directions, amplitudes and phases from the same radii
+        RF_HIP_TRY(ctx, rf_launch_iso_prep(p.nDim, p.mode, p.isoShared, kN, p.Nmc, P->d_kRad, P->d_Sk, P->d_uni,
+                                           P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
+        if (p.isoShared) {
+            RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->d_kIso, (size_t)(kN * p.nDim) * sizeof(double),
+                                           hipMemcpyDeviceToDevice, s));
+            if (P->d_kRev) RF_HIP_TRY(ctx, rf_launch_krev(kN * p.nDim, P->d_k, P->d_kRev, s));
+        }
+    } else {
+        RF_HIP_TRY(ctx, rf_launch_phase_from_uniform(P->d_uni, kN * p.Nmc, P->d_phase, s));
+    }
+    if (P->coeffsReady && p.method != RF_FFT) {
+        const int rc = build_form_operands(P, s);
+        if (rc) return rc;
+    }
+    P->gridCoefKey = NAN;   // the grid form re-folds on its next execution
     return RF_OK;
 }
 

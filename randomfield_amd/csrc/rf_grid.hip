@@ -918,7 +918,7 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // 278-279), or, Draw, element (*draw * Nmc + r) * kNr + src of the device
 // stream (seed, tag 2) -- the uniform the plan's generator would have drawn.
 // ampStrideR 0: every realization has the amplitudes of row 0.
-constexpr int kFoldT = 64;
+constexpr int kFoldM = 64, kFoldK = 32;   // coefficient-fold tile: rows x k
 
 // cos and sin of 2 pi U for U in [0, 1): FP32 sincospi of the FP32 part Uh
 // and the first-order term of the remainder U - Uh (|U - Uh| <= 2^-25), a few
@@ -933,51 +933,74 @@ __device__ __forceinline__ void cis_rev(double U, float* c, float* s) {
     *c = fmaf(-d, sh, ch);
 }
 
+// One 64-row x 32-k tile of the folded coefficients, transposed through LDS
+// to the K-major layout C[k * rows + m]; rows < 2^31 (checked by the launchers).
 template <bool Draw>
-__global__ __launch_bounds__(256) void k_coeffs_fold(int64_t rows, int K, int nDim, int fold, int nk1, int nk2,
-                                                     int nk3, int64_t kNu, int64_t kNr,
-                                                     const double* __restrict__ amp, int64_t ampStrideR,
-                                                     const double* __restrict__ uni, double ampMult,
-                                                     float2* __restrict__ C, uint64_t seed,
-                                                     const int64_t* __restrict__ draw, int Nmc) {
+__device__ __forceinline__ void fold_tile(const rf_fold_desc& f, int bx, int by, uint64_t seed, int64_t drawBase,
+                                          float2 (*tile)[kFoldM + 1]) {
 #pragma clang fp contract(off)
-    const int64_t drawBase = Draw ? (*draw) * (int64_t)Nmc * kNr : 0;
-    __shared__ float2 tile[kFoldT][kFoldT + 1];
-    const int64_t m0 = (int64_t)blockIdx.x * kFoldT;
-    const int k0 = blockIdx.y * kFoldT;
-    const int Q = (fold && nDim == 3) ? 2 : 1;
-    for (int e = threadIdx.x; e < kFoldT * kFoldT; e += 256) {
-        const int kk = e % kFoldT, mm = e / kFoldT;
-        const int64_t m = m0 + mm;
+    const int rows = (int)f.rows;
+    const int m0 = bx * kFoldM;
+    const int k0 = by * kFoldK;
+    const unsigned nk1 = (unsigned)f.nk1, nk2 = (unsigned)f.nk2, nk3 = (unsigned)f.nk3;
+    const unsigned Q = (f.fold && f.nDim == 3) ? 2u : 1u;
+    for (int e = threadIdx.x; e < kFoldM * kFoldK; e += 256) {
+        const int kk = e % kFoldK, mm = e / kFoldK;
+        const int m = m0 + mm;
         const int k = k0 + kk;
-        if (m >= rows || k >= K) continue;
-        const int j1 = k % nk1;
-        const int h = k / nk1;
-        int64_t rem = m;
-        const int j2 = (int)(rem % nk2); rem /= nk2;
-        const int q = (int)(rem % Q); rem /= Q;
-        const int j3 = nDim == 3 ? (int)(rem % nk3) : 0;
-        const int64_t r = nDim == 3 ? rem / nk3 : rem;
-        int b = 0;
-        if (fold) b = nDim == 3 ? (q == 0 ? (h ? 3 : 0) : (h ? 1 : 2)) : h;
-        const int64_t j = ((int64_t)j3 * nk2 + j2) * nk1 + j1;
-        const int64_t src = r * kNr + b * kNu + j;
-        const double a = amp[r * ampStrideR + b * kNu + j] * ampMult;
-        if (!Draw && !uni) {
+        if (m >= rows || k >= f.K) continue;
+        const unsigned j1 = (unsigned)k % nk1;
+        const unsigned h = (unsigned)k / nk1;
+        unsigned rem = (unsigned)m;
+        const unsigned j2 = rem % nk2; rem /= nk2;
+        const unsigned q = rem % Q; rem /= Q;
+        const unsigned j3 = f.nDim == 3 ? rem % nk3 : 0u;
+        const unsigned r = f.nDim == 3 ? rem / nk3 : rem;
+        unsigned b = 0;
+        if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
+        const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * nk2 + j2) * nk1 + j1;   // b kNu + j
+        const int64_t src = (int64_t)r * f.kNr + jb;
+        const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
+        if (!Draw && !f.uni) {
             tile[kk][mm] = make_float2((float)a, 0.0f);
         } else {
             float cs, sn;
-            cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : uni[src], &cs, &sn);
+            cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : f.uni[src], &cs, &sn);
             tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kFoldT * kFoldT; e += 256) {
-        const int mm = e % kFoldT, kk = e / kFoldT;
-        const int64_t m = m0 + mm;
+    for (int e = threadIdx.x; e < kFoldM * kFoldK; e += 256) {
+        const int mm = e % kFoldM, kk = e / kFoldM;
+        const int m = m0 + mm;
         const int k = k0 + kk;
-        if (m < rows && k < K) C[(int64_t)k * rows + m] = tile[kk][mm];
+        if (m < rows && k < f.K) f.C[(int64_t)k * rows + m] = tile[kk][mm];
     }
+}
+
+// The coefficients of one grid execution from the plan's phase uniforms
+// (f.uni) or, method 4, the real DCT coefficients (f.uni NULL).
+__global__ __launch_bounds__(256) void k_coeffs_fold(rf_fold_desc f) {
+    __shared__ float2 tile[kFoldK][kFoldM + 1];
+    fold_tile<false>(f, blockIdx.x, blockIdx.y, 0, 0, tile);
+}
+
+// Every field of a batch in one launch, phases drawn in place: field i owns
+// blocks block0 .. block0 + nbx * nby - 1 (ascending in i), block -> field by
+// binary search; realizations *draw * Nmc .. of the field's stream.
+__global__ __launch_bounds__(256) void k_coeffs_fold_draw(const rf_fold_item* __restrict__ items, int n,
+                                                          const int64_t* __restrict__ draw) {
+    __shared__ float2 tile[kFoldK][kFoldM + 1];
+    const int64_t blk = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block0 <= blk) lo = mid; else hi = mid - 1;
+    }
+    const rf_fold_item& it = items[lo];
+    const int lb = (int)(blk - it.block0);
+    const int64_t drawBase = (*draw) * (int64_t)it.Nmc * it.f.kNr;
+    fold_tile<true>(it.f, lb % it.nbx, lb / it.nbx, it.seed, drawBase, tile);
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -1097,11 +1120,11 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     fd.amp = d_amp; fd.ampStrideR = ampShared ? 0 : fd.kNr;
     fd.uni = kind == 1 ? nullptr : d_unif;
     fd.ampMult = ampMult * sc; fd.C = Cf;
+    if (fd.rows > INT32_MAX) return hipErrorInvalidValue;
     if (!fresh)
-        hipLaunchKernelGGL(k_coeffs_fold<false>,
-                           dim3((unsigned)((fd.rows + kFoldT - 1) / kFoldT), (unsigned)((K1 + kFoldT - 1) / kFoldT)),
-                           dim3(256), 0, s, fd.rows, fd.K, nDim, fold, fd.nk1, fd.nk2, fd.nk3, fd.kNu, fd.kNr,
-                           fd.amp, fd.ampStrideR, fd.uni, fd.ampMult, Cf, (uint64_t)0, (const int64_t*)nullptr, Nmc);
+        hipLaunchKernelGGL(k_coeffs_fold,
+                           dim3((unsigned)((fd.rows + kFoldM - 1) / kFoldM), (unsigned)((K1 + kFoldK - 1) / kFoldK)),
+                           dim3(256), 0, s, fd);
     // stage 1: rows (r, j3, q, j2) x K (h, j1) -> V[r][j3][q][j2][x1]
     const int64_t rows1 = nC / K1;
     float2* V = p; p += rows1 * n[0];
@@ -1160,13 +1183,23 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     return hipGetLastError();
 }
 
-hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw,
-                               hipStream_t s) {
-    if (!f.uni || !d_draw) return hipErrorInvalidValue;     // method 4 has no phases to redraw
-    hipLaunchKernelGGL(k_coeffs_fold<true>,
-                       dim3((unsigned)((f.rows + kFoldT - 1) / kFoldT), (unsigned)((f.K + kFoldT - 1) / kFoldT)),
-                       dim3(256), 0, s, f.rows, f.K, f.nDim, f.fold, f.nk1, f.nk2, f.nk3, f.kNu, f.kNr, f.amp,
-                       f.ampStrideR, f.uni, f.ampMult, f.C, seed, d_draw, Nmc);
+int rf_fold_item_blocks(const rf_fold_desc& f) {
+    return (int)(((f.rows + kFoldM - 1) / kFoldM) * ((f.K + kFoldK - 1) / kFoldK));
+}
+
+void rf_fold_item_init(rf_fold_item* it, const rf_fold_desc& f, int Nmc, uint64_t seed, int64_t block0) {
+    it->f = f;
+    it->seed = seed;
+    it->Nmc = Nmc;
+    it->nbx = (int)((f.rows + kFoldM - 1) / kFoldM);
+    it->block0 = block0;
+}
+
+hipError_t rf_launch_refold_draw(const rf_fold_item* d_items, int n, int64_t blocks, const int64_t* d_draw,
+                                 hipStream_t s) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    if (!d_items || !d_draw || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_coeffs_fold_draw, dim3((unsigned)blocks), dim3(256), 0, s, d_items, n, d_draw);
     return hipGetLastError();
 }
 

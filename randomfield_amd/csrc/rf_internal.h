@@ -47,6 +47,7 @@ struct rf_field_plan {
     int nCalc = 0;
     // device buffers (owned)
     double* d_k = nullptr;       // shared k: kN x nDim (normalised units)
+    double* d_kRad = nullptr;    // isotropic: radii kN
     double* d_kRev = nullptr;    // shared k / (2 pi)
     double* d_kIso = nullptr;    // per-realization k: Nmc x kN x nDim
     double* d_kIsoT = nullptr;   // FP32 isotropic form: kN x Rpad x nDim, revolutions
@@ -82,6 +83,9 @@ int rf_validate_params(const rf_params* p, std::string* err);
 // ---- kernel launchers (rf_sum.hip, rf_prep.hip, rf_mfma.hip, rf_post.hip)
 hipError_t rf_launch_uniforms(uint64_t seed, uint32_t tag, int64_t count, double* d_out,
                               hipStream_t s);
+// elements first .. first + count - 1 of stream (seed, tag)
+hipError_t rf_launch_uniforms_at(uint64_t seed, uint32_t tag, uint64_t first, int64_t count,
+                                 double* d_out, hipStream_t s);
 hipError_t rf_launch_phase_from_uniform(const double* d_u, int64_t n, double* d_phase,
                                         hipStream_t s);
 hipError_t rf_launch_iso_prep(int nDim, int mode, int shared, int64_t kN, int Nmc,
@@ -233,12 +237,25 @@ struct rf_grid_stages {
     int n;
     rf_fold_desc fold;    // the coefficient fold this execution formed (or reused)
 };
-// Re-forms the folded coefficients of a Shinozuka grid execution from new
-// phases 2 pi U, U element (*d_draw * Nmc + r) * kNr + b kNu + j of the
-// device stream (seed, tag 2): realizations *d_draw * Nmc .. of the stream
-// whose first Nmc are the plan's own device draws (draw index 0).  Reads the draw index on the
+// A field of a batch redraw: its fold, the seed of its phase stream and its
+// first block in the grouped launch.
+struct rf_fold_item {
+    rf_fold_desc f;
+    uint64_t seed;
+    int Nmc;
+    int nbx;          // row blocks of the fold
+    int64_t block0;
+};
+int rf_fold_item_blocks(const rf_fold_desc& f);
+void rf_fold_item_init(rf_fold_item* it, const rf_fold_desc& f, int Nmc, uint64_t seed, int64_t block0);
+// Re-forms the folded coefficients of the batch's Shinozuka fields (n items,
+// `blocks` = the last item's block0 + its blocks) from new phases 2 pi U, U
+// element (*d_draw * Nmc + r) * kNr + b kNu + j of each field's device stream
+// (seed, tag 2): realizations *d_draw * Nmc .. of the stream whose first Nmc
+// are the plan's own device draws (draw index 0).  Reads the draw index on the
 // device, so a captured graph draws what the counter holds at replay.
-hipError_t rf_grid_refold_draw(const rf_fold_desc& f, int Nmc, uint64_t seed, const int64_t* d_draw, hipStream_t s);
+hipError_t rf_launch_refold_draw(const rf_fold_item* d_items, int n, int64_t blocks, const int64_t* d_draw,
+                                 hipStream_t s);
 // *d_draw = value (value >= 0) or *d_draw += 1 (value < 0), one thread
 hipError_t rf_launch_draw_counter(int64_t* d_draw, int64_t value, hipStream_t s);
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);

@@ -10,9 +10,10 @@ constexpr int kBlock = 256;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-__global__ void k_uniforms(uint64_t seed, uint32_t tag, int64_t count, double* __restrict__ out) {
+__global__ void k_uniforms(uint64_t seed, uint32_t tag, uint64_t first, int64_t count,
+                           double* __restrict__ out) {
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e < count) out[e] = rf_uniform(seed, tag, (uint64_t)e);
+    if (e < count) out[e] = rf_uniform(seed, tag, first + (uint64_t)e);
 }
 
 // randomFieldND.f90:278-279: phiK = 2.0D0*pi*U, with the reference's pi.
@@ -148,12 +149,17 @@ __global__ void k_scale(int64_t n, const double* __restrict__ a, double s, doubl
 
 }  // namespace
 
+hipError_t rf_launch_uniforms_at(uint64_t seed, uint32_t tag, uint64_t first, int64_t count,
+                                 double* d_out, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_uniforms, dim3(grid_for(count)), dim3(kBlock), 0, s, seed, tag, first,
+                       count, d_out);
+    return hipGetLastError();
+}
+
 hipError_t rf_launch_uniforms(uint64_t seed, uint32_t tag, int64_t count, double* d_out,
                               hipStream_t s) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_uniforms, dim3(grid_for(count)), dim3(kBlock), 0, s, seed, tag, count,
-                       d_out);
-    return hipGetLastError();
+    return rf_launch_uniforms_at(seed, tag, 0, count, d_out, s);
 }
 
 hipError_t rf_launch_phase_from_uniform(const double* d_u, int64_t n, double* d_phase,

@@ -324,8 +324,8 @@ def run_cfg4(args):
                                 "note": "the same realizations re-evaluated (no redraw)"},
                  redraw_ms=float(ms_d.mean()), generation_ms=float(ms_g.mean()), blend_ms=float(ms_b.mean()),
                  redraw={"coefficients_per_step": ncoef, "kernel_ms": float(ms_d.mean()),
-                         "note": "Philox phases + FP64 sincos + fold, 16 B per coefficient (amplitude read, "
-                                 "FP32 complex write)"},
+                         "note": "one grouped launch over all fields: Philox uniforms, FP32 sincos of 2 pi U, "
+                                 "fold; 8 B written per coefficient (FP32 complex)"},
                  generation_roofline={"bound": "mfma", "unit": "TFLOP/s", "peak": 2500.0,
                                       "achieved": flop_gen / (ms_g.mean() * 1e-3) / 1e12,
                                       "frac": flop_gen / (ms_g.mean() * 1e-3) / 1e12 / 2500.0,

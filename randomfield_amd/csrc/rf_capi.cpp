@@ -420,10 +420,15 @@ int rf_plan_redraw(rf_field_plan* P, int64_t drawIndex, void* stream) {
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     // realizations d*Nmc .. d*Nmc + Nmc - 1 of the phase stream: element
     // (d*Nmc + r) * nPhaseUniform + j, the element plan creation draws for d = 0
-    RF_HIP_TRY(ctx, rf_launch_uniforms_at(p.seed, 2u, (uint64_t)drawIndex * (uint64_t)nPU, nPU, P->d_uni, s));
+    const uint64_t first = (uint64_t)drawIndex * (uint64_t)nPU;
     if (p.method == RF_FFT) {
-        RF_HIP_TRY(ctx, rf_launch_fft_coeffs(kN, p.Nmc, P->d_Sk, P->d_uni, P->d_calc, P->d_amp, P->d_phase, s));
-    } else if (p.method == RF_ISOTROPIC) {
+        // drawn inside the coefficient kernel (a third of them is discarded)
+        RF_HIP_TRY(ctx, rf_launch_fft_coeffs_draw(kN, p.Nmc, P->d_Sk, P->d_calc, p.seed, first, P->d_amp, s));
+        P->gridCoefKey = NAN;
+        return RF_OK;
+    }
+    RF_HIP_TRY(ctx, rf_launch_uniforms_at(p.seed, 2u, first, nPU, P->d_uni, s));
+    if (p.method == RF_ISOTROPIC) {
         // new directions, amplitudes and phases from the same radii
         RF_HIP_TRY(ctx, rf_launch_iso_prep(p.nDim, p.mode, p.isoShared, kN, p.Nmc, P->d_kRad, P->d_Sk, P->d_uni,
                                            P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
@@ -435,7 +440,7 @@ int rf_plan_redraw(rf_field_plan* P, int64_t drawIndex, void* stream) {
     } else {
         RF_HIP_TRY(ctx, rf_launch_phase_from_uniform(P->d_uni, kN * p.Nmc, P->d_phase, s));
     }
-    if (P->coeffsReady && p.method != RF_FFT) {
+    if (P->coeffsReady) {
         const int rc = build_form_operands(P, s);
         if (rc) return rc;
     }

@@ -98,6 +98,10 @@ hipError_t rf_launch_prep_fp32(int64_t kN, int Nmc, int Rpad, const double* d_ph
 hipError_t rf_launch_krev(int64_t count, const double* d_k, double* d_kRev, hipStream_t s);
 hipError_t rf_launch_fft_coeffs(int64_t kN, int Nmc, const double* d_Sk, const double* d_u,
                                 const uint8_t* d_calc, double* d_amp, double* d_phase, hipStream_t s);
+// k_fft_coeffs's amplitudes with the uniforms drawn in place from element
+// `first` of stream (seed, tag 2); the phases are left as they are (zero)
+hipError_t rf_launch_fft_coeffs_draw(int64_t kN, int Nmc, const double* d_Sk, const uint8_t* d_calc,
+                                     uint64_t seed, uint64_t first, double* d_amp, hipStream_t s);
 
 struct rf_sum_args {
     const double* x;      // nDim x N caller units

@@ -142,6 +142,27 @@ __global__ void k_fft_coeffs(int64_t kN, int Nmc, const double* __restrict__ Sk,
     phase[e] = 0.0;
 }
 
+// The same coefficients with the uniforms drawn in place: element
+// first + r (3 kN - 1) + i of stream (seed, tag 2) is U[r][i] (the values
+// k_uniforms writes); the discarded third is never drawn.  Phase stays 0.
+__global__ void k_fft_coeffs_draw(int64_t kN, int Nmc, const double* __restrict__ Sk,
+                                  const uint8_t* __restrict__ calc, uint64_t seed, uint64_t first,
+                                  double* __restrict__ amp) {
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kN * Nmc) return;
+    const int r = (int)(e / kN);
+    const int64_t j = e - (int64_t)r * kN;
+    if (!calc[r]) {
+        amp[e] = 0.0;
+        return;
+    }
+    const uint64_t row = first + (uint64_t)r * (uint64_t)(3 * kN - 1);
+    const double g = rf_uniform(seed, 2u, row + (uint64_t)j) - 0.5;
+    const double ph = rf_uniform(seed, 2u, row + (uint64_t)(2 * kN - 1 + j));
+    amp[e] = g * sqrt(Sk[j]) * cos(2.0 * RF_PI_REF * ph);
+}
+
 __global__ void k_scale(int64_t n, const double* __restrict__ a, double s, double* __restrict__ b) {
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e < n) b[e] = a[e] * s;
@@ -200,6 +221,15 @@ hipError_t rf_launch_prep_fp32_iso(int nDim, int64_t kN, int Nmc, int Rpad, cons
     const int64_t n = kN * Rpad;
     hipLaunchKernelGGL(k_prep_fp32_iso, dim3(grid_for(n)), dim3(kBlock), 0, s, nDim, kN, Nmc, Rpad,
                        d_kIso, d_phase, d_amp, ampMult, d_kT, d_pT, d_aT);
+    return hipGetLastError();
+}
+
+hipError_t rf_launch_fft_coeffs_draw(int64_t kN, int Nmc, const double* d_Sk, const uint8_t* d_calc,
+                                     uint64_t seed, uint64_t first, double* d_amp, hipStream_t s) {
+    const int64_t n = kN * Nmc;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fft_coeffs_draw, dim3(grid_for(n)), dim3(kBlock), 0, s, kN, Nmc, d_Sk, d_calc, seed,
+                       first, d_amp);
     return hipGetLastError();
 }
 

@@ -349,11 +349,13 @@ int rf_grid_batch_autotune(rf_grid_batch* batch, void* stream, int32_t* chosen);
  * type): points that one field alone covers -- weight 1 -- are stored by the
  * last contraction straight into d_group, the others into the field as
  * before, and rf_blend_plan_execute then forms only the points several
- * fields cover.  The group and the fields are the same as without fusion,
- * bit for bit, with the fields' interior neither written to nor read back
- * from the field buffers.  execute of the blend plan must then be given
- * d_group; d_group NULL undoes the fusion.  Synchronous set-up; the batch's
- * tables are replaced. */
+ * fields cover.  The group is the same as without fusion, bit for bit; the
+ * field buffers then hold only the points several fields cover (their
+ * single-cover interior is neither written nor read).  execute of the blend
+ * plan must then be given d_group; d_group NULL undoes the fusion, which a
+ * caller that destroys the batch and keeps using the blend plan must do
+ * first (a fused blend plan forms the overlaps only).  Synchronous set-up;
+ * the batch's tables are replaced. */
 int rf_grid_batch_fuse_blend(rf_grid_batch* batch, rf_blend_plan* blend, void* d_group);
 int rf_grid_batch_redraw(rf_grid_batch* batch, int64_t drawIndex, void* stream);
 int rf_grid_batch_destroy(rf_grid_batch* batch);

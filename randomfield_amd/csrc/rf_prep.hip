@@ -124,9 +124,22 @@ __global__ void k_prep_fp32_iso(int nDim, int64_t kN, int Nmc, int Rpad,
     }
 }
 
+// cos(2 pi U) for U in [0, 1) in FP32: cospi of the FP32 part Uh with the
+// first-order term of the remainder U - Uh (|U - Uh| <= 2^-25), a few FP32
+// ulps from the FP64 value (the grid form's coefficients are FP32).  Exact pi:
+// the reference's literal (constants_RF.f90:5) differs by 1e-14 relative.
+__device__ __forceinline__ float cos_rev(double U) {
+    const float uh = (float)U;
+    const float ul = (float)(U - (double)uh);
+    float sh, ch;
+    sincospif(2.0f * uh, &sh, &ch);
+    return fmaf(-6.28318530717958648f * ul, sh, ch);
+}
+
 // gen_Std_Gauss_FFT coefficients (randomFieldND.f90:683-689) per realization:
 // uniforms u[r][0 .. 3 kN - 2] = gammaK (kN), kN - 1 discarded, phiK (kN);
-// amp = (gammaK - 0.5) * sqrt(S) * cos(2 pi phiK), phase 0.
+// amp = (gammaK - 0.5) * sqrt(S) * cos(2 pi phiK), phase 0; the cosine in FP32
+// (cos_rev): the grid form folds these amplitudes into FP32 coefficients.
 __global__ void k_fft_coeffs(int64_t kN, int Nmc, const double* __restrict__ Sk,
                              const double* __restrict__ u, const uint8_t* __restrict__ calc,
                              double* __restrict__ amp, double* __restrict__ phase) {
@@ -138,7 +151,7 @@ __global__ void k_fft_coeffs(int64_t kN, int Nmc, const double* __restrict__ Sk,
     const double* U = u + (int64_t)r * (3 * kN - 1);
     const double g = U[j] - 0.5;
     const double ph = U[2 * kN - 1 + j];
-    amp[e] = calc[r] ? g * sqrt(Sk[j]) * cos(2.0 * RF_PI_REF * ph) : 0.0;
+    amp[e] = calc[r] ? g * sqrt(Sk[j]) * (double)cos_rev(ph) : 0.0;
     phase[e] = 0.0;
 }
 
@@ -160,7 +173,7 @@ __global__ void k_fft_coeffs_draw(int64_t kN, int Nmc, const double* __restrict_
     const uint64_t row = first + (uint64_t)r * (uint64_t)(3 * kN - 1);
     const double g = rf_uniform(seed, 2u, row + (uint64_t)j) - 0.5;
     const double ph = rf_uniform(seed, 2u, row + (uint64_t)(2 * kN - 1 + j));
-    amp[e] = g * sqrt(Sk[j]) * cos(2.0 * RF_PI_REF * ph);
+    amp[e] = g * sqrt(Sk[j]) * (double)cos_rev(ph);
 }
 
 __global__ void k_scale(int64_t n, const double* __restrict__ a, double s, double* __restrict__ b) {

@@ -944,29 +944,50 @@ __device__ __forceinline__ void fold_tile(const rf_fold_desc& f, int bx, int by,
     const int k0 = by * kFoldK;
     const unsigned nk1 = (unsigned)f.nk1, nk2 = (unsigned)f.nk2, nk3 = (unsigned)f.nk3;
     const unsigned Q = (f.fold && f.nDim == 3) ? 2u : 1u;
-    for (int e = threadIdx.x; e < kFoldM * kFoldK; e += 256) {
-        const int kk = e % kFoldK, mm = e / kFoldK;
-        const int m = m0 + mm;
-        const int k = k0 + kk;
-        if (m >= rows || k >= f.K) continue;
+    // a thread keeps one k (256 is a multiple of kFoldK) and walks rows
+    // mm0, mm0 + 8, ...: the row's digits (j2, q, j3, r) advance by carries
+    // instead of a division per element
+    static_assert(256 % kFoldK == 0, "one k per thread");
+    constexpr int kRowStep = 256 / kFoldK;
+    const int kk = threadIdx.x % kFoldK, mm0 = threadIdx.x / kFoldK;
+    const int k = k0 + kk;
+    if (k < f.K) {
         const unsigned j1 = (unsigned)k % nk1;
         const unsigned h = (unsigned)k / nk1;
-        unsigned rem = (unsigned)m;
-        const unsigned j2 = rem % nk2; rem /= nk2;
-        const unsigned q = rem % Q; rem /= Q;
-        const unsigned j3 = f.nDim == 3 ? rem % nk3 : 0u;
-        const unsigned r = f.nDim == 3 ? rem / nk3 : rem;
-        unsigned b = 0;
-        if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
-        const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * nk2 + j2) * nk1 + j1;   // b kNu + j
-        const int64_t src = (int64_t)r * f.kNr + jb;
-        const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
-        if (!Draw && !f.uni) {
-            tile[kk][mm] = make_float2((float)a, 0.0f);
-        } else {
-            float cs, sn;
-            cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : f.uni[src], &cs, &sn);
-            tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
+        unsigned rem = (unsigned)(m0 + mm0);
+        unsigned j2 = rem % nk2; rem /= nk2;
+        unsigned q = rem % Q; rem /= Q;
+        unsigned j3 = f.nDim == 3 ? rem % nk3 : 0u;
+        unsigned r = f.nDim == 3 ? rem / nk3 : rem;
+        for (int mm = mm0; mm < kFoldM; mm += kRowStep) {
+            const int m = m0 + mm;
+            if (m >= rows) break;
+            unsigned b = 0;
+            if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
+            const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * nk2 + j2) * nk1 + j1;   // b kNu + j
+            const int64_t src = (int64_t)r * f.kNr + jb;
+            const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
+            if (!Draw && !f.uni) {
+                tile[kk][mm] = make_float2((float)a, 0.0f);
+            } else {
+                float cs, sn;
+                cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : f.uni[src], &cs, &sn);
+                tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
+            }
+            // next row m + kRowStep: j2 fastest, then q, then j3 (3-D), then r
+            j2 += kRowStep;
+            while (j2 >= nk2) {
+                j2 -= nk2;
+                if (++q == Q) {
+                    q = 0;
+                    if (f.nDim != 3) {
+                        ++r;
+                    } else if (++j3 == nk3) {
+                        j3 = 0;
+                        ++r;
+                    }
+                }
+            }
         }
     }
     __syncthreads();

@@ -135,7 +135,7 @@ __device__ __forceinline__ void sum_fp32_iso_point(const rf_sum_args& a, gdbl* _
             double t = kj[r * D] * xn[0];
 #pragma unroll
             for (int d = 1; d < D; d++) t = fma(kj[r * D + d], xn[d], t);
-            t = t - rint(t);
+            t = __builtin_amdgcn_fract(t);   // k.x mod 1 revolution: one v_fract_f64, exact
             acc[r] = fmaf(aj[r], __builtin_amdgcn_cosf((float)t + pj[r]), acc[r]);
         }
     }

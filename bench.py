@@ -665,20 +665,23 @@ def main() -> int:
 
     # PCIe-inclusive rate, reported beside the HBM-resident value and never as
     # it: host points in, host field out through pinned buffers, one launch
-    hx = torch.empty((N_POINTS, 3), dtype=torch.float64, pin_memory=True)
-    hx.copy_(torch.from_numpy(x))
-    hout = torch.empty((N_MC, N_POINTS), dtype=torch.float32, pin_memory=True)
-    torch.cuda.synchronize()
-    t4 = time.perf_counter()
-    dx.copy_(hx, non_blocking=True)
-    plan.execute(dx, out, stream=sptr)
-    hout.copy_(out, non_blocking=True)
-    torch.cuda.synchronize()
-    pcie_s = time.perf_counter() - t4
-    pcie = {"value": work / pcie_s, "ms": pcie_s * 1e3, "unit": "points*realizations/s",
-            "note": "48 MB of points host->device, one launch, the 2.05 GB FP32 field device->host "
-                    "(pinned host buffers)"}
-    del hx, hout
+    try:
+        hx = torch.empty((N_POINTS, 3), dtype=torch.float64, pin_memory=True)
+        hx.copy_(torch.from_numpy(x))
+        hout = torch.empty((N_MC, N_POINTS), dtype=torch.float32, pin_memory=True)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        dx.copy_(hx, non_blocking=True)
+        plan.execute(dx, out, stream=sptr)
+        hout.copy_(out, non_blocking=True)
+        torch.cuda.synchronize()
+        pcie_s = time.perf_counter() - t4
+        pcie = {"value": work / pcie_s, "ms": pcie_s * 1e3, "unit": "points*realizations/s",
+                "note": "48 MB of points host->device, one launch, the 2.05 GB FP32 field device->host "
+                        "(pinned host buffers)"}
+        del hx, hout
+    except (RuntimeError, MemoryError) as e:        # no pinned memory: the headline still prints
+        pcie = {"error": f"{type(e).__name__}: {e}"}
 
     # CPU baseline: the oracle (the reference algorithm, FP64, libm cos) on a
     # bounded point sample of the same workload, all host threads

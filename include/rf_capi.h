@@ -201,11 +201,21 @@ int rf_plan_redraw(rf_field_plan* plan, int64_t drawIndex, void* stream);
 /* Host-buffer form of rf_plan_execute_grid (the Fortran facade's structured
  * path; method 4 runs here, as gen_Std_Gauss_FFT does on the mesh's grid,
  * randomFieldND.f90:520-763): plan, grid of nstep points from origin at step
- * (caller units, x fastest), out N x Nmc; synchronous. */
+ * (caller units, x fastest), out N x Nmc; synchronous.
+ *
+ * rf_generate and rf_generate_grid keep the plans they build on the context
+ * (at most 4, the least recently used dropped; freed by rf_context_destroy)
+ * with a staging buffer that grows to the largest call.  A call whose
+ * rf_params bytes, wavevector uniforms and calculate mask equal a kept
+ * plan's refreshes that plan's draws instead of building a new one: the
+ * caller's phaseUniform re-forms it, a NULL phaseUniform redraws draw 0 of
+ * the device stream -- the values a new plan would hold, bit for bit.  The
+ * reference rebuilds its k table and spectrum on every call (gen_Std_Gauss,
+ * randomFieldND.f90:44); here only the draws are renewed. */
 int rf_generate_grid(rf_context* ctx, const rf_params* p, const double* origin,
                      const double* step, const int32_t* nstep, const double* phaseUniform,
                      const uint8_t* calculate, void* out, int32_t outDtype);
-/* Host-buffer convenience: create, upload, execute, download, destroy.
+/* Host-buffer convenience: plan (kept, as above), upload, execute, download.
  * x: nDim x N, out: N x Nmc (outDtype). */
 int rf_generate(rf_context* ctx, const rf_params* p, const double* x,
                 int64_t N, const double* kUniform, const double* phaseUniform,

@@ -28,7 +28,7 @@ from ._capi import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate",
+    "Context", "FieldPlan", "make_params", "plan_spectrum", "spectrum_tables", "generate", "generate_grid",
     "create_random_field", "multivariate_transformation", "reference_seed", "reference_draws",
     "blend", "BlendPlan", "GridBatch", "Graph", "stats", "corr", "interpolate", "interpolate_box", "read_unv",
     "normalize", "localization_layout",
@@ -238,6 +238,21 @@ def generate(ctx: Context, params: Params, x: np.ndarray, kU=None, phU=None, cal
     c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
     check(lib().rf_generate(ctx.handle, C.byref(params), _ptr(x), x.shape[0], _ptr(kU), _ptr(phU),
                             _ptr(c), _ptr(out), _dtype_code(out.dtype)), ctx.handle)
+    return out
+
+
+def generate_grid(ctx: Context, params: Params, origin, step, nstep, phU=None, calc=None,
+                  dtype=np.float64) -> np.ndarray:
+    """rf_generate_grid on host arrays: the grid of nstep points from origin at
+    step (x fastest) -> field (Nmc, prod(nstep))."""
+    o = np.zeros(3); o[: len(origin)] = origin
+    st = np.ones(3); st[: len(step)] = step
+    ns = np.ones(3, dtype=np.int32); ns[: len(nstep)] = nstep
+    out = np.zeros((params.Nmc, int(np.prod(ns, dtype=np.int64))), dtype=dtype)
+    phU = None if phU is None else np.ascontiguousarray(phU, dtype=np.float64)
+    c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
+    check(lib().rf_generate_grid(ctx.handle, C.byref(params), _ptr(o), _ptr(st), _ptr(ns), _ptr(phU), _ptr(c),
+                                 _ptr(out), _dtype_code(out.dtype)), ctx.handle)
     return out
 
 

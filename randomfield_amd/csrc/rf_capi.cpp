@@ -122,6 +122,9 @@ int rf_context_destroy(rf_context* ctx) {
         (void)hipStreamDestroy(ctx->stream);
     }
     dfree(ctx->d_scratch);
+    for (auto& c : ctx->planCache) rf_plan_destroy(c.plan);
+    ctx->planCache.clear();
+    dfree(ctx->d_io);
     delete ctx;
     return RF_OK;
 }
@@ -407,6 +410,113 @@ int rf_plan_execute(rf_field_plan* P, const double* d_x, int64_t N, void* d_out,
     return RF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Re-forms what a plan builds from its phase uniforms d_uni: phases (methods
+// 2/3), directions, amplitudes and phases (method 1), DCT coefficients
+// (method 4), the form's operand tables; the grid form re-folds on its next
+// execution.
+int plan_reform(rf_field_plan* P, hipStream_t s) {
+    rf_context* ctx = P->ctx;
+    const rf_params& p = P->p;
+    const int64_t kN = P->spec.info.kNTotal;
+    if (p.method == RF_FFT) {
+        RF_HIP_TRY(ctx, rf_launch_fft_coeffs(kN, p.Nmc, P->d_Sk, P->d_uni, P->d_calc, P->d_amp, P->d_phase, s));
+    } else if (p.method == RF_ISOTROPIC) {
+        // new directions, amplitudes and phases from the same radii
+        RF_HIP_TRY(ctx, rf_launch_iso_prep(p.nDim, p.mode, p.isoShared, kN, p.Nmc, P->d_kRad, P->d_Sk, P->d_uni,
+                                           P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
+        if (p.isoShared) {
+            RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->d_kIso, (size_t)(kN * p.nDim) * sizeof(double),
+                                           hipMemcpyDeviceToDevice, s));
+            if (P->d_kRev) RF_HIP_TRY(ctx, rf_launch_krev(kN * p.nDim, P->d_k, P->d_kRev, s));
+        }
+    } else {
+        RF_HIP_TRY(ctx, rf_launch_phase_from_uniform(P->d_uni, kN * p.Nmc, P->d_phase, s));
+    }
+    if (P->coeffsReady && p.method != RF_FFT) {
+        const int rc = build_form_operands(P, s);
+        if (rc) return rc;
+    }
+    P->gridCoefKey = NAN;
+    return RF_OK;
+}
+
+// A plan of the context's cache for (p, kU, calc), refreshed with the
+// phase draws phU (host) or, phU NULL, the device stream's draw 0 -- the
+// same plan rf_plan_create would build -- or a new plan put in the cache.
+int cached_plan(rf_context* ctx, const rf_params* p, const double* kU, const double* phU, const uint8_t* calc,
+                rf_field_plan** out) {
+    *out = nullptr;
+    if (p) {
+        for (auto& c : ctx->planCache) {
+            // equal parameter bytes fix the table sizes, so the stored lengths apply
+            if (std::memcmp(&c.p, p, sizeof(rf_params)) != 0) continue;
+            rf_field_plan* P = c.plan;
+            const bool useKU = kU && P->spec.info.nKUniform > 0;   // kU is read by method 3 only
+            if (useKU != c.hasKU) continue;
+            if (useKU && std::memcmp(c.kU.data(), kU, c.kU.size() * sizeof(double)) != 0) continue;
+            if ((calc != nullptr) != c.hasCalc) continue;
+            if (calc && std::memcmp(c.calc.data(), calc, c.calc.size()) != 0) continue;
+            RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+            const hipStream_t s = ctx->stream;
+            int rc;
+            if (phU) {
+                const size_t nPU = (size_t)(P->spec.info.nPhaseUniform * p->Nmc);
+                RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_uni, phU, nPU * sizeof(double), hipMemcpyHostToDevice, s));
+                rc = plan_reform(P, s);
+                if (rc) (void)hipStreamSynchronize(s);   // the copy reads the caller's phU
+            } else {
+                rc = rf_plan_redraw(P, 0, s);
+            }
+            if (rc) return rc;
+            c.lastUse = ++ctx->useClock;
+            *out = P;
+            return RF_OK;
+        }
+    }
+    rf_field_plan* P = nullptr;
+    const int rc = rf_plan_create(ctx, p, kU, phU, calc, &P);
+    if (rc) return rc;
+    if (ctx->planCache.size() >= kPlanCache) {
+        auto lru = std::min_element(ctx->planCache.begin(), ctx->planCache.end(),
+                                    [](const rf_cached_plan& x, const rf_cached_plan& y) { return x.lastUse < y.lastUse; });
+        rf_plan_destroy(lru->plan);
+        ctx->planCache.erase(lru);
+    }
+    rf_cached_plan c;
+    c.p = *p;
+    c.hasKU = kU && P->spec.info.nKUniform > 0;
+    if (c.hasKU) c.kU.assign(kU, kU + P->spec.info.nKUniform);
+    c.hasCalc = calc != nullptr;
+    if (calc) c.calc.assign(calc, calc + p->Nmc);
+    c.plan = P;
+    c.lastUse = ++ctx->useClock;
+    ctx->planCache.push_back(std::move(c));
+    *out = P;
+    return RF_OK;
+}
+
+// The context's staging buffer for the host-buffer entry points, at least bytes.
+int io_buffer(rf_context* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->ioBytes) {
+        RF_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        dfree(ctx->d_io);
+        ctx->d_io = nullptr;
+        ctx->ioBytes = 0;
+        if (hipMalloc(&ctx->d_io, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "staging allocation failed");
+        ctx->ioBytes = bytes;
+    }
+    *out = ctx->d_io;
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rf_plan_redraw(rf_field_plan* P, int64_t drawIndex, void* stream) {
     if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
     rf_context* ctx = P->ctx;
@@ -428,24 +538,7 @@ int rf_plan_redraw(rf_field_plan* P, int64_t drawIndex, void* stream) {
         return RF_OK;
     }
     RF_HIP_TRY(ctx, rf_launch_uniforms_at(p.seed, 2u, first, nPU, P->d_uni, s));
-    if (p.method == RF_ISOTROPIC) {
-        // new directions, amplitudes and phases from the same radii
-        RF_HIP_TRY(ctx, rf_launch_iso_prep(p.nDim, p.mode, p.isoShared, kN, p.Nmc, P->d_kRad, P->d_Sk, P->d_uni,
-                                           P->d_calc, P->d_kIso, P->d_amp, P->d_phase, s));
-        if (p.isoShared) {
-            RF_HIP_TRY(ctx, hipMemcpyAsync(P->d_k, P->d_kIso, (size_t)(kN * p.nDim) * sizeof(double),
-                                           hipMemcpyDeviceToDevice, s));
-            if (P->d_kRev) RF_HIP_TRY(ctx, rf_launch_krev(kN * p.nDim, P->d_k, P->d_kRev, s));
-        }
-    } else {
-        RF_HIP_TRY(ctx, rf_launch_phase_from_uniform(P->d_uni, kN * p.Nmc, P->d_phase, s));
-    }
-    if (P->coeffsReady) {
-        const int rc = build_form_operands(P, s);
-        if (rc) return rc;
-    }
-    P->gridCoefKey = NAN;   // the grid form re-folds on its next execution
-    return RF_OK;
+    return plan_reform(P, s);
 }
 
 int rf_plan_set_variant(rf_field_plan* P, int32_t variant) {
@@ -579,18 +672,18 @@ int rf_generate_grid(rf_context* ctx, const rf_params* p, const double* origin, 
     for (int a = 0; a < p->nDim && a < 3; a++) N *= nstep[a] > 0 ? nstep[a] : 0;
     if (N < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
     rf_field_plan* P = nullptr;
-    int rc = rf_plan_create(ctx, p, nullptr, phaseUniform, calculate, &P);
+    int rc = cached_plan(ctx, p, nullptr, phaseUniform, calculate, &P);
     if (rc) return rc;
-    std::unique_ptr<rf_field_plan, int (*)(rf_field_plan*)> guard(P, rf_plan_destroy);
     const size_t bytes = (size_t)N * p->Nmc * (outDtype == RF_DTYPE_F64 ? 8 : 4);
     void* d_out = nullptr;
-    if (hipMalloc(&d_out, bytes) != hipSuccess) return rf_set_error(ctx, RF_E_NOMEM, "output allocation failed");
+    rc = io_buffer(ctx, bytes, &d_out);
+    if (rc) return rc;
     rc = rf_plan_execute_grid(P, origin, step, nstep, d_out, outDtype, ctx->stream);
     hipError_t e = hipSuccess;
     if (rc == RF_OK) e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(d_out);
+    const hipError_t es = hipStreamSynchronize(ctx->stream);   // also when a launch failed: phU is the caller's
     if (rc) return rc;
+    if (e == hipSuccess) e = es;
     if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("rf_generate_grid: ") + hipGetErrorString(e));
     return RF_OK;
 }
@@ -601,27 +694,24 @@ int rf_generate(rf_context* ctx, const rf_params* p, const double* x, int64_t N,
     if (!ctx) return rf_set_error(nullptr, RF_E_ARG, "ctx is NULL");
     if (N < 0 || (N > 0 && (!x || !out))) return rf_set_error(ctx, RF_E_ARG, "bad point/output buffers");
     rf_field_plan* P = nullptr;
-    int rc = rf_plan_create(ctx, p, kUniform, phaseUniform, calculate, &P);
+    int rc = cached_plan(ctx, p, kUniform, phaseUniform, calculate, &P);
     if (rc) return rc;
-    std::unique_ptr<rf_field_plan, int (*)(rf_field_plan*)> guard(P, rf_plan_destroy);
     if (N == 0) return RF_OK;
     const size_t esz = (outDtype == RF_DTYPE_F64) ? 8 : 4;
-    double* d_x = nullptr;
-    void* d_out = nullptr;
-    RF_HIP_TRY(ctx, dmalloc(&d_x, (size_t)(N * p->nDim)));
-    hipError_t e = hipMalloc(&d_out, (size_t)N * p->Nmc * esz);
-    if (e != hipSuccess) {
-        dfree(d_x);
-        return rf_set_error(ctx, RF_E_NOMEM, "output allocation failed");
-    }
-    e = hipMemcpyAsync(d_x, x, (size_t)(N * p->nDim) * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) rc = rf_plan_execute(P, d_x, N, d_out, outDtype, ctx->stream);
-    if (e == hipSuccess && rc == RF_OK)
-        e = hipMemcpyAsync(out, d_out, (size_t)N * p->Nmc * esz, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dfree(d_x);
-    dfree(d_out);
+    // points, then the outputs at the next 256-byte boundary, in one staging buffer
+    const size_t xBytes = ((size_t)(N * p->nDim) * sizeof(double) + 255) & ~(size_t)255;
+    const size_t outBytes = (size_t)N * p->Nmc * esz;
+    void* d_io = nullptr;
+    rc = io_buffer(ctx, xBytes + outBytes, &d_io);
     if (rc) return rc;
+    double* d_x = static_cast<double*>(d_io);
+    void* d_out = static_cast<char*>(d_io) + xBytes;
+    hipError_t e = hipMemcpyAsync(d_x, x, (size_t)(N * p->nDim) * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) rc = rf_plan_execute(P, d_x, N, d_out, outDtype, ctx->stream);
+    if (e == hipSuccess && rc == RF_OK) e = hipMemcpyAsync(out, d_out, outBytes, hipMemcpyDeviceToHost, ctx->stream);
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    if (e == hipSuccess) e = es;
     if (e != hipSuccess) return rf_set_error(ctx, RF_E_HIP, std::string("rf_generate: ") + hipGetErrorString(e));
     return RF_OK;
 }

@@ -16,6 +16,18 @@
 #define RF_TWO_PI_EXACT 6.283185307179586476925286766559
 #define RF_INV_TWO_PI 0.15915494309189533576888376337251
 
+// A plan kept by the host-buffer entry points (rf_generate, rf_generate_grid)
+// for calls with the same parameters, wavevector draws and mask.
+struct rf_cached_plan {
+    rf_params p;
+    bool hasKU = false;            // caller wavevector uniforms (method 3)
+    std::vector<double> kU;
+    bool hasCalc = false;          // caller calculate mask
+    std::vector<uint8_t> calc;
+    rf_field_plan* plan = nullptr;
+    uint64_t lastUse = 0;
+};
+
 struct rf_context {
     int device = -1;
     hipStream_t stream = nullptr;
@@ -23,7 +35,12 @@ struct rf_context {
     int num_cu = 0;
     void* d_scratch = nullptr;   // reduction partials (grown on demand)
     size_t scratch_bytes = 0;
+    std::vector<rf_cached_plan> planCache;   // at most kPlanCache, least recently used evicted
+    uint64_t useClock = 0;
+    void* d_io = nullptr;        // host-buffer entry points' points and outputs (grown on demand)
+    size_t ioBytes = 0;
 };
+constexpr size_t kPlanCache = 4;
 
 // Host-side spectrum: the wavevector table in coordinates normalised by corrL.
 struct rf_spectrum {

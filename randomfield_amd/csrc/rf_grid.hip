@@ -194,14 +194,20 @@ __device__ __forceinline__ int64_t grp_row(const rf_gemm_desc& g, int m) {
 // 16-row groups per wave share each B fragment read); 3 64x128 real part
 // (two 32-column tiles per wave share each A fragment read; A holds only the
 // Re and -Im planes); 4 64x128 complex product (four 32-column tiles per
-// wave share each A fragment read)
+// wave share each A fragment read); 5 64x160 complex product (a column axis
+// of 129..160 in one tile: each row slab of A is staged once, not once per
+// 64 columns with the last tile nearly empty; five 32-column tiles per wave
+// share each A fragment read)
+constexpr int kTN5 = 160;
 template <int RW>
 struct HStage {
     _Float16 a[RW == 3 ? 4 : 6][kTM * (RW == 2 ? 2 : 1)][kHP];   // Re hi, Re lo, Im hi, Im lo, -Im hi, -Im lo
-    _Float16 b[4][kTN * (RW >= 3 ? 2 : 1)][kHP];                  // Re hi, Re lo, Im hi, Im lo
+    _Float16 b[4][RW == 5 ? kTN5 : kTN * (RW >= 3 ? 2 : 1)][kHP];  // Re hi, Re lo, Im hi, Im lo
 };
 __host__ __device__ inline int tile_m(const rf_gemm_desc& g) { return g.rw == 2 ? 2 * kTM : kTM; }
-__host__ __device__ inline int tile_n(const rf_gemm_desc& g) { return g.rw >= 3 ? 2 * kTN : kTN; }
+__host__ __device__ inline int tile_n(const rf_gemm_desc& g) {
+    return g.rw == 5 ? kTN5 : (g.rw == 3 || g.rw == 4) ? 2 * kTN : kTN;
+}
 
 // v -> FP16 hi (round to nearest) and lo = v - hi: packed conversions for
 // hi and lo, the residuals by mixed-precision FMAs reading the f16 halves in
@@ -220,6 +226,18 @@ __device__ inline void split4(const float* v, hv4* hi, hv4* lo) {
         (*lo)[i] = (_Float16)l0;
         (*lo)[i + 1] = (_Float16)l1;
     }
+}
+
+// split4 for two values
+typedef _Float16 hv2 __attribute__((ext_vector_type(2)));
+__device__ inline void split2(const float* v, hv2* hi, hv2* lo) {
+    const hv2 h = {(_Float16)v[0], (_Float16)v[1]};
+    const unsigned hb = __builtin_bit_cast(unsigned, h);
+    float l0, l1;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[0,0,0] op_sel_hi:[1,0,0]" : "=v"(l0) : "v"(hb), "v"(v[0]));
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(l1) : "v"(hb), "v"(v[1]));
+    *hi = h;
+    *lo = hv2{(_Float16)l0, (_Float16)l1};
 }
 
 // -x of FP16 values: the sign bits flipped
@@ -523,6 +541,139 @@ __device__ __forceinline__ void gemm_tile_c128(const rf_gemm_desc& g, int m0, in
         }
 }
 
+// 64x160 tiles of the complex product (the first contraction when its
+// column axis, the first grid axis, has 129..160 points): wave w owns complex
+// rows 16 w.. and the five 32-column tiles, reading each A fragment once for
+// all five; B (the phase table, small) is staged for all 160 columns.
+// Staging, operand split and epilogue as gemm_tile_c128.
+__device__ __forceinline__ void gemm_tile_c160(const rf_gemm_desc& g, int m0, int n0, int batch, HStage<5>* st) {
+    constexpr int NC = kTN5 / 32;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)((batch / g.aDiv) % g.aMod) * g.aStrideB + (int64_t)m0 * g.aSm);
+    const int64_t bCol0 = col_b(g, n0);
+    const gf2* B = (const gf2*)(g.B + (int64_t)batch * g.bStrideB + bCol0);
+    const int aSm = (int)g.aSm, aSk = (int)g.aSk, bSk = (int)g.bSk;
+    f32x16 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[c][i] = 0.0f;
+    // staging: A row t/4, k 4(t%4)..+3; B columns t%128 (k 8(t/128)..+7) and
+    // 128 + t%32 (k 2(t/32)..+1)
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 127, bK = (tid >> 7) * 8;
+    const int bCol2 = 128 + (tid & 31), bK2 = (tid >> 5) * 2;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N, bIn2 = n0 + bCol2 < g.N;
+    const int aOff = aRow * aSm;
+    const int bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
+    const int bOff2 = bIn2 ? (int)(col_b(g, n0 + bCol2) - bCol0) : 0;
+    const int kLast = g.K - 1;
+    f2v ra[4], rb[8], rb2[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i;
+            ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kb = k0 + bK + i;
+            rb[i] = (bIn && kb <= kLast) ? B[kb * bSk + bOff] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int kb = k0 + bK2 + i;
+            rb2[i] = (bIn2 && kb <= kLast) ? B[kb * bSk + bOff2] : f2v{0.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = ra[i].x;
+            im[i] = ra[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[1][aRow][aK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = lo;
+        *reinterpret_cast<hv4*>(&st->a[4][aRow][aK]) = neg4(hi);   // split(-x) = -split(x)
+        *reinterpret_cast<hv4*>(&st->a[5][aRow][aK]) = neg4(lo);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i] = rb[4 * h + i].x;
+                im[i] = rb[4 * h + i].y;
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[0][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[1][bCol][bK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[2][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[3][bCol][bK + 4 * h]) = lo;
+        }
+        float re2[2] = {rb2[0].x, rb2[1].x}, im2[2] = {rb2[0].y, rb2[1].y};
+        hv2 hi2, lo2;
+        split2(re2, &hi2, &lo2);
+        *reinterpret_cast<hv2*>(&st->b[0][bCol2][bK2]) = hi2;
+        *reinterpret_cast<hv2*>(&st->b[1][bCol2][bK2]) = lo2;
+        split2(im2, &hi2, &lo2);
+        *reinterpret_cast<hv2*>(&st->b[2][bCol2][bK2]) = hi2;
+        *reinterpret_cast<hv2*>(&st->b[3][bCol2][bK2]) = lo2;
+    };
+    const int isIm = l32 >> 4;
+    const int aPlane = isIm ? (half ? 0 : 2) : (half ? 4 : 0);
+    const int aRowL = wave * 16 + (l32 & 15);
+    const int bPlane = half ? 2 : 0;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + wave * 16 < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+                const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    if (n0 + 32 * c >= g.N) break;                                 // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][32 * c + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][32 * c + l32][kk]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dCol0 = col_d(g, n0);
+    const int64_t dBase = (int64_t)batch * g.dStrideB + (int64_t)m0 * g.dSm + dCol0;
+    const int dSm = (int)g.dSm;
+    gf2o* D = (gf2o*)(static_cast<float2*>(g.D) + dBase);
+    int oc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) oc[c] = n0 + 32 * c + l32 < g.N ? (int)(col_d(g, n0 + 32 * c + l32) - dCol0) : 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int r = wave * 16 + 8 * (i >> 2) + 4 * half + (i & 3);   // Re row r, Im row r + 16
+        if (m0 + r >= g.M) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if (n0 + 32 * c + l32 >= g.N) continue;
+            D[r * dSm + oc[c]] = f2v{acc[c][i] * os, acc[c][i + 8] * os};
+        }
+    }
+}
+
 // 64x128 tiles of the real part (the last contraction, whose column axis is
 // the long (x2, x1) plane): wave w owns rows 32 (w / 2).. and the two 32-column
 // tiles from 64 (w % 2), reading each A fragment once for both (6 fragment
@@ -794,6 +945,9 @@ __device__ __forceinline__ void run_tile(const rf_gemm_desc& g, int m0, int n0, 
     } else if constexpr (RW == 4) {
         static_assert(!RealOnly, "kind 4 is the complex product");
         gemm_tile_cw(g, m0, n0, batch, reinterpret_cast<HStage<4>*>(smem));
+    } else if constexpr (RW == 5) {
+        static_assert(!RealOnly, "kind 5 is the complex product");
+        gemm_tile_c160(g, m0, n0, batch, reinterpret_cast<HStage<5>*>(smem));
     } else if (g.split16) {
         gemm_tile_h<RealOnly>(g, m0, n0, batch, reinterpret_cast<HStage<1>*>(smem));
     } else {
@@ -834,8 +988,7 @@ __device__ inline int64_t xcd_tile(int64_t b, int64_t nTiles) {
 }
 
 template <bool RealOnly, int RW>
-__global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
-    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RW>];
+__device__ __forceinline__ void cgemm_body(const rf_gemm_desc& g, int64_t nTiles, char* smem) {
     const int64_t L = xcd_tile(blockIdx.x, nTiles);
     if (L >= nTiles) return;
     int mI, nI, batch;
@@ -843,12 +996,18 @@ __global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
     run_tile<RealOnly, RW>(g, mI * tile_m(g), nI * tile_n(g), batch, smem);
 }
 
+template <bool RealOnly, int RW>
+__global__ __launch_bounds__(256) void k_cgemm(rf_gemm_desc g, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RW>];
+    cgemm_body<RealOnly, RW>(g, nTiles, smem);
+}
+
 // Grouped form: the GEMMs of several fields in one launch; GEMM q owns
 // logical tiles tile0 .. tile0 + tiles(q) - 1.
 // RWMAX > 1: the GEMMs of the group with rw = RWMAX use that tile kind
 template <bool RealOnly, int RWMAX>
-__global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
-    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RWMAX>];
+__device__ __forceinline__ void cgemm_group_body(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles,
+                                                 char* smem) {
     const int64_t b = xcd_tile(blockIdx.x, nTiles);
     if (b >= nTiles) return;
     int lo = 0, hi = nG - 1;
@@ -862,6 +1021,27 @@ __global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restr
     tile_coords(g, b - g.tile0, &mI, &nI, &batch);
     if (RWMAX > 1 && g.rw == RWMAX) run_tile<RealOnly, RWMAX>(g, mI * tile_m(g), nI * tile_n(g), batch, smem);
     else run_tile<RealOnly, 1>(g, mI * kTM, nI * kTN, batch, smem);
+}
+
+template <bool RealOnly, int RWMAX>
+__global__ __launch_bounds__(256) void k_cgemm_group(const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<RWMAX>];
+    cgemm_group_body<RealOnly, RWMAX>(gs, nG, nTiles, smem);
+}
+
+// Kind 5 holds 80 accumulators per lane: asked for three waves per SIMD (its
+// LDS allows three blocks per CU) the compiler keeps them in the VGPRs
+// (160 registers, no spill) instead of 94 VGPRs + 80 AGPRs, which would
+// allow two.  The other kinds keep their own allocation.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_cgemm_c160(rf_gemm_desc g,
+                                                                                         int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<5>];
+    cgemm_body<false, 5>(g, nTiles, smem);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_cgemm_group_c160(
+    const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kStageBytes<5>];
+    cgemm_group_body<false, 5>(gs, nG, nTiles, smem);
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -1045,6 +1225,7 @@ hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s) {
     else if (g.realOut) hipLaunchKernelGGL((k_cgemm<true, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else if (g.rw == 2) hipLaunchKernelGGL((k_cgemm<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else if (g.rw == 4) hipLaunchKernelGGL((k_cgemm<false, 4>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else if (g.rw == 5) hipLaunchKernelGGL(k_cgemm_c160, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     else hipLaunchKernelGGL((k_cgemm<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
 }
@@ -1062,6 +1243,8 @@ hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, i
         hipLaunchKernelGGL((k_cgemm_group<false, 2>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     else if (rwMax == 4)
         hipLaunchKernelGGL((k_cgemm_group<false, 4>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
+    else if (rwMax == 5)
+        hipLaunchKernelGGL(k_cgemm_group_c160, dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     else
         hipLaunchKernelGGL((k_cgemm_group<false, 1>), dim3((unsigned)blocks), dim3(256), 0, s, d_gs, nG, nTiles);
     return hipGetLastError();
@@ -1154,7 +1337,9 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     g.B = E[0]; g.bSk = n[0]; g.bSn = 1; g.bStrideB = 0;     // K (h, j1): variant h of E1
     g.D = V; g.dSm = n[0]; g.dSn = 1; g.dStrideB = 0;
     g.M = (int)rows1; g.N = n[0]; g.K = K1; g.batches = 1; g.realOut = 0;
-    g.rw = rows1 >= 128 ? 2 : 1;              // 128-row tiles over the long row axis
+    // 128-row tiles over the long row axis; a first axis of 129..160 points
+    // in one 160-column tile
+    g.rw = n[0] > 2 * kTN && n[0] <= kTN5 ? 5 : rows1 >= 128 ? 2 : 1;
     g.outScale = 1.0f; g.split16 = split;
     const float2* last = V;
     int64_t plane = n[0];                      // points already resolved
@@ -1193,7 +1378,7 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
     for (int i = 0; i < st->n; i++) {
         rf_gemm_desc& q = st->g[i];
         const int64_t lim = (int64_t)1 << 30;
-        const int64_t tm = q.rw == 2 ? 128 : 64, tn = q.rw >= 3 ? 128 : 64;
+        const int64_t tm = tile_m(q), tn = tile_n(q);
         if ((int64_t)q.K * q.aSk + tm * q.aSm >= lim || (int64_t)q.K * q.bSk + tn * q.bSn + q.bSnHi >= lim ||
             tm * q.dSm + tn * q.dSn + q.dSnHi >= lim)
             for (int j = 0; j < st->n; j++) {

@@ -50,6 +50,11 @@ def grid_case(ctx, nstep, step, origin, corrL, Nmc, seed, dtype=np.float32, marg
     ((129, 5, 3), (0.2, 0.4, 0.5), (1.0, 1.0, 1.0)),
     # x2 = 140: partial 64-row tiles of the middle contraction
     ((7, 140, 9), (0.5, 0.1, 0.4), (1.0, 1.0, 1.0)),
+    # x1 of 129..160 runs the first contraction on 160-column tiles: a full
+    # tile, a short last column group, and the 2-D form
+    ((160, 6, 4), (0.1, 0.4, 0.5), (1.0, 1.0, 1.0)),
+    ((137, 70, 5), (0.15, 0.2, 0.5), (1.1, 0.9, 1.0)),
+    ((150, 23), (0.1, 0.3), (1.0, 1.2)),
 ])
 def test_grid_form_matches_oracle(ctx, nstep, step, corrL):
     d = len(nstep)

@@ -348,11 +348,15 @@ int rf_grid_batch_execute(rf_grid_batch* batch, void* stream);
  * free, capturable; method 4 batches: RF_E_UNSUPPORTED. */
 #define RF_DRAW_NEXT (-1)
 /* Times each contraction stage of the batch with the wide tile kinds the
- * set-up chose and with 64 x 64 tiles on every GEMM of the stage, keeps the
- * faster (chosen[3]: the widest kind now used per stage, 1 = 64 x 64) and
+ * set-up chose, with 128 x 64 tiles where it chose a 64 x 160 tile spanning a
+ * whole 129..160-point axis, and with 64 x 64 tiles on every GEMM of the
+ * stage, keeps the fastest (chosen[3]: the kind now used per stage, 1 =
+ * 64 x 64, 2 128x64, 3 64x128 real part, 4 64x128, 5 64x160) and
  * leaves one execute's result in the outputs.  The field values do not depend
  * on the tile kind (the same K order per element).  Synchronous; no
- * counterpart in the reference. */
+ * counterpart in the reference.  This and rf_grid_batch_fuse_blend rewrite
+ * the batch's device tables: a graph captured before either keeps its old
+ * launch shapes and must be captured again. */
 int rf_grid_batch_autotune(rf_grid_batch* batch, void* stream, int32_t* chosen);
 /* The blend fused into the batch's last contraction (FIXED mode, a
  * tensor-product layout whose fields are the batch's outputs; same element

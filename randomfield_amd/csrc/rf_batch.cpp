@@ -239,11 +239,19 @@ int rf_grid_batch_autotune(rf_grid_batch* B, void* stream, int32_t* chosen) {
         bool wide = false;
         for (int r : B->kind0[k]) wide |= r > 1;
         if (!wide) continue;
-        // candidates: the set-up's wide kinds, and 64 x 64 tiles everywhere
+        // candidates: the set-up's wide kinds, 128x64 tiles where it chose
+        // the 64x160 whole-axis tile, and 64 x 64 tiles everywhere
+        auto kind_of = [&](int opt, int i) {
+            const int r = B->kind0[k][i];
+            return opt == 0 ? r : opt == 1 ? (r == 5 ? 2 : r) : 1;
+        };
+        bool narrower = false;
+        for (int r : B->kind0[k]) narrower |= r == 5;
         float best = 0.0f;
         int bestOpt = 0;
-        for (int opt = 0; opt < 2 && e == hipSuccess; opt++) {
-            for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = opt == 0 ? B->kind0[k][i] : 1;
+        for (int opt = 0; opt < 3 && e == hipSuccess; opt++) {
+            if (opt == 1 && !narrower) continue;
+            for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = kind_of(opt, (int)i);
             e = upload_stages(B, s);
             float ms = 0.0f;
             if (e == hipSuccess) e = time_stage(k, &ms);
@@ -252,7 +260,7 @@ int rf_grid_batch_autotune(rf_grid_batch* B, void* stream, int32_t* chosen) {
                 bestOpt = opt;
             }
         }
-        for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = bestOpt == 0 ? B->kind0[k][i] : 1;
+        for (size_t i = 0; i < B->stage[k].size(); i++) B->stage[k][i].rw = kind_of(bestOpt, (int)i);
     }
     if (e == hipSuccess) e = upload_stages(B, s);
     (void)hipEventDestroy(e0);

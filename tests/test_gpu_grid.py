@@ -55,6 +55,10 @@ def grid_case(ctx, nstep, step, origin, corrL, Nmc, seed, dtype=np.float32, marg
     ((160, 6, 4), (0.1, 0.4, 0.5), (1.0, 1.0, 1.0)),
     ((137, 70, 5), (0.15, 0.2, 0.5), (1.1, 0.9, 1.0)),
     ((150, 23), (0.1, 0.3), (1.0, 1.2)),
+    # a last axis of 129..160 points: the last contraction's short last row tile
+    ((16, 8, 150), (0.3, 0.4, 0.1), (1.0, 1.0, 1.0)),
+    ((24, 9, 160), (0.25, 0.3, 0.1), (0.9, 1.0, 1.1)),
+    ((100, 140), (0.1, 0.1), (1.0, 0.9)),
 ])
 def test_grid_form_matches_oracle(ctx, nstep, step, corrL):
     d = len(nstep)
@@ -339,14 +343,21 @@ def test_grid_batch_redraw(ctx):
         pl.close()
 
 
-def test_grid_batch_autotune_and_fused_blend(ctx):
+@pytest.mark.parametrize("groupN,parts,nOvl", [
+    ((40, 36, 20), (3, 2, 2), 5),
+    # subdomains of 131/132 points along x3 (a short last row tile in the last
+    # contraction) and along x1 (64x160 tiles in the first)
+    ((40, 36, 256), (2, 2, 2), 7),
+    ((256, 20, 16), (2, 1, 2), 7),
+])
+def test_grid_batch_autotune_and_fused_blend(ctx, groupN, parts, nOvl):
     """The tile autotuner and the blend fused into the last contraction leave
     every group value and every overlap value of the fields unchanged, bit for
     bit; a fused graph with fresh draws matches the eager fused result; the
     blend plan refuses another group buffer while fused; unfusing restores
     the separate blend."""
     import torch
-    groupN, parts, nOvl, step, Nmc = (40, 36, 20), (3, 2, 2), 5, (0.25, 0.25, 0.25), 3
+    step, Nmc = (0.25, 0.25, 0.25), 3
     origins, sizes, masks = W.cfg4_layout(groupN, parts, nOvl)
     specs = []
     for s in range(len(sizes)):
@@ -363,7 +374,7 @@ def test_grid_batch_autotune_and_fused_blend(ctx):
     ref_f = [o.cpu().numpy().copy() for o in outs]
     chosen = batch.autotune()
     print(f"[grid] autotune chose tile kinds {chosen}", flush=True)
-    assert len(chosen) == 3 and all(c in (1, 2, 3, 4) for c in chosen)
+    assert len(chosen) == 3 and all(c in (1, 2, 3, 4, 5) for c in chosen)
     g.fill_(-7.0)
     batch.execute()
     bp.execute(g)

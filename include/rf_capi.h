@@ -362,10 +362,13 @@ int rf_grid_batch_autotune(rf_grid_batch* batch, void* stream, int32_t* chosen);
  * tensor-product layout whose fields are the batch's outputs; same element
  * type): points that one field alone covers -- weight 1 -- are stored by the
  * last contraction straight into d_group, the others into the field as
- * before, and rf_blend_plan_execute then forms only the points several
- * fields cover.  The group is the same as without fusion, bit for bit; the
- * field buffers then hold only the points several fields cover (their
- * single-cover interior is neither written nor read).  execute of the blend
+ * before -- except the overlap bands along the first axis, which go to
+ * arrays the blend plan allocates here, row by row, so that the blend reads
+ * them as whole cache lines -- and rf_blend_plan_execute then forms only the
+ * points several fields cover.  The group is the same as without fusion,
+ * bit for bit; the field buffers then hold only the points of the overlaps
+ * along the other axes (the single-cover interior and the first-axis bands
+ * are neither written nor read there).  execute of the blend
  * plan must then be given d_group; d_group NULL undoes the fusion, which a
  * caller that destroys the batch and keeps using the blend plan must do
  * first (a fused blend plan forms the overlaps only).  Synchronous set-up;

@@ -16,6 +16,7 @@ Mirror of the reference surface:
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -125,17 +126,30 @@ class Context:
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         self.device = int(device)
+        self._children = []            # weak references to the objects made on this context
         check(lib().rf_context_create(C.byref(self._h), device))
 
     @property
     def handle(self) -> C.c_void_p:
         return self._h
 
+    def _adopt(self, obj) -> None:
+        """Registers an object whose native handle lives on this context, so
+        that close() releases it first (a native object must not outlive its
+        context: its destroy call uses the context's device and stream)."""
+        self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
+
     def sync(self) -> None:
         check(lib().rf_context_sync(self._h), self._h)
 
     def close(self) -> None:
         if self._h:
+            for r in reversed(self._children):      # newest first
+                obj = r()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             lib().rf_context_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -164,6 +178,7 @@ class FieldPlan:
         c = None if calc is None else np.ascontiguousarray(calc, dtype=np.uint8)
         check(lib().rf_plan_create(ctx.handle, C.byref(params), _ptr(kU), _ptr(phU), _ptr(c),
                                    C.byref(self._h)), ctx.handle)
+        ctx._adopt(self)
 
     @property
     def info(self) -> SpectrumInfo:
@@ -315,6 +330,7 @@ class BlendPlan:
         check(lib().rf_blend_plan_create(ctx.handle, len(groupN), _ptr(g), Nmc, len(self._fields), _ptr(o), _ptr(s),
                                          _ptr(n), _ptr(m), mode, C.cast(ptrs, C.c_void_p), dtype_code,
                                          C.byref(self._h)), ctx.handle)
+        ctx._adopt(self)
 
     def execute(self, d_group, stream=None) -> None:
         check(lib().rf_blend_plan_execute(self._h, _ptr(d_group), _ptr(stream)), self.ctx.handle)
@@ -350,6 +366,7 @@ class GridBatch:
         self._h = C.c_void_p()
         check(lib().rf_grid_batch_create(ctx.handle, n, C.cast(pp, C.c_void_p), _ptr(o), _ptr(st), _ptr(ns),
                                          C.cast(op, C.c_void_p), dtype_code, C.byref(self._h)), ctx.handle)
+        ctx._adopt(self)
 
     def execute(self, stream=None) -> None:
         check(lib().rf_grid_batch_execute(self._h, _ptr(stream)), self.ctx.handle)
@@ -398,6 +415,8 @@ class Graph:
 
     def __exit__(self, et, ev, tb):
         rc = lib().rf_graph_capture_end(self.ctx.handle, _ptr(self.stream), C.byref(self._h))
+        if self._h:
+            self.ctx._adopt(self)
         if et is None:
             check(rc, self.ctx.handle)
 

@@ -296,9 +296,17 @@ int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) 
         P->d_boxes = nullptr;
         P->nBoxes = 0;
         P->boxPoints = 0;
+        if (P->d_subFused) (void)hipFree(P->d_subFused);
+        P->d_subFused = nullptr;
+        if (P->d_bands) (void)hipFree(P->d_bands);
+        P->d_bands = nullptr;
+        P->sep.sub = P->subPlain;
     };
     if (!d_group) {                                    // back to the separate blend
-        for (auto& g : last) g.grp = nullptr;
+        for (auto& g : last) {
+            g.grp = nullptr;
+            g.bandLo = g.bandHi = nullptr;
+        }
         P->sep.skipSingle = 0;
         P->fusedGroup = nullptr;
         drop_boxes();
@@ -349,6 +357,55 @@ int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) 
         g.grpCol0 = o[0] + (d == 3 ? (int64_t)P->groupN[0] * o[1] : 0);
         g.grpRow0 = (int64_t)o[d - 1] * planeG;
         g.grpPlaneG = planeG;
+        g.grpNy = d == 3 ? n[1] : 1;
+        g.bandLo = g.bandHi = nullptr;
+        g.bandLoStrideB = (int64_t)g.grpNy * n[d - 1] * xl;            // realization strides
+        g.bandHiStrideB = (int64_t)g.grpNy * n[d - 1] * (n[0] - xh);
+    }
+    // the first-axis overlap bands of every field in arrays of their own,
+    // [Nmc][rows][band width], 256-byte aligned
+    const size_t esz = P->dtype == RF_DTYPE_F64 ? 8 : 4;
+    std::vector<size_t> offLo(fused.size()), offHi(fused.size());
+    size_t bandBytes = 0;
+    auto take = [&](int64_t elems) {
+        const size_t at = bandBytes;
+        bandBytes += ((size_t)elems * esz + 255) & ~(size_t)255;
+        return at;
+    };
+    for (size_t i = 0; i < fused.size(); i++) {
+        offLo[i] = take(fused[i].bandLoStrideB * fused[i].batches);
+        offHi[i] = take(fused[i].bandHiStrideB * fused[i].batches);
+    }
+    char* d_bands = nullptr;
+    if (bandBytes && hipMalloc((void**)&d_bands, bandBytes) != hipSuccess)
+        return rf_set_error(ctx, RF_E_NOMEM, "fused blend band arrays");
+    for (size_t i = 0; i < fused.size(); i++) {
+        if (fused[i].bandLoStrideB > 0) fused[i].bandLo = d_bands + offLo[i];
+        if (fused[i].bandHiStrideB > 0) fused[i].bandHi = d_bands + offHi[i];
+    }
+    // the blend's subdomain table with the band arrays
+    const int64_t nCells = (int64_t)P->sep.nInt[0] * P->sep.nInt[1] * P->sep.nInt[2];
+    std::vector<rf_blend_sub> subs((size_t)nCells);
+    rf_blend_sub* d_subFused = nullptr;
+    hipError_t es = hipMemcpy(subs.data(), P->subPlain, subs.size() * sizeof(rf_blend_sub), hipMemcpyDeviceToHost);
+    for (auto& S : subs) {
+        for (const auto& g : fused)
+            if (S.field && S.field == g.D) {
+                S.bandLo = g.bandLo;
+                S.bandHi = g.bandHi;
+                S.bandLoTot = g.bandLoStrideB;
+                S.bandHiTot = g.bandHiStrideB;
+                S.wLo = g.grpXlo;
+                S.wHi = g.grpNx - g.grpXhi;
+            }
+    }
+    if (es == hipSuccess) es = hipMalloc((void**)&d_subFused, subs.size() * sizeof(rf_blend_sub));
+    if (es == hipSuccess)
+        es = hipMemcpy(d_subFused, subs.data(), subs.size() * sizeof(rf_blend_sub), hipMemcpyHostToDevice);
+    if (es != hipSuccess) {
+        if (d_subFused) (void)hipFree(d_subFused);
+        if (d_bands) (void)hipFree(d_bands);
+        return rf_set_error(ctx, RF_E_HIP, std::string("fused blend tables: ") + hipGetErrorString(es));
     }
     // the overlap bands: per axis, each run of group indices two intervals cover
     std::vector<rf_blend_box> boxes;
@@ -372,26 +429,38 @@ int rf_grid_batch_fuse_blend(rf_grid_batch* B, rf_blend_plan* P, void* d_group) 
         }
     }
     rf_blend_box* d_boxes = nullptr;
+    auto free_new = [&]() {
+        if (d_boxes) (void)hipFree(d_boxes);
+        (void)hipFree(d_subFused);
+        if (d_bands) (void)hipFree(d_bands);
+    };
     if (!boxes.empty()) {
-        if (hipMalloc((void**)&d_boxes, boxes.size() * sizeof(rf_blend_box)) != hipSuccess)
+        if (hipMalloc((void**)&d_boxes, boxes.size() * sizeof(rf_blend_box)) != hipSuccess) {
+            free_new();
             return rf_set_error(ctx, RF_E_NOMEM, "fused blend boxes");
+        }
         const hipError_t e = hipMemcpy(d_boxes, boxes.data(), boxes.size() * sizeof(rf_blend_box),
                                        hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            (void)hipFree(d_boxes);
+            free_new();
             return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
         }
     }
+    const std::vector<rf_gemm_desc> before = last;
     last = fused;
     const hipError_t e = upload_stages(B, s);
     if (e != hipSuccess) {
-        if (d_boxes) (void)hipFree(d_boxes);
+        last = before;
+        free_new();
         return rf_set_error(ctx, RF_E_HIP, hipGetErrorString(e));
     }
     drop_boxes();
     P->d_boxes = d_boxes;
     P->nBoxes = (int)boxes.size();
     P->boxPoints = first;
+    P->d_subFused = d_subFused;
+    P->d_bands = d_bands;
+    P->sep.sub = d_subFused;
     P->sep.skipSingle = 1;
     P->fusedGroup = d_group;
     return RF_OK;

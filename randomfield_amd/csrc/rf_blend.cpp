@@ -317,6 +317,7 @@ extern "C" int rf_blend_plan_create(rf_context* ctx, int32_t nDim, const int32_t
         }
         q.groupTot = gTot;
         q.sub = reinterpret_cast<const rf_blend_sub*>(d_blob + oSubT);
+        P->subPlain = q.sub;
     }
     *out = P.release();
     return RF_OK;
@@ -344,6 +345,8 @@ extern "C" int rf_blend_plan_destroy(rf_blend_plan* P) {
     (void)hipSetDevice(P->ctx->device);
     (void)hipFree(P->d_blob);          // the caller has no launch of this plan pending
     if (P->d_boxes) (void)hipFree(P->d_boxes);
+    if (P->d_subFused) (void)hipFree(P->d_subFused);
+    if (P->d_bands) (void)hipFree(P->d_bands);
     delete P;
     return RF_OK;
 }

@@ -55,6 +55,12 @@ struct rf_blend_axis {
 struct rf_blend_sub {
     const void* field;            // NULL: no subdomain on this interval triple
     int64_t subTot;
+    // fused with a grid batch: the first-axis overlap bands of the field,
+    // [Nmc][n2][n1][wLo] and [Nmc][n2][n1][wHi] (NULL: none)
+    const void* bandLo;
+    const void* bandHi;
+    int64_t bandLoTot, bandHiTot;   // realization strides
+    int32_t wLo, wHi;
 };
 struct rf_blend_sep {
     int nDim, Nmc, mode;
@@ -94,6 +100,9 @@ struct rf_blend_plan {
     std::vector<int32_t> cover[3];            // [g][2]: intervals covering group index g
     void* fusedGroup = nullptr;               // the group a fused batch writes
     rf_blend_box* d_boxes = nullptr;          // fused: the overlap bands
+    const rf_blend_sub* subPlain = nullptr;   // the sub table without band arrays
+    rf_blend_sub* d_subFused = nullptr;       // fused: the sub table with them
+    void* d_bands = nullptr;                  // fused: the band arrays of every field
     int nBoxes = 0;
     int64_t boxPoints = 0;
 };

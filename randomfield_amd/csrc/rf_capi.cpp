@@ -117,13 +117,13 @@ int rf_context_create(rf_context** out, int device) {
 int rf_context_destroy(rf_context* ctx) {
     if (!ctx) return RF_OK;
     (void)hipSetDevice(ctx->device);
+    for (auto& c : ctx->planCache) rf_plan_destroy(c.plan);   // they synchronize on the stream
+    ctx->planCache.clear();
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
     dfree(ctx->d_scratch);
-    for (auto& c : ctx->planCache) rf_plan_destroy(c.plan);
-    ctx->planCache.clear();
     dfree(ctx->d_io);
     delete ctx;
     return RF_OK;

@@ -189,6 +189,22 @@ __device__ __forceinline__ int64_t grp_col(const rf_gemm_desc& g, int n) {
 __device__ __forceinline__ int64_t grp_row(const rf_gemm_desc& g, int m) {
     return (m >= g.grpZlo && m < g.grpZhi) ? g.grpRow0 + g.grpPlaneG * (int64_t)m : (int64_t)-1;
 }
+// Fused blend: plane column n inside a first-axis overlap band sits in the
+// band array of its side (*side 0 low, 1 high) at element offset (return) +
+// m * (*rowStride) for row m; -1 outside the bands.
+// (selects, not early returns: the blend's lookup of the same bands was
+// miscompiled in an if-return form)
+__device__ __forceinline__ int64_t band_col(const rf_gemm_desc& g, int n, int batch, int* side, int* rowStride) {
+    const int x = n % g.grpNx, y = n / g.grpNx;
+    const int wLo = g.grpXlo, wHi = g.grpNx - g.grpXhi;
+    const bool lo = g.bandLo != nullptr && x < g.grpXlo;
+    const bool hi = !lo && g.bandHi != nullptr && x >= g.grpXhi;
+    const int64_t offLo = (int64_t)batch * g.bandLoStrideB + (int64_t)y * wLo + x;
+    const int64_t offHi = (int64_t)batch * g.bandHiStrideB + (int64_t)y * wHi + (x - g.grpXhi);
+    *side = hi ? 1 : 0;
+    *rowStride = lo ? g.grpNy * wLo : hi ? g.grpNy * wHi : 0;
+    return lo ? offLo : hi ? offHi : (int64_t)-1;
+}
 
 // Tile kinds (rf_gemm_desc::rw): 1 64x64; 2 128x64 complex product (two
 // 16-row groups per wave share each B fragment read); 3 64x128 real part
@@ -381,9 +397,12 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         const int nn = ct * 32 + l32;
         const bool nIn = n0 + nn < g.N;
         const int oc = dcol(nn);
-        // fused blend: this lane's column offset in the group grid, or -1
+        // fused blend: this lane's column offset in the group grid, or -1,
+        // and in a first-axis band array, or -1
         const int64_t gc = (g.grp && nIn) ? grp_col(g, n0 + nn) : -1;
-        auto store = [&](auto map, auto* D, auto* G) {
+        int bside = 0, bst = 0;
+        const int64_t bo = (g.grp && nIn && gc < 0) ? band_col(g, n0 + nn, batch, &bside, &bst) : -1;
+        auto store = [&](auto map, auto* D, auto* G, auto* BL, auto* BH) {
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
@@ -391,6 +410,7 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
                     const auto v = map(acc0[i] * os);
                     const int64_t gr = gc >= 0 ? grp_row(g, m0 + r) : -1;
                     if (gr >= 0) G[gr + gc] = v;
+                    else if (bo >= 0) (bside ? BH : BL)[bo + (int64_t)(m0 + r) * bst] = v;
                     else D[r * dSm + oc] = v;
                 }
             }
@@ -400,14 +420,20 @@ __device__ __forceinline__ void gemm_tile_h(const rf_gemm_desc& g, int m0, int n
         gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
         gfo* G32 = (gfo*)(static_cast<float*>(g.grp) + (int64_t)batch * g.grpStrideB);
         gdo* G64 = (gdo*)(static_cast<double*>(g.grp) + (int64_t)batch * g.grpStrideB);
+        gfo* L32 = (gfo*)static_cast<float*>(g.bandLo);
+        gfo* H32 = (gfo*)static_cast<float*>(g.bandHi);
+        gdo* L64 = (gdo*)static_cast<double*>(g.bandLo);
+        gdo* H64 = (gdo*)static_cast<double*>(g.bandHi);
         if (g.outDtype == RF_DTYPE_F64) {
-            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64);
-            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64);
-            else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64);
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64, L64, H64);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL)
+                store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64, L64, H64);
+            else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64, L64, H64);
         } else {
-            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32);
-            else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32);
-            else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32);
+            if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32, L32, H32);
+            else if (g.epi.margi == RF_FOM_LOGNORMAL)
+                store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32, L32, H32);
+            else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32, L32, H32);
         }
     }
 }
@@ -776,10 +802,19 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     auto dcol = [&](int nn) { return n0 + nn < g.N ? (int)(col_d(g, n0 + nn) - dCol0) : 0; };
     const int oc[2] = {dcol(64 * ct + l32), dcol(64 * ct + 32 + l32)};
     const bool nIn[2] = {n0 + 64 * ct + l32 < g.N, n0 + 64 * ct + 32 + l32 < g.N};
-    // fused blend: the lane's column offsets in the group grid, or -1
-    const int64_t gc[2] = {(g.grp && nIn[0]) ? grp_col(g, n0 + 64 * ct + l32) : -1,
-                           (g.grp && nIn[1]) ? grp_col(g, n0 + 64 * ct + 32 + l32) : -1};
-    auto store = [&](auto map, auto* D, auto* G) {
+    // fused blend: the lane's column offsets in the group grid, or -1, and
+    // in a first-axis band array, or -1
+    int64_t gc[2], bo[2];
+    int bside[2], bst[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const int nn = n0 + 64 * ct + 32 * c + l32;
+        gc[c] = (g.grp && nIn[c]) ? grp_col(g, nn) : -1;
+        bside[c] = 0;
+        bst[c] = 0;
+        bo[c] = (g.grp && nIn[c] && gc[c] < 0) ? band_col(g, nn, batch, &bside[c], &bst[c]) : -1;
+    }
+    auto store = [&](auto map, auto* D, auto* G, auto* BL, auto* BH) {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
@@ -790,6 +825,7 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
                 if (!nIn[c]) continue;
                 const auto v = map(acc[c][i] * os);
                 if (gr >= 0 && gc[c] >= 0) G[gr + gc[c]] = v;
+                else if (bo[c] >= 0) (bside[c] ? BH : BL)[bo[c] + (int64_t)(m0 + r) * bst[c]] = v;
                 else D[r * dSm + oc[c]] = v;
             }
         }
@@ -799,14 +835,20 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
     gfo* G32 = (gfo*)(static_cast<float*>(g.grp) + (int64_t)batch * g.grpStrideB);
     gdo* G64 = (gdo*)(static_cast<double*>(g.grp) + (int64_t)batch * g.grpStrideB);
+    gfo* L32 = (gfo*)static_cast<float*>(g.bandLo);
+    gfo* H32 = (gfo*)static_cast<float*>(g.bandHi);
+    gdo* L64 = (gdo*)static_cast<double*>(g.bandLo);
+    gdo* H64 = (gdo*)static_cast<double*>(g.bandHi);
     if (g.outDtype == RF_DTYPE_F64) {
-        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64);
-        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64);
-        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64);
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64, G64, L64, H64);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL)
+            store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64, G64, L64, H64);
+        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64, G64, L64, H64);
     } else {
-        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32);
-        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32);
-        else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32);
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32, G32, L32, H32);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL)
+            store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32, G32, L32, H32);
+        else store([=](float u) { return fmaf(u, sc, sh); }, D32, G32, L32, H32);
     }
 }
 

@@ -242,6 +242,13 @@ struct rf_gemm_desc {
     void* grp;
     int64_t grpStrideB, grpRow0, grpPlaneG, grpCol0, grpG0;
     int grpNx, grpXlo, grpXhi, grpYlo, grpYhi, grpZlo, grpZhi;
+    // fused: the points of the first-axis overlap bands (x < grpXlo, the low
+    // band, x >= grpXhi, the high band) go to band arrays [batch][m][y][x']
+    // (x' = x, resp. x - grpXhi; grpNy plane rows) instead of to D, so the
+    // blend reads them as whole cache lines
+    void* bandLo; void* bandHi;
+    int64_t bandLoStrideB, bandHiStrideB;
+    int grpNy;
 };
 // The coefficient fold of one grid execution: Cf[k][m] for rows m (r, j3, q,
 // j2) and K index k (h, j1) from amp / phase [r][b][j] (rf_grid.hip).

@@ -383,6 +383,28 @@ __device__ __forceinline__ void blend_store(T* p, double u) {
     __builtin_nontemporal_store((T)u, p);   // written once, read by the next kernel
 }
 
+// Where field S holds its value at local position (p0, p1, p2) (s0, s1: its
+// first two extents): a first-axis overlap band array of a fused batch, or
+// the field itself; *st: the realization stride.
+// Written with selects: an if-return form of this lookup was compiled (ROCm
+// 7.2, gfx950) so that lanes leaving the high-band test for the field kept
+// the band width as their stride.
+template <typename T>
+__device__ __forceinline__ const T* blend_locate(const rf_blend_sub& S, int64_t p0, int64_t p1, int64_t p2,
+                                                 int64_t s0, int64_t s1, int64_t* st) {
+    const int64_t row = p1 + p2 * s1;
+    const int64_t wLo = S.wLo, wHi = S.wHi;
+    const int64_t tLo = S.bandLoTot, tHi = S.bandHiTot, tF = S.subTot;
+    const T* bLo = static_cast<const T*>(S.bandLo);
+    const T* bHi = static_cast<const T*>(S.bandHi);
+    const T* bF = static_cast<const T*>(S.field);
+    const bool lo = bLo != nullptr && p0 < wLo;
+    const bool hi = !lo && bHi != nullptr && p0 >= s0 - wHi;
+    const int64_t offField = p0 + row * s0, offLo = row * wLo + p0, offHi = row * wHi + (p0 - (s0 - wHi));
+    *st = lo ? tLo : (hi ? tHi : tF);
+    return lo ? bLo + offLo : (hi ? bHi + offHi : bF + offField);
+}
+
 // One group point (gi0, gi1, gi2) of the separable blend, every realization.
 template <typename T>
 __device__ __forceinline__ void blend_sep_point(const rf_blend_sep& b, int gi0, int gi1, int gi2,
@@ -425,10 +447,11 @@ __device__ __forceinline__ void blend_sep_point(const rf_blend_sep& b, int gi0, 
         const int64_t s0 = c0 ? A0.sz[1] : A0.sz[0], s1 = c1 ? A1.sz[1] : A1.sz[0];
         const int64_t p0 = c0 ? A0.pos[1] : A0.pos[0], p1 = c1 ? A1.pos[1] : A1.pos[0];
         const int64_t p2 = c2 ? A2.pos[1] : A2.pos[0];
-        const T* F = static_cast<const T*>(S.field) + p0 + p1 * s0 + p2 * s0 * s1;
+        int64_t st;
+        const T* F = blend_locate<T>(S, p0, p1, p2, s0, s1, &st);
         const double w = ((c0 ? A0.f[1] : A0.f[0]) * (c1 ? A1.f[1] : A1.f[0])) * (c2 ? A2.f[1] : A2.f[0]);
-        if (n == 0) { F0 = F; st0 = S.subTot; w0 = w; }
-        else { F1 = F; st1 = S.subTot; w1 = w; }
+        if (n == 0) { F0 = F; st0 = st; w0 = w; }
+        else { F1 = F; st1 = st; w1 = w; }
         n++;
     }
     if (b.skipSingle && n == 1) return;            // written by the fused last contraction
@@ -488,11 +511,12 @@ __device__ __forceinline__ void blend_sep_point(const rf_blend_sep& b, int gi0, 
             const int64_t s0 = c0 ? A0.sz[1] : A0.sz[0], s1 = c1 ? A1.sz[1] : A1.sz[0];
             const int64_t p0 = c0 ? A0.pos[1] : A0.pos[0], p1 = c1 ? A1.pos[1] : A1.pos[0];
             const int64_t p2 = c2 ? A2.pos[1] : A2.pos[0];
-            const T* F = static_cast<const T*>(S.field) + p0 + p1 * s0 + p2 * s0 * s1;
+            int64_t st;
+            const T* F = blend_locate<T>(S, p0, p1, p2, s0, s1, &st);
             const double w = ((c0 ? A0.f[1] : A0.f[0]) * (c1 ? A1.f[1] : A1.f[0])) * (c2 ? A2.f[1] : A2.f[0]);
             T v[RB];
 #pragma unroll
-            for (int j = 0; j < RB; j++) v[j] = F[(int64_t)min(r0 + j, Nmc - 1) * S.subTot];
+            for (int j = 0; j < RB; j++) v[j] = F[(int64_t)min(r0 + j, Nmc - 1) * st];
 #pragma unroll
             for (int j = 0; j < RB; j++) acc[j] = (double)v[j] * w + acc[j];
         }
@@ -521,7 +545,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
 // points one subdomain covers): boxes of group indices, box k covering
 // points first .. first + n0 n1 n2 - 1 of the launch, x fastest.
 template <typename T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_blend_boxes(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_blend_boxes(
         rf_blend_sep b, const rf_blend_box* __restrict__ boxes, int nBoxes, int64_t total, T* __restrict__ group) {
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e >= total) return;

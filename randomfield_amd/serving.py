@@ -29,6 +29,7 @@ class Server:
         cfg.batchTerms = float(batch_terms)
         self._h = C.c_void_p()
         check(lib().rf_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)), ctx.handle)
+        ctx._adopt(self)
         self._keep = {}
 
     def submit(self, x, method: int, Nmc: int, corrL, seed: int, margi: int = FOM_NONE,

@@ -1230,11 +1230,14 @@ __global__ __launch_bounds__(256) void k_coeffs_fold(rf_fold_desc f) {
 
 // Every field of a batch in one launch, phases drawn in place: field i owns
 // blocks block0 .. block0 + nbx * nby - 1 (ascending in i), block -> field by
-// binary search; realizations *draw * Nmc .. of the field's stream.
+// binary search; realizations *draw * Nmc .. of the field's stream.  Logical
+// blocks are placed per XCD (xcd_tile): each XCD folds a contiguous run of
+// fields, so a field's amplitudes are fetched into one L2, not eight.
 __global__ __launch_bounds__(256) void k_coeffs_fold_draw(const rf_fold_item* __restrict__ items, int n,
-                                                          const int64_t* __restrict__ draw) {
+                                                          const int64_t* __restrict__ draw, int64_t nBlocks) {
     __shared__ float2 tile[kFoldK][kFoldM + 1];
-    const int64_t blk = blockIdx.x;
+    const int64_t blk = xcd_tile(blockIdx.x, nBlocks);
+    if (blk >= nBlocks) return;
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -1446,8 +1449,9 @@ void rf_fold_item_init(rf_fold_item* it, const rf_fold_desc& f, int Nmc, uint64_
 hipError_t rf_launch_refold_draw(const rf_fold_item* d_items, int n, int64_t blocks, const int64_t* d_draw,
                                  hipStream_t s) {
     if (n <= 0 || blocks <= 0) return hipSuccess;
-    if (!d_items || !d_draw || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_coeffs_fold_draw, dim3((unsigned)blocks), dim3(256), 0, s, d_items, n, d_draw);
+    if (!d_items || !d_draw || xcd_blocks(blocks) > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_coeffs_fold_draw, dim3((unsigned)xcd_blocks(blocks)), dim3(256), 0, s, d_items, n, d_draw,
+                       blocks);
     return hipGetLastError();
 }
 

@@ -117,8 +117,9 @@ def run_cfg1(args):
 
 def run_cfg2(args):
     """cfg 2: isotropic, 128^3 grid spacing 0.1, corrL 0.5, lognormal (avg 1,
-    var 0.25), Nmc 64, seed 7; FP32 direct (per-realization directions) and
-    BF16 hi/lo MFMA (shared directions); FP32 output with the fused map."""
+    var 0.25), Nmc 64, seed 7; FP32 direct (per-realization directions), BF16
+    and FP16 hi/lo MFMA (shared directions), and the grid form (per-realization
+    directions, FP16 hi/lo MFMA); FP32 output with the fused map."""
     import numpy as np
     import torch
 
@@ -153,6 +154,18 @@ def run_cfg2(args):
         forms[name] = {"value": work / (wall * 1e-3), "ms_per_step": wall, "kernel_ms": float(ms.mean()),
                        **_err(got, ref)}
         plan.close()
+    # per-realization directions on the grid form: each term factorises over
+    # the axes, one real-part split-FP16 MFMA GEMM per realization over the
+    # shells, operands formed from per-axis phase tables
+    plan = R.FieldPlan(ctx, R.make_params(form=R.FORM_FP32, isoShared=0, **base), None, U)
+    out = torch.empty((Nmc, N), dtype=torch.float32, device="cuda")
+    wall, ms = _timed(lambda: plan.execute_grid((0.0, 0.0, 0.0), (0.1, 0.1, 0.1), (128, 128, 128), out,
+                                                stream=sptr), args.steps, args.warmup)
+    ref = O.marginal(2, 1.0, 0.25, O.isotropic(po, x[sel] / 0.5, k, sk, U))
+    got = out[:, torch.from_numpy(sel).cuda()].cpu().numpy().astype(np.float64)
+    forms["grid_f16_per_realization"] = {"value": work / (wall * 1e-3), "ms_per_step": wall,
+                                         "kernel_ms": float(ms.mean()), **_err(got, ref)}
+    plan.close()
     # the marginal map alone: 8 B per FP32 value (read + write), HBM-bound
     out = torch.randn((Nmc, N), dtype=torch.float32, device="cuda") * 0.1
     wall, ms = _timed(lambda: R.check(R.lib().rf_marginal_device(ctx.handle, R.FOM_LOGNORMAL, 1.0, 0.25,

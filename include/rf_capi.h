@@ -177,8 +177,12 @@ int rf_plan_autotune(rf_field_plan* plan, const double* d_x, int64_t N,
  * math_RF.f90:70-88), evaluated axis by axis: the Shinozuka lattice is a
  * tensor product per sign block, so sum_j C_j exp(i k.x) factorises into
  * three small contractions.  Method 2, FIXED mode, nDim 2 or 3, or method 4
- * (gen_Std_Gauss_FFT: nstep = params.gridN, DCT-III tables); otherwise
- * RF_E_UNSUPPORTED.  d_out: prod(nstep) x Nmc.  The plan keeps a scratch
+ * (gen_Std_Gauss_FFT: nstep = params.gridN, DCT-III tables), or method 1
+ * (randomFieldND.f90:368-514), nDim 2 or 3, either mode: each term
+ * exp(i(k.x + phi)) factorises over the axes, so per realization the field
+ * is the real part of one GEMM over the shells whose operands are per-axis
+ * phase tables; otherwise RF_E_UNSUPPORTED.  d_out: prod(nstep) x Nmc.  The
+ * plan keeps a scratch
  * buffer, grown on the first call with a larger grid, and the folded
  * coefficients, formed on the first call and reused while the scale is the
  * same: calls on one plan are ordered by their stream, so calls on different

@@ -595,6 +595,65 @@ int rf_plan_autotune(rf_field_plan* P, const double* d_x, int64_t N, void* d_out
     return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The isotropic method on a grid: the per-realization wavevectors, amplitudes
+// and phases of the plan (FIXED or REF_COMPAT draws alike) summed by the
+// factorised form of rf_launch_iso_grid.
+int execute_grid_iso(rf_field_plan* P, const double* origin, const double* step, const int32_t* nstep,
+                     void* d_out, int32_t outDtype, void* stream) {
+    rf_context* ctx = P->ctx;
+    const rf_params& p = P->p;
+    if (p.nDim < 2) return rf_set_error(ctx, RF_E_UNSUPPORTED, "the isotropic grid form needs nDim 2 or 3");
+    int64_t N = 1, plane = 1;
+    for (int a = 0; a < p.nDim; a++) {
+        if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
+        if (!(step[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid steps must be > 0");
+        N *= nstep[a];
+        if (a + 1 < p.nDim) plane *= nstep[a];
+    }
+    const int64_t kN = P->spec.info.kNTotal;
+    if (plane > INT32_MAX || kN > INT32_MAX) return rf_set_error(ctx, RF_E_ARG, "grid or shell set too large for the grid form");
+    RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const size_t bytes = rf_iso_grid_scratch_elems(p.nDim, kN, p.Nmc, nstep) * sizeof(float2);
+    if (bytes > P->gridScratchBytes) {
+        RF_HIP_TRY(ctx, hipStreamSynchronize(s));
+        dfree(P->d_gridScratch);
+        P->d_gridScratch = nullptr;
+        P->gridScratchBytes = 0;
+        RF_HIP_TRY(ctx, hipMalloc(&P->d_gridScratch, bytes));
+        P->gridScratchBytes = bytes;
+    }
+    rf_iso_grid_args a{};
+    a.nDim = p.nDim;
+    a.Nmc = p.Nmc;
+    a.kN = kN;
+    for (int d = 0; d < 3; d++) {
+        a.n[d] = d < p.nDim ? nstep[d] : 1;
+        a.origin[d] = d < p.nDim ? origin[d] : 0.0;
+        a.step[d] = d < p.nDim ? step[d] : 1.0;
+        a.invL[d] = d < p.nDim ? 1.0 / p.corrL[d] : 1.0;
+    }
+    a.kIso = P->d_kIso;
+    a.amp = P->d_amp;
+    a.phase = P->d_phase;
+    a.ampMult = P->spec.info.ampMult;
+    a.coefMax = a.ampMult * P->ampMax;
+    a.epi = P->epi;
+    a.d_out = d_out;
+    a.outDtype = outDtype;
+    a.scratch = static_cast<float2*>(P->d_gridScratch);
+    RF_HIP_TRY(ctx, rf_launch_iso_grid(a, s));
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* step,
                          const int32_t* nstep, void* d_out, int32_t outDtype, void* stream) {
     if (!P) return rf_set_error(nullptr, RF_E_ARG, "plan is NULL");
@@ -604,9 +663,11 @@ int rf_plan_execute_grid(rf_field_plan* P, const double* origin, const double* s
     if (outDtype != RF_DTYPE_F64 && outDtype != RF_DTYPE_F32)
         return rf_set_error(ctx, RF_E_ARG, "unknown output dtype");
     const bool fft = p.method == RF_FFT;
+    if (p.method == RF_ISOTROPIC) return execute_grid_iso(P, origin, step, nstep, d_out, outDtype, stream);
     if (!fft && (p.method != RF_SHINOZUKA || p.mode != RF_FIXED || p.nDim < 2))
         return rf_set_error(ctx, RF_E_UNSUPPORTED,
-                            "the grid form needs the Shinozuka lattice, FIXED mode and nDim 2 or 3, or method 4");
+                            "the grid form needs the Shinozuka lattice in FIXED mode, the isotropic method "
+                            "(nDim 2 or 3), or method 4");
     for (int a = 0; a < p.nDim; a++) {
         if (nstep[a] < 1) return rf_set_error(ctx, RF_E_ARG, "grid counts must be >= 1");
         if (!(step[a] > 0.0)) return rf_set_error(ctx, RF_E_ARG, "grid steps must be > 0");

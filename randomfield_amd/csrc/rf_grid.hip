@@ -852,6 +852,147 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
     }
 }
 
+// Isotropic sum on a structured grid, per-realization directions.  With
+// x = o + i step per axis, e^{i(k.x + phi)} factorises over the axes, so per
+// realization r
+//   u_r[l][(i, j)] = Re sum_k Z_k[l] B_k[(i, j)],   B_k[(i, j)] = W_k[i] Y_k[j],
+//   W_k[i] = a_k e^{i(k_x x_i + phi_k)},  Y_k[j] = e^{i k_y y_j},  Z_k[l] = e^{i k_z z_l}
+// (2-D: u_r[j][i] = Re sum_k Y_k[j] W_k[i]): one real-part GEMM with M the
+// last axis, N the plane and K the shells, whose B operand is formed while
+// staged -- one complex product per element -- instead of read.  Tables are
+// [r][k][index]; a 64x128 tile of the real part as gemm_tile_r128.
+struct rf_iso_gemm {
+    const float2* A;              // last-axis table [r][k][m]
+    const float2* W;              // first-axis table with the coefficients [r][k][i]
+    const float2* Y;              // second-axis table [r][k][j] (3-D; NULL in 2-D)
+    int M, N, K, nx, ny;
+    int64_t aStrideB, wStrideB, yStrideB;
+    void* D;                      // [r][m][n]
+    int outDtype;
+    rf_epilogue epi;
+    float outScale;
+};
+__device__ __forceinline__ void gemm_tile_iso(const rf_iso_gemm& g, int m0, int n0, int batch, HStage<3>* st) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + m0);
+    const gf2* Wt = (const gf2*)(g.W + (int64_t)batch * g.wStrideB);
+    const gf2* Yt = (const gf2*)(g.Y ? g.Y + (int64_t)batch * g.yStrideB : nullptr);
+    f32x16 acc[2];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        acc[0][i] = 0.0f;
+        acc[1][i] = 0.0f;
+    }
+    const int rt = wave >> 1, ct = wave & 1;
+    // staging: A row t/4, k 4(t%4)..+3; B column t%128, k 8(t/128)..+7
+    const int aRow = tid >> 2, aK = (tid & 3) * 4;
+    const int bCol = tid & 127, bK = (tid >> 7) * 8;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
+    const int bI = bIn ? (n0 + bCol) % g.nx : 0, bJ = bIn ? (n0 + bCol) / g.nx : 0;
+    const bool hasY = Yt != nullptr;
+    const int kLast = g.K - 1;
+    f2v ra[4], rw[8], ry[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ka = k0 + aK + i;
+            ra[i] = (aIn && ka <= kLast) ? A[(int64_t)ka * g.M + aRow] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kb = k0 + bK + i;
+            const bool in = bIn && kb <= kLast;
+            rw[i] = in ? Wt[(int64_t)kb * g.nx + bI] : f2v{0.f, 0.f};
+            ry[i] = (in && hasY) ? Yt[(int64_t)kb * g.ny + bJ] : f2v{1.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            re[i] = ra[i].x;
+            im[i] = ra[i].y;
+        }
+        split4(re, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[0][aRow][aK]) = hi;
+        *reinterpret_cast<hv4*>(&st->a[1][aRow][aK]) = lo;
+        split4(im, &hi, &lo);
+        *reinterpret_cast<hv4*>(&st->a[2][aRow][aK]) = neg4(hi);   // split(-x) = -split(x)
+        *reinterpret_cast<hv4*>(&st->a[3][aRow][aK]) = neg4(lo);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const f2v w = rw[4 * h + i], y = ry[4 * h + i];
+                re[i] = fmaf(w.x, y.x, -w.y * y.y);
+                im[i] = fmaf(w.x, y.y, w.y * y.x);
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[0][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[1][bCol][bK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[2][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[3][bCol][bK + 4 * h]) = lo;
+        }
+    };
+    // K' half 0 pairs Re A with Re B, half 1 -Im A with Im B
+    const int aPlane = half ? 2 : 0, bPlane = half ? 2 : 0;
+    const int aRowL = rt * 32 + l32;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + rt * 32 < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+                const hv8 ah = *reinterpret_cast<const hv8*>(&st->a[aPlane][aRowL][kk]);
+                const hv8 al = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][aRowL][kk]);
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const int col = 64 * ct + 32 * c;
+                    if (n0 + col >= g.N) break;                                    // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][col + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][col + l32][kk]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dBase = (int64_t)batch * g.M * g.N + (int64_t)m0 * g.N + n0;
+    const int nc[2] = {64 * ct + l32, 64 * ct + 32 + l32};
+    const bool nIn[2] = {n0 + nc[0] < g.N, n0 + nc[1] < g.N};
+    auto store = [&](auto map, auto* D) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rt * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+            if (m0 + r >= g.M) continue;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+                if (nIn[c]) D[(int64_t)r * g.N + nc[c]] = map(acc[c][i] * os);
+        }
+    };
+    const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
+    gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
+    gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
+    if (g.outDtype == RF_DTYPE_F64) {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
+        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+    } else {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
+        else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+    }
+}
+
 // 64x128 tiles of the complex product (the middle contraction, whose
 // columns (j3, x1) are long): wave w owns complex rows 16 w.. and four
 // 32-column tiles, reading each A fragment once for all four (10 fragment
@@ -1084,6 +1225,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const rf_gemm_desc* __restrict__ gs, int nG, int64_t nTiles) {
     __shared__ __attribute__((aligned(16))) char smem[kStageBytes<5>];
     cgemm_group_body<false, 5>(gs, nG, nTiles, smem);
+}
+
+// One 64x128 tile of one realization's isotropic grid GEMM per block; tiles
+// numbered row tiles fastest (they share a column slab of B), placed per XCD.
+__global__ __launch_bounds__(256) void k_iso_gemm(rf_iso_gemm g, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[sizeof(HStage<3>)];
+    const int64_t L = xcd_tile(blockIdx.x, nTiles);
+    if (L >= nTiles) return;
+    const int ntM = (g.M + kTM - 1) / kTM, ntN = (g.N + 2 * kTN - 1) / (2 * kTN);
+    const int mI = (int)(L % ntM);
+    const int64_t rest = L / ntM;
+    const int nI = (int)(rest % ntN);
+    const int batch = (int)(rest / ntN);
+    gemm_tile_iso(g, mI * kTM, nI * 2 * kTN, batch, reinterpret_cast<HStage<3>*>(smem));
+}
+
+// Isotropic phase tables [r][k][index] for the axes of the grid, in FP64
+// then rounded: axis 0 carries the coefficient, W = c a e^{i(k_x x + phi)}
+// (c: ampMult times a power of two), the others e^{i k_a x}.
+__global__ void k_iso_tables(int nDim, int64_t kN, int Nmc, int n0, int n1, int n2, const double* __restrict__ kIso,
+                             const double* __restrict__ amp, const double* __restrict__ phase, double coef,
+                             double o0, double o1, double o2, double s0, double s1, double s2, double i0, double i1,
+                             double i2, float2* __restrict__ T0, float2* __restrict__ T1, float2* __restrict__ T2) {
+    const int nT = n0 + n1 + (nDim == 3 ? n2 : 0);
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)Nmc * kN * nT) return;
+    const int t = (int)(e % nT);
+    const int64_t rk = e / nT;                       // r * kN + k
+    const int a = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
+    const int idx = a == 0 ? t : (a == 1 ? t - n0 : t - n0 - n1);
+    const double o = a == 0 ? o0 : (a == 1 ? o1 : o2);
+    const double st = a == 0 ? s0 : (a == 1 ? s1 : s2);
+    const double inv = a == 0 ? i0 : (a == 1 ? i1 : i2);
+    const double x = (o + st * (double)idx) * inv;
+    double th = kIso[rk * nDim + a] * x;
+    if (a == 0) th += phase[rk];
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    const double m = a == 0 ? amp[rk] * coef : 1.0;
+    const int na = a == 0 ? n0 : (a == 1 ? n1 : n2);
+    float2* T = a == 0 ? T0 : (a == 1 ? T1 : T2);
+    T[rk * na + idx] = make_float2((float)(m * cs), (float)(m * sn));
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -1472,4 +1655,52 @@ hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+size_t rf_iso_grid_scratch_elems(int nDim, int64_t kN, int Nmc, const int32_t* n) {
+    int64_t t = 0;
+    for (int a = 0; a < nDim; a++) t += n[a];
+    return (size_t)(Nmc * kN * t);
+}
+
+hipError_t rf_launch_iso_grid(const rf_iso_grid_args& a, hipStream_t s) {
+    if (a.nDim < 2 || a.nDim > 3 || a.Nmc < 1 || a.kN < 1) return hipErrorInvalidValue;
+    // coefficients into [1/2, 1) by a power of two, undone by the epilogue
+    double sc = 1.0;
+    if (a.coefMax > 0.0 && std::isfinite(a.coefMax)) {
+        int e = 0;
+        std::frexp(a.coefMax, &e);
+        sc = std::ldexp(1.0, -e);
+    }
+    const int n0 = a.n[0], n1 = a.n[1], n2 = a.nDim == 3 ? a.n[2] : 1;
+    float2* T0 = a.scratch;
+    float2* T1 = T0 + (int64_t)a.Nmc * a.kN * n0;
+    float2* T2 = T1 + (int64_t)a.Nmc * a.kN * n1;
+    const int64_t cnt = (int64_t)a.Nmc * a.kN * (n0 + n1 + (a.nDim == 3 ? n2 : 0));
+    hipLaunchKernelGGL(k_iso_tables, dim3(blocks_for(cnt)), dim3(256), 0, s, a.nDim, a.kN, a.Nmc, n0, n1, n2, a.kIso,
+                       a.amp, a.phase, a.ampMult * sc, a.origin[0], a.origin[1], a.origin[2], a.step[0], a.step[1],
+                       a.step[2], a.invL[0], a.invL[1], a.invL[2], T0, T1, T2);
+    rf_iso_gemm g{};
+    g.K = (int)a.kN;
+    g.W = T0;
+    g.nx = n0;
+    g.wStrideB = a.kN * n0;
+    if (a.nDim == 3) {
+        g.A = T2; g.M = n2; g.aStrideB = a.kN * n2;
+        g.Y = T1; g.ny = n1; g.yStrideB = a.kN * n1;
+        g.N = n0 * n1;
+    } else {
+        g.A = T1; g.M = n1; g.aStrideB = a.kN * n1;
+        g.Y = nullptr; g.ny = 1; g.yStrideB = 0;
+        g.N = n0;
+    }
+    g.D = a.d_out;
+    g.outDtype = a.outDtype;
+    g.epi = a.epi;
+    g.outScale = (float)(1.0 / sc);
+    const int64_t nTiles = (int64_t)((g.M + kTM - 1) / kTM) * ((g.N + 2 * kTN - 1) / (2 * kTN)) * a.Nmc;
+    const int64_t blocks = xcd_blocks(nTiles);
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_iso_gemm, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    return hipGetLastError();
 }

@@ -304,6 +304,26 @@ hipError_t rf_grid_prepare(int kind, int nDim, int Nmc, const int32_t* kNStep, c
                            double ampMult, double coefMax, const rf_epilogue& epi, void* d_out, int outDtype,
                            float2* scratch, hipStream_t s, rf_grid_stages* st, rf_grid_coef* coef = nullptr);
 hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s);
+// Isotropic sum (per-realization directions) on a structured grid x = o + i
+// step per axis (x fastest): per-axis phase tables, then per realization one
+// real-part MFMA GEMM whose B operand is formed while staged (rf_grid.hip).
+struct rf_iso_grid_args {
+    int nDim, Nmc;
+    int64_t kN;
+    int n[3];                     // grid points per axis (1 beyond nDim)
+    double origin[3], step[3], invL[3];
+    const double* kIso;           // [Nmc][kN][nDim] wavevectors (radians per correlation length)
+    const double* amp;            // [Nmc][kN]
+    const double* phase;          // [Nmc][kN] radians
+    double coefMax;               // bound of ampMult * |amp|
+    double ampMult;
+    rf_epilogue epi;
+    void* d_out;                  // [Nmc][prod n]
+    int outDtype;
+    float2* scratch;              // rf_iso_grid_scratch_elems float2
+};
+size_t rf_iso_grid_scratch_elems(int nDim, int64_t kN, int Nmc, const int32_t* n);
+hipError_t rf_launch_iso_grid(const rf_iso_grid_args& a, hipStream_t s);
 int64_t rf_gemm_tiles(const rf_gemm_desc& g);
 hipError_t rf_launch_gemm(const rf_gemm_desc& g, hipStream_t s);
 hipError_t rf_launch_gemm_group(int realOut, const rf_gemm_desc* d_gs, int nG, int64_t nTiles, int rwMax,

@@ -76,13 +76,14 @@ def test_grid_form_lognormal_and_fp64_out(ctx):
 
 def test_grid_form_rejects_other_methods(ctx):
     import torch
-    p = R.make_params(nDim=3, method=R.SHINOZUKA, mode=R.REF_COMPAT, Nmc=1, xRange=(2, 2, 2))
-    plan = R.FieldPlan(ctx, p)
-    out = torch.empty((1, 27), dtype=torch.float32, device="cuda")
-    with pytest.raises(R.RFError) as e:
-        plan.execute_grid((0, 0, 0), (1, 1, 1), (3, 3, 3), out)
-    assert e.value.code == R.RF_E_UNSUPPORTED
-    plan.close()
+    for method, mode in ((R.SHINOZUKA, R.REF_COMPAT), (R.RANDOMIZATION, R.FIXED)):
+        p = R.make_params(nDim=3, method=method, mode=mode, Nmc=1, xRange=(2, 2, 2))
+        plan = R.FieldPlan(ctx, p)
+        out = torch.empty((1, 27), dtype=torch.float32, device="cuda")
+        with pytest.raises(R.RFError) as e:
+            plan.execute_grid((0, 0, 0), (1, 1, 1), (3, 3, 3), out)
+        assert e.value.code == R.RF_E_UNSUPPORTED
+        plan.close()
 
 
 def test_localization_pipeline_small(ctx):

@@ -993,6 +993,146 @@ __device__ __forceinline__ void gemm_tile_iso(const rf_iso_gemm& g, int m0, int 
     }
 }
 
+// The isotropic grid GEMM on 128x128 tiles (a last axis longer than 64):
+// each B element formed once per realization instead of once per 64 rows;
+// wave w owns rows 64 (w / 2).. and columns 64 (w % 2).., four 32x32 MFMA
+// tiles sharing two A and two B fragment reads per K-step.
+struct IsoStage128 {
+    _Float16 a[4][128][kHP];      // Re hi, Re lo, -Im hi, -Im lo
+    _Float16 b[4][128][kHP];      // Re hi, Re lo, Im hi, Im lo
+};
+__device__ __forceinline__ void gemm_tile_iso128(const rf_iso_gemm& g, int m0, int n0, int batch, IsoStage128* st) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
+    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + m0);
+    const gf2* Wt = (const gf2*)(g.W + (int64_t)batch * g.wStrideB);
+    const gf2* Yt = (const gf2*)(g.Y ? g.Y + (int64_t)batch * g.yStrideB : nullptr);
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q >> 1][q & 1][i] = 0.0f;
+    const int rt = wave >> 1, ct = wave & 1;
+    // staging: A row t/2, k 8(t%2)..+7; B column t%128, k 8(t/128)..+7
+    const int aRow = tid >> 1, aK = (tid & 1) * 8;
+    const int bCol = tid & 127, bK = (tid >> 7) * 8;
+    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
+    const int bI = bIn ? (n0 + bCol) % g.nx : 0, bJ = bIn ? (n0 + bCol) / g.nx : 0;
+    const bool hasY = Yt != nullptr;
+    const int kLast = g.K - 1;
+    f2v ra[8], rw[8], ry[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int ka = k0 + aK + i;
+            ra[i] = (aIn && ka <= kLast) ? A[(int64_t)ka * g.M + aRow] : f2v{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int kb = k0 + bK + i;
+            const bool in = bIn && kb <= kLast;
+            rw[i] = in ? Wt[(int64_t)kb * g.nx + bI] : f2v{0.f, 0.f};
+            ry[i] = (in && hasY) ? Yt[(int64_t)kb * g.ny + bJ] : f2v{1.f, 0.f};
+        }
+    };
+    auto stash = [&]() {
+        float re[4], im[4];
+        hv4 hi, lo;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                re[i] = ra[4 * h + i].x;
+                im[i] = ra[4 * h + i].y;
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->a[0][aRow][aK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->a[1][aRow][aK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->a[2][aRow][aK + 4 * h]) = neg4(hi);   // split(-x) = -split(x)
+            *reinterpret_cast<hv4*>(&st->a[3][aRow][aK + 4 * h]) = neg4(lo);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const f2v w = rw[4 * h + i], y = ry[4 * h + i];
+                re[i] = fmaf(w.x, y.x, -w.y * y.y);
+                im[i] = fmaf(w.x, y.y, w.y * y.x);
+            }
+            split4(re, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[0][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[1][bCol][bK + 4 * h]) = lo;
+            split4(im, &hi, &lo);
+            *reinterpret_cast<hv4*>(&st->b[2][bCol][bK + 4 * h]) = hi;
+            *reinterpret_cast<hv4*>(&st->b[3][bCol][bK + 4 * h]) = lo;
+        }
+    };
+    // K' half 0 pairs Re A with Re B, half 1 -Im A with Im B
+    const int aPlane = half ? 2 : 0, bPlane = half ? 2 : 0;
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stash();
+        __syncthreads();
+        if (k0 + kTK < g.K) fetch(k0 + kTK);      // in flight during the MFMAs
+        if (m0 + rt * 64 < g.M) {
+#pragma unroll
+            for (int kk = 0; kk < kTK; kk += 8) {
+                hv8 ah[2], al[2];
+#pragma unroll
+                for (int rr = 0; rr < 2; rr++) {
+                    ah[rr] = *reinterpret_cast<const hv8*>(&st->a[aPlane][rt * 64 + 32 * rr + l32][kk]);
+                    al[rr] = *reinterpret_cast<const hv8*>(&st->a[aPlane + 1][rt * 64 + 32 * rr + l32][kk]);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const int col = 64 * ct + 32 * c;
+                    if (n0 + col >= g.N) break;                                    // uniform
+                    const hv8 bh = *reinterpret_cast<const hv8*>(&st->b[bPlane][col + l32][kk]);
+                    const hv8 bl = *reinterpret_cast<const hv8*>(&st->b[bPlane + 1][col + l32][kk]);
+#pragma unroll
+                    for (int rr = 0; rr < 2; rr++) {
+                        if (rr > 0 && m0 + rt * 64 + 32 >= g.M) break;            // uniform
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rr], bh, acc[rr][c], 0, 0, 0);
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[rr], bl, acc[rr][c], 0, 0, 0);
+                        acc[rr][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[rr], bh, acc[rr][c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of a 32x32 tile: lane column l32, rows 8*(i/4) + 4*half + i%4
+    const float os = g.outScale;
+    const int64_t dBase = (int64_t)batch * g.M * g.N + (int64_t)m0 * g.N + n0;
+    const int nc[2] = {64 * ct + l32, 64 * ct + 32 + l32};
+    const bool nIn[2] = {n0 + nc[0] < g.N, n0 + nc[1] < g.N};
+    auto store = [&](auto map, auto* D) {
+#pragma unroll
+        for (int rr = 0; rr < 2; rr++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = rt * 64 + 32 * rr + 8 * (i >> 2) + 4 * half + (i & 3);
+                if (m0 + r >= g.M) continue;
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    if (nIn[c]) D[(int64_t)r * g.N + nc[c]] = map(acc[rr][c][i] * os);
+            }
+    };
+    const float sc = (float)g.epi.scale, sh = (float)g.epi.shift;
+    gfo* D32 = (gfo*)(static_cast<float*>(g.D) + dBase);
+    gdo* D64 = (gdo*)(static_cast<double*>(g.D) + dBase);
+    if (g.outDtype == RF_DTYPE_F64) {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return (double)u; }, D64);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return (double)expf(fmaf(u, sc, sh)); }, D64);
+        else store([=](float u) { return (double)fmaf(u, sc, sh); }, D64);
+    } else {
+        if (g.epi.margi == RF_FOM_NONE) store([](float u) { return u; }, D32);
+        else if (g.epi.margi == RF_FOM_LOGNORMAL) store([=](float u) { return expf(fmaf(u, sc, sh)); }, D32);
+        else store([=](float u) { return fmaf(u, sc, sh); }, D32);
+    }
+}
+
 // 64x128 tiles of the complex product (the middle contraction, whose
 // columns (j3, x1) are long): wave w owns complex rows 16 w.. and four
 // 32-column tiles, reading each A fragment once for all four (10 fragment
@@ -1229,16 +1369,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // One 64x128 tile of one realization's isotropic grid GEMM per block; tiles
 // numbered row tiles fastest (they share a column slab of B), placed per XCD.
-__global__ __launch_bounds__(256) void k_iso_gemm(rf_iso_gemm g, int64_t nTiles) {
-    __shared__ __attribute__((aligned(16))) char smem[sizeof(HStage<3>)];
+template <int TM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_iso_gemm(rf_iso_gemm g, int64_t nTiles) {
+    __shared__ __attribute__((aligned(16))) char smem[TM == 128 ? sizeof(IsoStage128) : sizeof(HStage<3>)];
     const int64_t L = xcd_tile(blockIdx.x, nTiles);
     if (L >= nTiles) return;
-    const int ntM = (g.M + kTM - 1) / kTM, ntN = (g.N + 2 * kTN - 1) / (2 * kTN);
+    const int ntM = (g.M + TM - 1) / TM, ntN = (g.N + 2 * kTN - 1) / (2 * kTN);
     const int mI = (int)(L % ntM);
     const int64_t rest = L / ntM;
     const int nI = (int)(rest % ntN);
     const int batch = (int)(rest / ntN);
-    gemm_tile_iso(g, mI * kTM, nI * 2 * kTN, batch, reinterpret_cast<HStage<3>*>(smem));
+    if constexpr (TM == 128)
+        gemm_tile_iso128(g, mI * TM, nI * 2 * kTN, batch, reinterpret_cast<IsoStage128*>(smem));
+    else
+        gemm_tile_iso(g, mI * TM, nI * 2 * kTN, batch, reinterpret_cast<HStage<3>*>(smem));
 }
 
 // Isotropic phase tables [r][k][index] for the axes of the grid, in FP64
@@ -1698,9 +1842,13 @@ hipError_t rf_launch_iso_grid(const rf_iso_grid_args& a, hipStream_t s) {
     g.outDtype = a.outDtype;
     g.epi = a.epi;
     g.outScale = (float)(1.0 / sc);
-    const int64_t nTiles = (int64_t)((g.M + kTM - 1) / kTM) * ((g.N + 2 * kTN - 1) / (2 * kTN)) * a.Nmc;
+    // a last axis longer than 64 on 128-row tiles (each B element formed once
+    // per 128 rows)
+    const int TM = g.M > kTM ? 128 : kTM;
+    const int64_t nTiles = (int64_t)((g.M + TM - 1) / TM) * ((g.N + 2 * kTN - 1) / (2 * kTN)) * a.Nmc;
     const int64_t blocks = xcd_blocks(nTiles);
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_iso_gemm, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    if (TM == 128) hipLaunchKernelGGL(k_iso_gemm<128>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
+    else hipLaunchKernelGGL(k_iso_gemm<kTM>, dim3((unsigned)blocks), dim3(256), 0, s, g, nTiles);
     return hipGetLastError();
 }

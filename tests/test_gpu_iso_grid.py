@@ -37,6 +37,9 @@ CASES = [
     ((12, 12, 12), (0.3, 0.3, 0.3), (0.0, 0.0, 0.0), (0.9, 0.9, 0.9), R.FIXED, 1, R.FOM_GAUSSIAN, torch.float32),
     ((33, 41), (0.2, 0.15), (0.5, -1.0), (1.0, 1.3), R.FIXED, 0, R.FOM_NONE, torch.float32),
     ((40, 29), (0.25, 0.25), (0.0, 0.0), (0.8, 0.8), R.REF_COMPAT, 0, R.FOM_NONE, torch.float64),
+    # a last axis longer than 64 runs on 128-row tiles: one full, one partial
+    ((9, 7, 150), (0.3, 0.4, 0.1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), R.FIXED, 0, R.FOM_NONE, torch.float32),
+    ((50, 130), (0.2, 0.1), (1.0, 0.0), (1.0, 0.9), R.REF_COMPAT, 0, R.FOM_LOGNORMAL, torch.float32),
 ]
 
 

@@ -33,7 +33,8 @@ struct rf_grid_batch {
     std::vector<uint64_t> seeds;
     std::vector<int> nmc;
     bool redrawable = true;
-    int64_t* d_draw = nullptr;
+    int64_t* d_draw = nullptr;          // draw state: [0] next, [1] current, [2] coefficient mode
+    bool genActive = false;             // the last redraw left the first contraction forming the coefficients
     rf_fold_item* d_items = nullptr;    // the folds as one grouped launch
     int64_t foldBlocks = 0;
     // host copies of the stage descriptors and the tile kinds the set-up
@@ -166,6 +167,15 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         if (gs.n == 3) stage[1].push_back(gs.g[1]);
         stage[2].push_back(gs.g[gs.n - 1]);
     }
+    B->d_draw = reinterpret_cast<int64_t*>(B->d_mem + drawOff);
+    if (B->redrawable)
+        for (size_t i = 0; i < stage[0].size(); i++) {
+            // the first contraction can form fresh coefficients itself
+            stage[0][i].gen = B->folds[i];
+            stage[0][i].genSeed = B->seeds[i];
+            stage[0][i].genNmc = B->nmc[i];
+            stage[0][i].genState = B->d_draw;
+        }
     size_t dOff = 0;
     for (int k = 0; k < 3; k++) {
         B->stage[k] = stage[k];
@@ -179,8 +189,8 @@ int rf_grid_batch_create(rf_context* ctx, int32_t n, rf_field_plan* const* plans
         (void)hipFree(B->d_mem);
         return rf_set_error(ctx, RF_E_ARG, "batch too large for one grouped launch");
     }
-    B->d_draw = reinterpret_cast<int64_t*>(B->d_mem + drawOff);
-    if (e == hipSuccess) e = rf_launch_draw_counter(B->d_draw, 1, s);   // draw 0: the plans' own realizations
+    // draw 0, the plans' own realizations, folded at creation; next draw 1
+    if (e == hipSuccess) e = rf_launch_draw_counter(B->d_draw, -2, 0, s);
     std::vector<rf_fold_item> items((size_t)n);
     if (B->redrawable) {
         for (int i = 0; i < n; i++) {
@@ -215,6 +225,13 @@ int rf_grid_batch_autotune(rf_grid_batch* B, void* stream, int32_t* chosen) {
     rf_context* ctx = B->ctx;
     RF_HIP_TRY(ctx, hipSetDevice(ctx->device));
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (B->genActive) {
+        // the current realizations' coefficients stored, as every tile kind
+        // reads them, and the first contraction back to reading them
+        RF_HIP_TRY(ctx, rf_launch_refold_draw(B->d_items, (int)B->folds.size(), B->foldBlocks, B->d_draw + 1, s));
+        RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, -3, 0, s));
+        B->genActive = false;
+    }
     int rc = rf_grid_batch_execute(B, stream);          // every stage's inputs in place
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -277,9 +294,16 @@ int rf_grid_batch_redraw(rf_grid_batch* B, int64_t drawIndex, void* stream) {
     if (!B->redrawable)
         return rf_set_error(ctx, RF_E_UNSUPPORTED, "rf_grid_batch_redraw: method 4 fields have no phases to redraw");
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    if (drawIndex >= 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, drawIndex, s));
-    RF_HIP_TRY(ctx, rf_launch_refold_draw(B->d_items, (int)B->folds.size(), B->foldBlocks, B->d_draw, s));
-    if (drawIndex < 0) RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, -1, s));
+    // when every field's first contraction runs on 64x160 tiles, those form
+    // the coefficients of the current draw themselves (no fold, no
+    // coefficient round trip through HBM); otherwise the grouped fold
+    // stores them
+    bool gen = !B->stage[0].empty();
+    for (const auto& g : B->stage[0]) gen = gen && g.rw == 5 && g.genState;
+    RF_HIP_TRY(ctx, rf_launch_draw_counter(B->d_draw, drawIndex >= 0 ? drawIndex : -1, gen ? 1 : 0, s));
+    if (!gen)
+        RF_HIP_TRY(ctx, rf_launch_refold_draw(B->d_items, (int)B->folds.size(), B->foldBlocks, B->d_draw + 1, s));
+    B->genActive = gen;
     return RF_OK;
 }
 

@@ -567,6 +567,37 @@ __device__ __forceinline__ void gemm_tile_c128(const rf_gemm_desc& g, int m0, in
         }
 }
 
+// cos and sin of 2 pi U for U in [0, 1): FP32 sincospi of the FP32 part Uh
+// and the first-order term of the remainder U - Uh (|U - Uh| <= 2^-25), a few
+// FP32 ulps from the FP64 values -- the coefficients are FP32.
+__device__ __forceinline__ void cis_rev(double U, float* c, float* s) {
+    const float uh = (float)U;
+    const float ul = (float)(U - (double)uh);
+    float sh, ch;
+    sincospif(2.0f * uh, &sh, &ch);
+    const float d = 6.28318530717958648f * ul;
+    *s = fmaf(d, ch, sh);
+    *c = fmaf(-d, sh, ch);
+}
+
+// The folded coefficient of row (r, j3, q, j2) and column (h, j1) with its
+// phase drawn from element drawBase + src of the stream (seed, tag 2): the
+// one formula of the coefficient fold and of the first contraction that forms
+// its coefficients itself.
+__device__ __forceinline__ float2 fold_coef_draw(const rf_fold_desc& f, unsigned r, unsigned j3, unsigned q,
+                                                 unsigned j2, unsigned j1, unsigned h, uint64_t seed,
+                                                 int64_t drawBase) {
+#pragma clang fp contract(off)
+    unsigned b = 0;
+    if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
+    const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * f.nk2 + j2) * f.nk1 + j1;   // b kNu + j
+    const int64_t src = (int64_t)r * f.kNr + jb;
+    const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
+    float cs, sn;
+    cis_rev(rf_uniform(seed, 2u, (uint64_t)(drawBase + src)), &cs, &sn);
+    return make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
+}
+
 // 64x160 tiles of the complex product (the first contraction when its
 // column axis, the first grid axis, has 129..160 points): wave w owns complex
 // rows 16 w.. and the five 32-column tiles, reading each A fragment once for
@@ -595,12 +626,41 @@ __device__ __forceinline__ void gemm_tile_c160(const rf_gemm_desc& g, int m0, in
     const int bOff = bIn ? (int)(col_b(g, n0 + bCol) - bCol0) : 0;
     const int bOff2 = bIn2 ? (int)(col_b(g, n0 + bCol2) - bCol0) : 0;
     const int kLast = g.K - 1;
+    // a batch's fresh realizations: the coefficients of the current draw are
+    // formed here (fold_coef_draw, as the fold would store them) instead of
+    // read; the row's digits (j2, q, j3, r) once per tile
+    const bool gen = g.genState != nullptr && g.genState[2] != 0;
+    int64_t drawBase = 0;
+    unsigned gr = 0, gj3 = 0, gq = 0, gj2 = 0;
+    if (gen) {
+        drawBase = g.genState[1] * (int64_t)g.genNmc * g.gen.kNr;
+        const unsigned Q = (g.gen.fold && g.gen.nDim == 3) ? 2u : 1u;
+        unsigned rem = (unsigned)(m0 + aRow);
+        gj2 = rem % (unsigned)g.gen.nk2; rem /= (unsigned)g.gen.nk2;
+        gq = rem % Q; rem /= Q;
+        gj3 = g.gen.nDim == 3 ? rem % (unsigned)g.gen.nk3 : 0u;
+        gr = g.gen.nDim == 3 ? rem / (unsigned)g.gen.nk3 : rem;
+    }
     f2v ra[4], rb[8], rb2[2];
     auto fetch = [&](int k0) {
+        if (gen) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int ka = k0 + aK + i;
-            ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+            for (int i = 0; i < 4; i++) {
+                const int ka = k0 + aK + i;
+                if (aIn && ka <= kLast) {
+                    const float2 c = fold_coef_draw(g.gen, gr, gj3, gq, gj2, (unsigned)ka % (unsigned)g.gen.nk1,
+                                                    (unsigned)ka / (unsigned)g.gen.nk1, g.genSeed, drawBase);
+                    ra[i] = f2v{c.x, c.y};
+                } else {
+                    ra[i] = f2v{0.f, 0.f};
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int ka = k0 + aK + i;
+                ra[i] = (aIn && ka <= kLast) ? A[aOff + ka * aSk] : f2v{0.f, 0.f};
+            }
         }
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -1469,18 +1529,6 @@ __global__ void k_dct_table(int n, float2* __restrict__ E) {
 // ampStrideR 0: every realization has the amplitudes of row 0.
 constexpr int kFoldM = 64, kFoldK = 32;   // coefficient-fold tile: rows x k
 
-// cos and sin of 2 pi U for U in [0, 1): FP32 sincospi of the FP32 part Uh
-// and the first-order term of the remainder U - Uh (|U - Uh| <= 2^-25), a few
-// FP32 ulps from the FP64 values -- the coefficients are FP32.
-__device__ __forceinline__ void cis_rev(double U, float* c, float* s) {
-    const float uh = (float)U;
-    const float ul = (float)(U - (double)uh);
-    float sh, ch;
-    sincospif(2.0f * uh, &sh, &ch);
-    const float d = 6.28318530717958648f * ul;
-    *s = fmaf(d, ch, sh);
-    *c = fmaf(-d, sh, ch);
-}
 
 // One 64-row x 32-k tile of the folded coefficients, transposed through LDS
 // to the K-major layout C[k * rows + m]; rows < 2^31 (checked by the launchers).
@@ -1511,17 +1559,21 @@ __device__ __forceinline__ void fold_tile(const rf_fold_desc& f, int bx, int by,
         for (int mm = mm0; mm < kFoldM; mm += kRowStep) {
             const int m = m0 + mm;
             if (m >= rows) break;
-            unsigned b = 0;
-            if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
-            const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * nk2 + j2) * nk1 + j1;   // b kNu + j
-            const int64_t src = (int64_t)r * f.kNr + jb;
-            const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
-            if (!Draw && !f.uni) {
-                tile[kk][mm] = make_float2((float)a, 0.0f);
+            if (Draw) {
+                tile[kk][mm] = fold_coef_draw(f, r, j3, q, j2, j1, h, seed, drawBase);
             } else {
-                float cs, sn;
-                cis_rev(Draw ? rf_uniform(seed, 2u, (uint64_t)(drawBase + src)) : f.uni[src], &cs, &sn);
-                tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
+                unsigned b = 0;
+                if (f.fold) b = f.nDim == 3 ? (q == 0 ? (h ? 3u : 0u) : (h ? 1u : 2u)) : h;
+                const int64_t jb = (int64_t)b * f.kNu + ((int64_t)j3 * nk2 + j2) * nk1 + j1;   // b kNu + j
+                const int64_t src = (int64_t)r * f.kNr + jb;
+                const double a = f.amp[(int64_t)r * f.ampStrideR + jb] * f.ampMult;
+                if (!f.uni) {
+                    tile[kk][mm] = make_float2((float)a, 0.0f);
+                } else {
+                    float cs, sn;
+                    cis_rev(f.uni[src], &cs, &sn);
+                    tile[kk][mm] = make_float2((float)(a * (double)cs), (float)(h ? -(a * (double)sn) : a * (double)sn));
+                }
             }
             // next row m + kRowStep: j2 fastest, then q, then j3 (3-D), then r
             j2 += kRowStep;
@@ -1783,13 +1835,24 @@ hipError_t rf_launch_refold_draw(const rf_fold_item* d_items, int n, int64_t blo
 }
 
 namespace {
-__global__ void k_draw_counter(int64_t* d_draw, int64_t value) {
-    if (threadIdx.x == 0) *d_draw = value >= 0 ? value : *d_draw + 1;
+__global__ void k_draw_counter(int64_t* st, int64_t value, int gen) {
+    if (threadIdx.x != 0) return;
+    if (value >= 0) {
+        st[0] = value;
+        st[1] = value;
+    } else if (value == -1) {
+        st[1] = st[0];
+        st[0] = st[0] + 1;
+    } else if (value == -2) {
+        st[0] = 1;
+        st[1] = 0;
+    }
+    st[2] = gen;
 }
 }  // namespace
 
-hipError_t rf_launch_draw_counter(int64_t* d_draw, int64_t value, hipStream_t s) {
-    hipLaunchKernelGGL(k_draw_counter, dim3(1), dim3(64), 0, s, d_draw, value);
+hipError_t rf_launch_draw_counter(int64_t* d_state, int64_t value, int gen, hipStream_t s) {
+    hipLaunchKernelGGL(k_draw_counter, dim3(1), dim3(64), 0, s, d_state, value, gen);
     return hipGetLastError();
 }
 

@@ -214,6 +214,16 @@ inline rf_epilogue rf_make_epilogue(int margi, double avg, double var) {
 
 // structured-grid form (rf_grid.hip), nDim 2 or 3, Shinozuka lattice:
 // batched complex FP32 GEMMs D[b] = A[(b / aDiv) % aMod] B[b]
+// The coefficient fold of one grid execution: Cf[k][m] for rows m (r, j3, q,
+// j2) and K index k (h, j1) from amp / phase [r][b][j] (rf_grid.hip).
+struct rf_fold_desc {
+    int64_t rows; int K, nDim, fold, nk1, nk2, nk3;
+    int64_t kNu, kNr;
+    const double* amp; int64_t ampStrideR;  // amplitudes [r * ampStrideR + b kNu + j]
+    const double* uni;                      // phase uniforms [r][kNr]; NULL: real coefficients (method 4)
+    double ampMult;
+    float2* C;
+};
 struct rf_gemm_desc {
     const float2* A; int64_t aSm, aSk, aStrideB; int aDiv, aMod;
     const float2* B; int64_t bSk, bSn, bStrideB;
@@ -249,16 +259,15 @@ struct rf_gemm_desc {
     void* bandLo; void* bandHi;
     int64_t bandLoStrideB, bandHiStrideB;
     int grpNy;
-};
-// The coefficient fold of one grid execution: Cf[k][m] for rows m (r, j3, q,
-// j2) and K index k (h, j1) from amp / phase [r][b][j] (rf_grid.hip).
-struct rf_fold_desc {
-    int64_t rows; int K, nDim, fold, nk1, nk2, nk3;
-    int64_t kNu, kNr;
-    const double* amp; int64_t ampStrideR;  // amplitudes [r * ampStrideR + b kNu + j]
-    const double* uni;                      // phase uniforms [r][kNr]; NULL: real coefficients (method 4)
-    double ampMult;
-    float2* C;
+    // fresh realizations formed by the first contraction of a batch: the
+    // field's coefficient fold, seed and Nmc, and the batch's draw state
+    // [0] next draw, [1] draw of the current realizations, [2] 1 while the
+    // first contraction forms the coefficients of draw [1] itself instead of
+    // reading them (genState NULL: always read)
+    rf_fold_desc gen;
+    uint64_t genSeed;
+    int genNmc;
+    const int64_t* genState;
 };
 struct rf_grid_stages {
     rf_gemm_desc g[3];    // stage 1, stage 2 (3-D only), last stage (real)
@@ -284,8 +293,11 @@ void rf_fold_item_init(rf_fold_item* it, const rf_fold_desc& f, int Nmc, uint64_
 // device, so a captured graph draws what the counter holds at replay.
 hipError_t rf_launch_refold_draw(const rf_fold_item* d_items, int n, int64_t blocks, const int64_t* d_draw,
                                  hipStream_t s);
-// *d_draw = value (value >= 0) or *d_draw += 1 (value < 0), one thread
-hipError_t rf_launch_draw_counter(int64_t* d_draw, int64_t value, hipStream_t s);
+// The draw state [0] next draw, [1] current draw, [2] coefficient mode of a
+// batch, one thread: value >= 0 sets [0] = [1] = value; RF_DRAW_NEXT (-1)
+// sets [1] = [0] and advances [0]; -2 initialises ([0] 1, [1] 0); -3 leaves
+// the draws; then [2] = gen
+hipError_t rf_launch_draw_counter(int64_t* d_state, int64_t value, int gen, hipStream_t s);
 size_t rf_grid_scratch_elems(int nDim, const int32_t* kNStep, int nb, int Nmc, const int32_t* n);
 // kind 0: Shinozuka lattice, tables exp(i k_j x); kind 1: method 4, tables of
 // the DCT-III (FFTW REDFT01) of each axis, w_j cos(pi j (2i+1) / (2n))

@@ -279,14 +279,26 @@ def test_repeated_grid_calls_reuse_coefficients(ctx, method):
         assert np.array_equal(o.cpu().numpy(), got)
 
 
-def test_grid_batch_redraw(ctx):
+REDRAW_SPECS = {
+    "fold": SPECS[:2] + [((12, 10, 9), (0.3, 0.3, 0.3), (1.0, 0.0, -1.0), (1.0, 1.0, 1.0), 4, 21)],
+    # first axes of 129..160 points: every first contraction on 64x160 tiles,
+    # which then form the fresh coefficients themselves (no fold)
+    "formed": [((131, 6, 5), (0.1, 0.3, 0.4), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 3, 41),
+               ((140, 7), (0.08, 0.3), (0.5, 0.0), (1.0, 1.2), 2, 42),
+               ((135, 5, 4), (0.1, 0.3, 0.3), (1.0, 0.0, -1.0), (1.0, 1.0, 1.0), 4, 43)],
+}
+
+
+@pytest.mark.parametrize("which", ["fold", "formed"])
+def test_grid_batch_redraw(ctx, which):
     """Fresh realizations in a batch: redraw(0) gives the plans' own fields
     bit for bit; redraw(2) gives realizations 2 Nmc .. 3 Nmc - 1 of each
     plan's device stream (checked against the oracle on those draws); a graph
     captured with the draw counter draws the next set on every replay, equal
-    to the eager redraw of that index."""
+    to the eager redraw of that index.  Both ways of forming the fresh
+    coefficients: the grouped fold, and the first contraction itself."""
     import torch
-    specs = SPECS[:2] + [((12, 10, 9), (0.3, 0.3, 0.3), (1.0, 0.0, -1.0), (1.0, 1.0, 1.0), 4, 21)]
+    specs = REDRAW_SPECS[which]
     plans, outs, geo = _grid_fields(ctx, specs, torch.float32)
     b = R.GridBatch(ctx, plans, [g[0] for g in geo], [g[1] for g in geo], [g[2] for g in geo], outs, R.DTYPE_F32)
     b.execute()
@@ -339,6 +351,16 @@ def test_grid_batch_redraw(ctx):
         for i, o in enumerate(outs):
             assert np.array_equal(o.cpu().numpy(), eager[d][i]), (rep, i)
     graph.close()
+    # the autotuner after a redraw keeps the current realizations, whichever
+    # tile kind it picks
+    b.redraw(2)
+    b.autotune()
+    for o in outs:
+        o.fill_(-1.0)
+    b.execute()
+    ctx.sync()
+    for i, o in enumerate(outs):
+        assert np.array_equal(o.cpu().numpy(), eager[2][i]), i
     b.close()
     for pl in plans:
         pl.close()

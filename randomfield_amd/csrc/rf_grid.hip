@@ -20,6 +20,11 @@
 // The negative sign blocks are folded into the first contraction
 // (conjugated tables and coefficients).  The phase tables are formed in FP64
 // and rounded to FP32; accumulation is FP32 over <= 4*N_k^(1/3) terms.
+//
+// The isotropic method (randomFieldND.f90:368-514) has no lattice, but on a
+// grid each of its terms still factorises over the axes: per realization one
+// real-part GEMM over the shells whose operands are per-axis phase tables
+// (k_iso_tables, k_iso_gemm).
 #include "rf_internal.h"
 #include "rf_philox.h"
 

@@ -181,6 +181,19 @@ typedef const f2v __attribute__((address_space(1))) gf2;      // global-memory v
 typedef f2v __attribute__((address_space(1))) gf2o;
 typedef float __attribute__((address_space(1))) gfo;
 typedef double __attribute__((address_space(1))) gdo;
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const f4v __attribute__((address_space(1))) gf4;
+// n consecutive float2 (n even, 16-byte aligned) by 16-byte loads
+template <int n>
+__device__ __forceinline__ void ld_run(const gf2* p, f2v* v) {
+    const gf4* q = reinterpret_cast<const gf4*>(p);
+#pragma unroll
+    for (int i = 0; i < n / 2; i++) {
+        const f4v t = q[i];
+        v[2 * i] = f2v{t.x, t.y};
+        v[2 * i + 1] = f2v{t.z, t.w};
+    }
+}
 typedef _Float16 hv8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 constexpr int kHP = 24;                       // LDS row pitch (halves): 16 k + pad
@@ -925,7 +938,11 @@ __device__ __forceinline__ void gemm_tile_r128(const rf_gemm_desc& g, int m0, in
 // (2-D: u_r[j][i] = Re sum_k Y_k[j] W_k[i]): one real-part GEMM with M the
 // last axis, N the plane and K the shells, whose B operand is formed while
 // staged -- one complex product per element -- instead of read.  Tables are
-// [r][k][index]; a 64x128 tile of the real part as gemm_tile_r128.
+// [r][index][k], k fastest and padded to whole kTK chunks by zero entries:
+// a thread's k run of a chunk is one 16-byte-aligned piece of a row, read
+// by 16-byte loads from one address, with no masks; rows and columns past a
+// tile's edge read the last row / column (finite values in accumulators
+// that are never stored).  A 64x128 tile of the real part as gemm_tile_r128.
 struct rf_iso_gemm {
     const float2* A;              // last-axis table [r][k][m]
     const float2* W;              // first-axis table with the coefficients [r][k][i]
@@ -940,7 +957,7 @@ struct rf_iso_gemm {
 __device__ __forceinline__ void gemm_tile_iso(const rf_iso_gemm& g, int m0, int n0, int batch, HStage<3>* st) {
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
-    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + m0);
+    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + (int64_t)m0 * g.K);
     const gf2* Wt = (const gf2*)(g.W + (int64_t)batch * g.wStrideB);
     const gf2* Yt = (const gf2*)(g.Y ? g.Y + (int64_t)batch * g.yStrideB : nullptr);
     f32x16 acc[2];
@@ -953,23 +970,21 @@ __device__ __forceinline__ void gemm_tile_iso(const rf_iso_gemm& g, int m0, int 
     // staging: A row t/4, k 4(t%4)..+3; B column t%128, k 8(t/128)..+7
     const int aRow = tid >> 2, aK = (tid & 3) * 4;
     const int bCol = tid & 127, bK = (tid >> 7) * 8;
-    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
-    const int bI = bIn ? (n0 + bCol) % g.nx : 0, bJ = bIn ? (n0 + bCol) / g.nx : 0;
+    const int aRowC = m0 + aRow < g.M ? aRow : g.M - 1 - m0;
+    const int bColC = n0 + bCol < g.N ? n0 + bCol : g.N - 1;
+    const int bI = bColC % g.nx, bJ = bColC / g.nx;
     const bool hasY = Yt != nullptr;
-    const int kLast = g.K - 1;
+    const gf2* aP = A + (int64_t)aRowC * g.K + aK;
+    const gf2* wP = Wt + (int64_t)bI * g.K + bK;
+    const gf2* yP = hasY ? Yt + (int64_t)bJ * g.K + bK : nullptr;
     f2v ra[4], rw[8], ry[8];
     auto fetch = [&](int k0) {
+        ld_run<4>(aP + k0, ra);
+        ld_run<8>(wP + k0, rw);
+        if (hasY) ld_run<8>(yP + k0, ry);
+        else {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int ka = k0 + aK + i;
-            ra[i] = (aIn && ka <= kLast) ? A[(int64_t)ka * g.M + aRow] : f2v{0.f, 0.f};
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int kb = k0 + bK + i;
-            const bool in = bIn && kb <= kLast;
-            rw[i] = in ? Wt[(int64_t)kb * g.nx + bI] : f2v{0.f, 0.f};
-            ry[i] = (in && hasY) ? Yt[(int64_t)kb * g.ny + bJ] : f2v{1.f, 0.f};
+            for (int i = 0; i < 8; i++) ry[i] = f2v{1.f, 0.f};
         }
     };
     auto stash = [&]() {
@@ -1069,7 +1084,7 @@ struct IsoStage128 {
 __device__ __forceinline__ void gemm_tile_iso128(const rf_iso_gemm& g, int m0, int n0, int batch, IsoStage128* st) {
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, half = lane >> 5, l32 = lane & 31;
-    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + m0);
+    const gf2* A = (const gf2*)(g.A + (int64_t)batch * g.aStrideB + (int64_t)m0 * g.K);
     const gf2* Wt = (const gf2*)(g.W + (int64_t)batch * g.wStrideB);
     const gf2* Yt = (const gf2*)(g.Y ? g.Y + (int64_t)batch * g.yStrideB : nullptr);
     f32x16 acc[2][2];
@@ -1081,23 +1096,21 @@ __device__ __forceinline__ void gemm_tile_iso128(const rf_iso_gemm& g, int m0, i
     // staging: A row t/2, k 8(t%2)..+7; B column t%128, k 8(t/128)..+7
     const int aRow = tid >> 1, aK = (tid & 1) * 8;
     const int bCol = tid & 127, bK = (tid >> 7) * 8;
-    const bool aIn = m0 + aRow < g.M, bIn = n0 + bCol < g.N;
-    const int bI = bIn ? (n0 + bCol) % g.nx : 0, bJ = bIn ? (n0 + bCol) / g.nx : 0;
+    const int aRowC = m0 + aRow < g.M ? aRow : g.M - 1 - m0;
+    const int bColC = n0 + bCol < g.N ? n0 + bCol : g.N - 1;
+    const int bI = bColC % g.nx, bJ = bColC / g.nx;
     const bool hasY = Yt != nullptr;
-    const int kLast = g.K - 1;
+    const gf2* aP = A + (int64_t)aRowC * g.K + aK;
+    const gf2* wP = Wt + (int64_t)bI * g.K + bK;
+    const gf2* yP = hasY ? Yt + (int64_t)bJ * g.K + bK : nullptr;
     f2v ra[8], rw[8], ry[8];
     auto fetch = [&](int k0) {
+        ld_run<8>(aP + k0, ra);
+        ld_run<8>(wP + k0, rw);
+        if (hasY) ld_run<8>(yP + k0, ry);
+        else {
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int ka = k0 + aK + i;
-            ra[i] = (aIn && ka <= kLast) ? A[(int64_t)ka * g.M + aRow] : f2v{0.f, 0.f};
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int kb = k0 + bK + i;
-            const bool in = bIn && kb <= kLast;
-            rw[i] = in ? Wt[(int64_t)kb * g.nx + bI] : f2v{0.f, 0.f};
-            ry[i] = (in && hasY) ? Yt[(int64_t)kb * g.ny + bJ] : f2v{1.f, 0.f};
+            for (int i = 0; i < 8; i++) ry[i] = f2v{1.f, 0.f};
         }
     };
     auto stash = [&]() {
@@ -1450,20 +1463,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         gemm_tile_iso(g, mI * TM, nI * 2 * kTN, batch, reinterpret_cast<HStage<3>*>(smem));
 }
 
-// Isotropic phase tables [r][k][index] for the axes of the grid, in FP64
+// Isotropic phase tables [r][index][k] for the axes of the grid, in FP64
 // then rounded: axis 0 carries the coefficient, W = c a e^{i(k_x x + phi)}
-// (c: ampMult times a power of two), the others e^{i k_a x}.
-__global__ void k_iso_tables(int nDim, int64_t kN, int Nmc, int n0, int n1, int n2, const double* __restrict__ kIso,
+// (c: ampMult times a power of two), the others e^{i k_a x}; k runs to kP
+// (kN rounded up to whole chunks), zero past kN.
+__global__ void k_iso_tables(int nDim, int64_t kN, int64_t kP, int Nmc, int n0, int n1, int n2,
+                             const double* __restrict__ kIso,
                              const double* __restrict__ amp, const double* __restrict__ phase, double coef,
                              double o0, double o1, double o2, double s0, double s1, double s2, double i0, double i1,
                              double i2, float2* __restrict__ T0, float2* __restrict__ T1, float2* __restrict__ T2) {
     const int nT = n0 + n1 + (nDim == 3 ? n2 : 0);
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (int64_t)Nmc * kN * nT) return;
-    const int t = (int)(e % nT);
-    const int64_t rk = e / nT;                       // r * kN + k
+    if (e >= (int64_t)Nmc * kP * nT) return;
+    const int64_t k = e % kP;                        // k fastest, as stored
+    const int64_t rt = e / kP;                       // r * nT + t
+    const int t = (int)(rt % nT);
+    const int64_t r = rt / nT;
+    const int64_t rk = r * kN + k;
     const int a = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
     const int idx = a == 0 ? t : (a == 1 ? t - n0 : t - n0 - n1);
+    const int na = a == 0 ? n0 : (a == 1 ? n1 : n2);
+    float2* T = a == 0 ? T0 : (a == 1 ? T1 : T2);
+    const int64_t dst = (r * na + idx) * kP + k;     // [r][idx][k]
+    if (k >= kN) {
+        T[dst] = make_float2(0.f, 0.f);
+        return;
+    }
     const double o = a == 0 ? o0 : (a == 1 ? o1 : o2);
     const double st = a == 0 ? s0 : (a == 1 ? s1 : s2);
     const double inv = a == 0 ? i0 : (a == 1 ? i1 : i2);
@@ -1473,9 +1498,7 @@ __global__ void k_iso_tables(int nDim, int64_t kN, int Nmc, int n0, int n1, int 
     double sn, cs;
     sincos(th, &sn, &cs);
     const double m = a == 0 ? amp[rk] * coef : 1.0;
-    const int na = a == 0 ? n0 : (a == 1 ? n1 : n2);
-    float2* T = a == 0 ? T0 : (a == 1 ? T1 : T2);
-    T[rk * na + idx] = make_float2((float)(m * cs), (float)(m * sn));
+    T[dst] = make_float2((float)(m * cs), (float)(m * sn));
 }
 
 // E[v][j][x] = exp(i s_v k_j x), k_j = j dk, x = (o + i step) / L, in FP64;
@@ -1869,10 +1892,12 @@ hipError_t rf_grid_run(const rf_grid_stages& st, hipStream_t s) {
     return hipSuccess;
 }
 
+static int64_t iso_k_padded(int64_t kN) { return (kN + kTK - 1) / kTK * kTK; }
+
 size_t rf_iso_grid_scratch_elems(int nDim, int64_t kN, int Nmc, const int32_t* n) {
     int64_t t = 0;
     for (int a = 0; a < nDim; a++) t += n[a];
-    return (size_t)(Nmc * kN * t);
+    return (size_t)(Nmc * iso_k_padded(kN) * t);
 }
 
 hipError_t rf_launch_iso_grid(const rf_iso_grid_args& a, hipStream_t s) {
@@ -1885,24 +1910,25 @@ hipError_t rf_launch_iso_grid(const rf_iso_grid_args& a, hipStream_t s) {
         sc = std::ldexp(1.0, -e);
     }
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.nDim == 3 ? a.n[2] : 1;
+    const int64_t kP = iso_k_padded(a.kN);
     float2* T0 = a.scratch;
-    float2* T1 = T0 + (int64_t)a.Nmc * a.kN * n0;
-    float2* T2 = T1 + (int64_t)a.Nmc * a.kN * n1;
-    const int64_t cnt = (int64_t)a.Nmc * a.kN * (n0 + n1 + (a.nDim == 3 ? n2 : 0));
-    hipLaunchKernelGGL(k_iso_tables, dim3(blocks_for(cnt)), dim3(256), 0, s, a.nDim, a.kN, a.Nmc, n0, n1, n2, a.kIso,
+    float2* T1 = T0 + (int64_t)a.Nmc * kP * n0;
+    float2* T2 = T1 + (int64_t)a.Nmc * kP * n1;
+    const int64_t cnt = (int64_t)a.Nmc * kP * (n0 + n1 + (a.nDim == 3 ? n2 : 0));
+    hipLaunchKernelGGL(k_iso_tables, dim3(blocks_for(cnt)), dim3(256), 0, s, a.nDim, a.kN, kP, a.Nmc, n0, n1, n2, a.kIso,
                        a.amp, a.phase, a.ampMult * sc, a.origin[0], a.origin[1], a.origin[2], a.step[0], a.step[1],
                        a.step[2], a.invL[0], a.invL[1], a.invL[2], T0, T1, T2);
     rf_iso_gemm g{};
-    g.K = (int)a.kN;
+    g.K = (int)kP;
     g.W = T0;
     g.nx = n0;
-    g.wStrideB = a.kN * n0;
+    g.wStrideB = kP * n0;
     if (a.nDim == 3) {
-        g.A = T2; g.M = n2; g.aStrideB = a.kN * n2;
-        g.Y = T1; g.ny = n1; g.yStrideB = a.kN * n1;
+        g.A = T2; g.M = n2; g.aStrideB = kP * n2;
+        g.Y = T1; g.ny = n1; g.yStrideB = kP * n1;
         g.N = n0 * n1;
     } else {
-        g.A = T1; g.M = n1; g.aStrideB = a.kN * n1;
+        g.A = T1; g.M = n1; g.aStrideB = kP * n1;
         g.Y = nullptr; g.ny = 1; g.yStrideB = 0;
         g.N = n0;
     }

@@ -40,7 +40,20 @@ CASES = [
     # a last axis longer than 64 runs on 128-row tiles: one full, one partial
     ((9, 7, 150), (0.3, 0.4, 0.1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), R.FIXED, 0, R.FOM_NONE, torch.float32),
     ((50, 130), (0.2, 0.1), (1.0, 0.0), (1.0, 0.9), R.REF_COMPAT, 0, R.FOM_LOGNORMAL, torch.float32),
+    # shell counts at the 16-shell chunk edges: 16 (one whole chunk, no
+    # padding) and 6 (one chunk, 10 zero shells)
+    ((11, 11, 11), (0.1, 0.1, 0.1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), R.FIXED, 0, R.FOM_NONE, torch.float32),
+    ((6, 6), (0.1, 0.1), (0.0, 0.5), (1.0, 1.0), R.FIXED, 0, R.FOM_NONE, torch.float32),
 ]
+
+
+@pytest.mark.parametrize("nstep,kn", [((11, 11, 11), 16), ((6, 6), 6)])
+def test_iso_grid_chunk_edge_cases_have_those_shells(nstep, kn):
+    """The two chunk-edge cases above keep the shell counts they are there for."""
+    d = len(nstep)
+    p = R.make_params(nDim=d, method=R.ISOTROPIC, mode=R.FIXED, form=R.FORM_FP32, Nmc=3, corrL=(1.0,) * d,
+                      xRange=[(n - 1) * 0.1 for n in nstep])
+    assert R.plan_spectrum(p).kNTotal == kn
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "n%s-mode%d-sh%d-m%d" % ("x".join(map(str, c[0])), c[4], c[5], c[6]))
